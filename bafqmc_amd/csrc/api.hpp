@@ -1,0 +1,366 @@
+// Host side of the C ABI (include/bafqmc.h), shared by the HIP build and the
+// CPU emulator build.  A back end provides the rt_* functions below.
+#pragma once
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bafqmc.h"
+#include "sweep.hpp"
+
+namespace bafq {
+
+enum KernelId { K_PREPARE = 0, K_SWEEP = 1, K_WARMUP = 2 };
+
+// back-end hooks (0 on success)
+int rt_malloc(void** p, size_t bytes);
+void rt_free(void* p);
+int rt_h2d(void* dst, const void* src, size_t bytes);
+int rt_d2h(void* dst, const void* src, size_t bytes);
+int rt_memset(void* dst, int v, size_t bytes);
+int rt_launch(KernelId k, const LaunchArgs& a, int grid, int block, size_t smem, double* ms);
+int rt_sync();
+const char* rt_error();
+int rt_block_threads();        // threads per workgroup for the sweep kernels
+int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
+int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
+
+constexpr double kZero = 1.0e-10;   // src/number_conserving/src/calc_basic.f90:5
+
+}  // namespace bafq
+
+struct bafq_handle {
+    std::string err;
+    std::vector<bafq::ModelDev> models;       // host copies with device pointers
+    std::vector<bafq::ChainDev> chains;
+    std::vector<int> chain_n, chain_ns, chain_lt;
+    int obs_stride = 0;
+    void* dev = nullptr;                      // one allocation for everything
+    size_t dev_bytes = 0;
+    bafq::ModelDev* d_models = nullptr;
+    bafq::ChainDev* d_chains = nullptr;
+    int* d_ids = nullptr;
+    int* d_seeds = nullptr;
+    double* d_phi = nullptr;
+    double* d_obs = nullptr;
+    double* d_stats = nullptr;
+    size_t phi_total = 0;
+    std::vector<size_t> phi_off;
+    size_t max_lds = 0;
+    double last_ms = 0.0;
+};
+
+namespace bafq {
+
+inline int fail(bafq_handle* h, const std::string& msg) {
+    if (h) h->err = msg;
+    return 1;
+}
+
+struct Bump {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    }
+};
+
+// Lattice helpers: x fastest, forward bonds (x+1,y), (x,y+1), (x-1,y+1)
+// (src/number_conserving/src/lattice.f90:36-48,95-106); K = (4pi/3, 0) phase
+// table exp(i K . a_imj) (lattice.f90:76-91, obser_equal.f90:123-125,150-164).
+inline void lattice_tables(int lx, int ly, std::vector<int>& bonds, std::vector<cd>& phase, int& has_k) {
+    const int ns = lx * ly;
+    auto inv = [&](int x, int y) { return (((y % ly) + ly) % ly) * lx + (((x % lx) + lx) % lx); };
+    bonds.assign(3 * ns, 0);
+    for (int y = 0; y < ly; ++y)
+        for (int x = 0; x < lx; ++x) {
+            const int i = inv(x, y);
+            bonds[3 * i + 0] = inv(x + 1, y);
+            bonds[3 * i + 1] = inv(x, y + 1);
+            bonds[3 * i + 2] = inv(x - 1, y + 1);
+        }
+    phase.assign(ns, cd{0.0, 0.0});
+    has_k = (lx % 3 == 0 && ly % 3 == 0) ? 1 : 0;
+    if (!has_k) return;
+    const double pi = std::acos(-1.0);
+    const int kx = 2 * lx / 3, ky = ly / 3;
+    const double b1x = 2.0 * pi, b1y = -2.0 * pi / std::sqrt(3.0);
+    const double b2x = 0.0, b2y = 4.0 * pi / std::sqrt(3.0);
+    const double kvx = kx * b1x / lx + ky * b2x / ly, kvy = kx * b1y / lx + ky * b2y / ly;
+    for (int r = 0; r < ns; ++r) {
+        const int x = r % lx, y = r / lx;
+        const double ax = x * 1.0 + y * 0.5, ay = y * std::sqrt(3.0) / 2.0;
+        const double kr = kvx * ax + kvy * ay;
+        phase[r] = cd{std::cos(kr), std::sin(kr)};
+    }
+}
+
+}  // namespace bafq
+
+extern "C" {
+
+int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chain_model, int n_chains,
+                bafq_handle** out) {
+    using namespace bafq;
+    *out = nullptr;
+    bafq_handle* h = new bafq_handle();
+    if (n_models <= 0 || n_chains <= 0) {
+        h->err = "n_models and n_chains must be positive";
+        *out = h;
+        return 1;
+    }
+    // ---- validate models (calc_basic.f90:61-62,100-104)
+    std::vector<std::vector<int>> bonds(n_models);
+    std::vector<std::vector<cd>> phases(n_models);
+    h->models.resize(n_models);
+    for (int k = 0; k < n_models; ++k) {
+        const bafq_model_desc& d = models[k];
+        ModelDev& m = h->models[k];
+        std::memset(&m, 0, sizeof(m));
+        if (d.kind != BAFQ_NUMBER_CONSERVING && d.kind != BAFQ_PAIRING) { *out = h; return fail(h, "unknown solver kind"); }
+        if (d.nlx < 1 || d.nly < 1 || d.ltrot < 1 || d.nwrap < 1) { *out = h; return fail(h, "lattice sizes must be positive"); }
+        if (d.ltrot % d.nwrap != 0) { *out = h; return fail(h, "Ltrot is not a multiple of Nwrap"); }
+        if (d.u1 < -kZero) { *out = h; return fail(h, "U1 must be non-negative (relative-density channel)"); }
+        if (d.u2 > kZero) { *out = h; return fail(h, "U2 must be non-positive (total-density channel)"); }
+        if (!(d.beta > 0.0)) { *out = h; return fail(h, "Beta must be positive"); }
+        if (!d.expk_p || !d.expk_m) { *out = h; return fail(h, "kinetic propagators missing"); }
+        m.kind = d.kind;
+        m.ns = d.nlx * d.nly;
+        m.n = d.kind == BAFQ_PAIRING ? 2 * m.ns : m.ns;
+        m.nlx = d.nlx; m.nly = d.nly; m.ltrot = d.ltrot; m.nwrap = d.nwrap; m.nst = d.ltrot / d.nwrap;
+        m.u1 = d.u1; m.u2 = d.u2; m.mu = d.mu; m.rdelta = d.kind == BAFQ_PAIRING ? d.rdelta : 0.0;
+        m.shift_loc = d.shift_loc;
+        const double dtau = d.beta / d.ltrot;
+        // alpha = sqrt(-2 U dtau): operator_Hubbard.f90:36-43 (NC), :62-88 (pairing)
+        m.alpha[0] = d.u1 > kZero ? cd{0.0, std::sqrt(2.0 * d.u1 * dtau)} : cd{0.0, 0.0};
+        m.alpha[1] = d.u2 < -kZero ? cd{std::sqrt(-2.0 * d.u2 * dtau), 0.0} : cd{0.0, 0.0};
+        m.active[0] = std::fabs(d.u1) > kZero;
+        m.active[1] = std::fabs(d.u2) > kZero;
+        lattice_tables(d.nlx, d.nly, bonds[k], phases[k], m.has_k);
+    }
+    // ---- layout of the device allocation
+    Bump b;
+    const size_t off_models = b.take(sizeof(ModelDev) * n_models);
+    std::vector<size_t> off_ep(n_models), off_em(n_models), off_ph(n_models), off_bd(n_models);
+    for (int k = 0; k < n_models; ++k) {
+        const int n = h->models[k].n, ns = h->models[k].ns;
+        off_ep[k] = b.take(sizeof(double) * n * n);
+        off_em[k] = b.take(sizeof(double) * n * n);
+        off_ph[k] = b.take(sizeof(cd) * ns);
+        off_bd[k] = b.take(sizeof(int) * 3 * ns);
+    }
+    const size_t off_chains = b.take(sizeof(ChainDev) * n_chains);
+    const size_t off_ids = b.take(sizeof(int) * n_chains);
+    const size_t off_seeds = b.take(sizeof(int) * n_chains);
+    int max_ns = 0;
+    for (int c = 0; c < n_chains; ++c) {
+        if (chain_model[c] < 0 || chain_model[c] >= n_models) { *out = h; return fail(h, "chain_model out of range"); }
+        max_ns = std::max(max_ns, h->models[chain_model[c]].ns);
+    }
+    h->obs_stride = O_SITE0 + max_ns;
+    const size_t off_obs = b.take(sizeof(double) * (size_t)h->obs_stride * n_chains);
+    const size_t off_stats = b.take(sizeof(double) * (size_t)S_LEN * n_chains);
+    h->phi_off.resize(n_chains);
+    h->phi_total = 0;
+    for (int c = 0; c < n_chains; ++c) {
+        const ModelDev& m = h->models[chain_model[c]];
+        h->phi_off[c] = h->phi_total;
+        h->phi_total += (size_t)m.ltrot * m.ns * 2;
+    }
+    const size_t off_phi = b.take(sizeof(double) * h->phi_total);
+    struct COff { size_t g, ur, vr, ul, vl, dr, dl, url, vrl, ull, vll, drl, dll, wa, w1, w2, w3; };
+    std::vector<COff> co(n_chains);
+    for (int c = 0; c < n_chains; ++c) {
+        const ModelDev& m = h->models[chain_model[c]];
+        const size_t mat = sizeof(cd) * (size_t)m.n * m.n, vec = sizeof(double) * m.n;
+        const size_t lists = (size_t)(m.nst + 1);
+        COff& o = co[c];
+        o.g = b.take(mat); o.ur = b.take(mat); o.vr = b.take(mat); o.ul = b.take(mat); o.vl = b.take(mat);
+        o.dr = b.take(vec); o.dl = b.take(vec);
+        o.url = b.take(mat * lists); o.vrl = b.take(mat * lists);
+        o.ull = b.take(mat * lists); o.vll = b.take(mat * lists);
+        o.drl = b.take(vec * lists); o.dll = b.take(vec * lists);
+        o.wa = b.take(mat); o.w1 = b.take(mat); o.w2 = b.take(mat); o.w3 = b.take(mat);
+    }
+    h->dev_bytes = b.off;
+    if (rt_malloc(&h->dev, h->dev_bytes)) { *out = h; return fail(h, std::string("device allocation failed: ") + rt_error()); }
+    char* base = (char*)h->dev;
+    if (rt_memset(h->dev, 0, h->dev_bytes)) { *out = h; return fail(h, rt_error()); }
+    // ---- model tables
+    for (int k = 0; k < n_models; ++k) {
+        ModelDev& m = h->models[k];
+        const size_t nn = (size_t)m.n * m.n;
+        rt_h2d(base + off_ep[k], models[k].expk_p, sizeof(double) * nn);
+        rt_h2d(base + off_em[k], models[k].expk_m, sizeof(double) * nn);
+        rt_h2d(base + off_ph[k], phases[k].data(), sizeof(cd) * m.ns);
+        rt_h2d(base + off_bd[k], bonds[k].data(), sizeof(int) * 3 * m.ns);
+        m.ep = (const double*)(base + off_ep[k]);
+        m.em = (const double*)(base + off_em[k]);
+        m.phasek = (const cd*)(base + off_ph[k]);
+        m.bonds = (const int*)(base + off_bd[k]);
+    }
+    h->d_models = (ModelDev*)(base + off_models);
+    h->d_chains = (ChainDev*)(base + off_chains);
+    h->d_ids = (int*)(base + off_ids);
+    h->d_seeds = (int*)(base + off_seeds);
+    h->d_obs = (double*)(base + off_obs);
+    h->d_stats = (double*)(base + off_stats);
+    h->d_phi = (double*)(base + off_phi);
+    rt_h2d(h->d_models, h->models.data(), sizeof(ModelDev) * n_models);
+    h->chains.resize(n_chains);
+    std::vector<int> ids(n_chains);
+    int nw = rt_block_threads() / 64;
+    for (int c = 0; c < n_chains; ++c) {
+        ChainDev& ch = h->chains[c];
+        const COff& o = co[c];
+        const ModelDev& m = h->models[chain_model[c]];
+        ch.model = chain_model[c];
+        ch.seed = h->d_seeds + c;
+        ch.phi = h->d_phi + h->phi_off[c];
+        ch.g = (cd*)(base + o.g); ch.ur = (cd*)(base + o.ur); ch.vr = (cd*)(base + o.vr);
+        ch.ul = (cd*)(base + o.ul); ch.vl = (cd*)(base + o.vl);
+        ch.dr = (double*)(base + o.dr); ch.dl = (double*)(base + o.dl);
+        ch.urlist = (cd*)(base + o.url); ch.vrlist = (cd*)(base + o.vrl);
+        ch.ullist = (cd*)(base + o.ull); ch.vllist = (cd*)(base + o.vll);
+        ch.drlist = (double*)(base + o.drl); ch.dllist = (double*)(base + o.dll);
+        ch.wa = (cd*)(base + o.wa); ch.w1 = (cd*)(base + o.w1); ch.w2 = (cd*)(base + o.w2); ch.w3 = (cd*)(base + o.w3);
+        ch.obs = h->d_obs + (size_t)h->obs_stride * c;
+        ch.stats = h->d_stats + (size_t)S_LEN * c;
+        ids[c] = c;
+        h->chain_n.push_back(m.n);
+        h->chain_ns.push_back(m.ns);
+        h->chain_lt.push_back(m.ltrot);
+        h->max_lds = std::max(h->max_lds, lds_bytes(m.n, nw));
+    }
+    rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * n_chains);
+    rt_h2d(h->d_ids, ids.data(), sizeof(int) * n_chains);
+    if (rt_sync()) { *out = h; return fail(h, rt_error()); }
+    *out = h;
+    return 0;
+}
+
+void bafq_destroy(bafq_handle* h) {
+    if (!h) return;
+    if (h->dev) bafq::rt_free(h->dev);
+    delete h;
+}
+
+const char* bafq_last_error(const bafq_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int bafq_dims(const bafq_handle* h, int chain, int32_t* n, int32_t* obs_stride, int32_t* stats_stride,
+              int32_t* n_fields) {
+    if (!h || chain < 0 || chain >= (int)h->chains.size()) return 1;
+    if (n) *n = h->chain_n[chain];
+    if (obs_stride) *obs_stride = h->obs_stride;
+    if (stats_stride) *stats_stride = bafq::S_LEN;
+    if (n_fields) *n_fields = h->chain_lt[chain] * h->chain_ns[chain] * 2;
+    return 0;
+}
+
+int bafq_set_fields(bafq_handle* h, const double* phi, const int32_t* seeds) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (rt_h2d(h->d_phi, phi, sizeof(double) * h->phi_total)) return fail(h, rt_error());
+    for (size_t c = 0; c < h->chains.size(); ++c)
+        if (seeds[c] <= 0) return fail(h, "RNG seeds must be positive");
+    if (rt_h2d(h->d_seeds, seeds, sizeof(int) * h->chains.size())) return fail(h, rt_error());
+    return rt_sync() ? fail(h, rt_error()) : 0;
+}
+
+int bafq_get_fields(bafq_handle* h, double* phi, int32_t* seeds) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (rt_sync()) return fail(h, rt_error());
+    if (phi && rt_d2h(phi, h->d_phi, sizeof(double) * h->phi_total)) return fail(h, rt_error());
+    if (seeds && rt_d2h(seeds, h->d_seeds, sizeof(int) * h->chains.size())) return fail(h, rt_error());
+    return 0;
+}
+
+static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure) {
+    using namespace bafq;
+    LaunchArgs a;
+    a.models = h->d_models;
+    a.chains = h->d_chains;
+    a.ids = h->d_ids;
+    a.count = (int)h->chains.size();
+    a.nsweep = nsweep;
+    a.measure = measure;
+    int grid, block;
+    size_t smem = 0;
+    if (k == K_WARMUP) {
+        block = 64;
+        grid = (a.count + block - 1) / block;
+    } else {
+        block = rt_block_threads();
+        grid = a.count;
+        smem = h->max_lds;
+    }
+    if (rt_launch(k, a, grid, block, smem, &h->last_ms)) return fail(h, rt_error());
+    if (rt_sync()) return fail(h, rt_error());
+    return 0;
+}
+
+int bafq_warmup(bafq_handle* h, int n) {
+    if (!h) return 1;
+    if (n <= 0) return 0;
+    return launch_all(h, bafq::K_WARMUP, n, 0);
+}
+
+int bafq_prepare(bafq_handle* h) {
+    if (!h) return 1;
+    return launch_all(h, bafq::K_PREPARE, 0, 0);
+}
+
+int bafq_sweep(bafq_handle* h, int n_sweeps, int measure) {
+    if (!h) return 1;
+    if (n_sweeps <= 0) return 0;
+    return launch_all(h, bafq::K_SWEEP, n_sweeps, measure);
+}
+
+int bafq_read_bin(bafq_handle* h, double* obs, double* stats) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (rt_sync()) return fail(h, rt_error());
+    const size_t nc = h->chains.size();
+    if (obs && rt_d2h(obs, h->d_obs, sizeof(double) * h->obs_stride * nc)) return fail(h, rt_error());
+    if (stats && rt_d2h(stats, h->d_stats, sizeof(double) * S_LEN * nc)) return fail(h, rt_error());
+    if (rt_memset(h->d_obs, 0, sizeof(double) * h->obs_stride * nc)) return fail(h, rt_error());
+    // keep the last log|det G| (S_LOGDET) and the running drift maximum; reset counters
+    std::vector<double> st(S_LEN * nc);
+    if (rt_d2h(st.data(), h->d_stats, sizeof(double) * S_LEN * nc)) return fail(h, rt_error());
+    for (size_t c = 0; c < nc; ++c)
+        for (int k = 0; k < S_LEN; ++k)
+            if (k != S_LOGDET && k != S_XMAX) st[c * S_LEN + k] = 0.0;
+    if (rt_h2d(h->d_stats, st.data(), sizeof(double) * S_LEN * nc)) return fail(h, rt_error());
+    return rt_sync() ? fail(h, rt_error()) : 0;
+}
+
+int bafq_get_green(bafq_handle* h, int chain, double* g) {
+    using namespace bafq;
+    if (!h || chain < 0 || chain >= (int)h->chains.size()) return 1;
+    if (rt_sync()) return fail(h, rt_error());
+    const int n = h->chain_n[chain];
+    if (rt_d2h(g, h->chains[chain].g, sizeof(cd) * n * n)) return fail(h, rt_error());
+    return 0;
+}
+
+int bafq_last_kernel_ms(const bafq_handle* h, double* ms) {
+    if (!h || !ms) return 1;
+    *ms = h->last_ms;
+    return 0;
+}
+
+int bafq_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c) {
+    return bafq::rt_debug_gemm(n, a, op_a, b, op_b, c);
+}
+
+int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
+    return bafq::rt_debug_qrp(n, a, tau, perm, d);
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// HIP back end: gfx950 kernels and the runtime hooks of the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <string>
+
+#include "api.hpp"
+
+namespace bafq {
+
+static thread_local std::string g_err;
+
+static int check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return 1;
+}
+
+__global__ void __launch_bounds__(512) k_prepare(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem};
+    body_prepare(c, a);
+}
+
+__global__ void __launch_bounds__(512) k_sweep(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem};
+    body_sweep(c, a);
+}
+
+__global__ void __launch_bounds__(64) k_warmup(LaunchArgs a) {
+    Ctx c{nullptr};
+    body_warmup(c, a);
+}
+
+__global__ void __launch_bounds__(512) k_debug_gemm(cd* C, const cd* A, int oa, const cd* B, int ob, int n) {
+    Ctx c{nullptr};
+    gemm_cc(c, C, A, oa != 0, B, ob != 0, n);
+}
+
+__global__ void __launch_bounds__(512) k_debug_qrp(cd* A, int n, cd* tau, int* perm, double* d) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem};
+    const Lds L = carve(smem, n, c.nwaves());
+    qrp(c, A, n, L.qr, L.dtmp);
+    for (int k = c.tid(); k < n; k += c.nthreads()) {
+        tau[k] = L.qr.tau[k];
+        perm[k] = L.qr.perm[k];
+        d[k] = L.dtmp[k];
+    }
+}
+
+int rt_malloc(void** p, size_t bytes) { return check(hipMalloc(p, bytes), "hipMalloc"); }
+void rt_free(void* p) { (void)hipFree(p); }
+int rt_h2d(void* dst, const void* src, size_t bytes) {
+    return check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+int rt_d2h(void* dst, const void* src, size_t bytes) {
+    return check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+int rt_memset(void* dst, int v, size_t bytes) { return check(hipMemset(dst, v, bytes), "hipMemset"); }
+int rt_sync() { return check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+const char* rt_error() { return g_err.c_str(); }
+
+int rt_block_threads() {
+    const char* s = std::getenv("BAFQ_BLOCK");
+    int b = s ? std::atoi(s) : 256;
+    if (b < 64 || b > 512 || (b % 64) != 0) b = 256;
+    return b;
+}
+
+int rt_launch(KernelId k, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
+    static hipEvent_t ev[2] = {nullptr, nullptr};
+    if (!ev[0]) {
+        if (check(hipEventCreate(&ev[0]), "hipEventCreate") || check(hipEventCreate(&ev[1]), "hipEventCreate"))
+            return 1;
+    }
+    if (smem > 65536 &&
+        check(hipFuncSetAttribute(k == K_PREPARE ? (const void*)k_prepare : (const void*)k_sweep,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
+              "hipFuncSetAttribute"))
+        return 1;
+    if (check(hipEventRecord(ev[0], 0), "hipEventRecord")) return 1;
+    switch (k) {
+    case K_PREPARE:
+        hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(block), smem, 0, a);
+        break;
+    case K_SWEEP:
+        hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(block), smem, 0, a);
+        break;
+    case K_WARMUP:
+        hipLaunchKernelGGL(k_warmup, dim3(grid), dim3(block), 0, 0, a);
+        break;
+    }
+    if (check(hipGetLastError(), "kernel launch")) return 1;
+    if (check(hipEventRecord(ev[1], 0), "hipEventRecord") || check(hipEventSynchronize(ev[1]), "hipEventSynchronize"))
+        return 1;
+    float f = 0.0f;
+    if (check(hipEventElapsedTime(&f, ev[0], ev[1]), "hipEventElapsedTime")) return 1;
+    if (ms) *ms = (double)f;
+    return 0;
+}
+
+int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c) {
+    if (n < 1 || n > 512) { g_err = "n out of range"; return 1; }
+    const size_t bytes = sizeof(cd) * (size_t)n * n;
+    cd *da = nullptr, *db = nullptr, *dc = nullptr;
+    if (rt_malloc((void**)&da, bytes) || rt_malloc((void**)&db, bytes) || rt_malloc((void**)&dc, bytes)) return 1;
+    int rc = rt_h2d(da, a, bytes) || rt_h2d(db, b, bytes);
+    if (!rc) {
+        hipLaunchKernelGGL(k_debug_gemm, dim3(1), dim3(rt_block_threads()), 0, 0, dc, da, op_a, db, op_b, n);
+        rc = check(hipGetLastError(), "k_debug_gemm") || rt_sync() || rt_d2h(c, dc, bytes);
+    }
+    rt_free(da); rt_free(db); rt_free(dc);
+    return rc;
+}
+
+int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
+    if (n < 1 || n > 512) { g_err = "n out of range"; return 1; }
+    const size_t bytes = sizeof(cd) * (size_t)n * n;
+    cd *da = nullptr, *dt = nullptr;
+    int* dp = nullptr;
+    double* dd = nullptr;
+    if (rt_malloc((void**)&da, bytes) || rt_malloc((void**)&dt, sizeof(cd) * n) ||
+        rt_malloc((void**)&dp, sizeof(int) * n) || rt_malloc((void**)&dd, sizeof(double) * n)) return 1;
+    int rc = rt_h2d(da, a, bytes);
+    if (!rc) {
+        const int block = rt_block_threads();
+        hipLaunchKernelGGL(k_debug_qrp, dim3(1), dim3(block), lds_bytes(n, block / 64), 0, da, n, dt, dp, dd);
+        rc = check(hipGetLastError(), "k_debug_qrp") || rt_sync() || rt_d2h(a, da, bytes) ||
+             rt_d2h(tau, dt, sizeof(cd) * n) || rt_d2h(perm, dp, sizeof(int) * n) || rt_d2h(d, dd, sizeof(double) * n);
+    }
+    rt_free(da); rt_free(dt); rt_free(dp); rt_free(dd);
+    return rc;
+}
+
+}  // namespace bafq

@@ -1,0 +1,123 @@
+// Execution-context layer for the workgroup-cooperative solver code.
+//
+// The solver's device code (la.hpp, sweep.hpp) is written against a small
+// context object `Ctx` that provides thread/wave indices, the workgroup
+// barrier, 64-lane wavefront shuffles and the FP64 MFMA 16x16x4 tile
+// operation.  Two back ends implement it:
+//
+//   * HipCtx  (default): CDNA4 / gfx950, compiled by hipcc.
+//   * EmuCtx  (-DBAFQ_EMU): a CPU emulator in which every GPU thread is an OS
+//     thread, barriers are real barriers and a wavefront's MFMA is evaluated
+//     from the 64 lanes' operands.  It exists so the same kernels can be run
+//     and bounds-checked on a machine without a GPU.
+//
+// MFMA f64 16x16x4 operand layout (CDNA3/CDNA4 ISA): lane l holds
+//   A[i][k] with i = l % 16, k = l / 16;  B[k][j] with j = l % 16, k = l / 16;
+//   D[i][j] in accumulator register r with i = 4 * (l / 16) + r, j = l % 16.
+#pragma once
+
+#include <cstdint>
+#include <cmath>
+
+#ifndef BAFQ_EMU
+#include <hip/hip_runtime.h>
+#define DEV __device__ __forceinline__
+#define DEVFN __device__
+#define HOSTDEV __host__ __device__ inline
+typedef double d4 __attribute__((ext_vector_type(4)));
+#else
+#include <atomic>
+#include <barrier>
+#include <cstring>
+#include <vector>
+#define DEV inline
+#define DEVFN inline
+#define HOSTDEV inline
+struct d4 {
+    double v[4];
+    double& operator[](int i) { return v[i]; }
+    double operator[](int i) const { return v[i]; }
+};
+#endif
+
+namespace bafq {
+
+constexpr int kWave = 64;
+
+#ifndef BAFQ_EMU
+struct HipCtx {
+    char* smem;
+    DEV int tid() const { return (int)threadIdx.x; }
+    DEV int nthreads() const { return (int)blockDim.x; }
+    DEV int lane() const { return (int)(threadIdx.x & 63); }
+    DEV int wave() const { return (int)(threadIdx.x >> 6); }
+    DEV int nwaves() const { return (int)(blockDim.x >> 6); }
+    DEV int block() const { return (int)blockIdx.x; }
+    DEV int nblocks() const { return (int)gridDim.x; }
+    DEV void sync() const { __syncthreads(); }
+    DEV double shfl_xor(double v, int mask) const { return __shfl_xor(v, mask, 64); }
+    DEV int shfl_xor_i(int v, int mask) const { return __shfl_xor(v, mask, 64); }
+    DEV d4 mfma(double a, double b, d4 c) const {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    DEV int atomic_add(int* p, int v) const { return atomicAdd(p, v); }
+};
+using Ctx = HipCtx;
+#else
+// ---------------------------------------------------------------- emulator
+struct EmuGroup {
+    int nthreads;
+    std::barrier<>* bar;
+    std::vector<std::barrier<>*> wbar;   // one barrier per wavefront
+    std::vector<double> wbuf;            // [wave][3][64] exchange buffers
+    char* smem;
+    int block, nblocks;
+};
+
+struct EmuCtx {
+    EmuGroup* g;
+    int t;
+    char* smem;
+    int tid() const { return t; }
+    int nthreads() const { return g->nthreads; }
+    int lane() const { return t & 63; }
+    int wave() const { return t >> 6; }
+    int nwaves() const { return g->nthreads >> 6; }
+    int block() const { return g->block; }
+    int nblocks() const { return g->nblocks; }
+    void sync() const { g->bar->arrive_and_wait(); }
+    void wsync() const { g->wbar[wave()]->arrive_and_wait(); }
+    double* wb(int slot) const { return &g->wbuf[(size_t)(wave() * 3 + slot) * 64]; }
+    double shfl_xor(double v, int mask) const {
+        double* b = wb(0);
+        b[lane()] = v;
+        wsync();
+        double r = b[lane() ^ mask];
+        wsync();
+        return r;
+    }
+    int shfl_xor_i(int v, int mask) const { return (int)shfl_xor((double)v, mask); }
+    d4 mfma(double a, double b, d4 c) const {
+        double* A = wb(0);
+        double* B = wb(1);
+        A[lane()] = a;
+        B[lane()] = b;
+        wsync();
+        const int l = lane(), j = l & 15;
+        for (int r = 0; r < 4; ++r) {
+            const int i = 4 * (l >> 4) + r;
+            double s = c[r];
+            for (int k = 0; k < 4; ++k) s += A[i + 16 * k] * B[j + 16 * k];
+            c[r] = s;
+        }
+        wsync();
+        return c;
+    }
+    int atomic_add(int* p, int v) const {
+        return reinterpret_cast<std::atomic<int>*>(p)->fetch_add(v);
+    }
+};
+using Ctx = EmuCtx;
+#endif
+
+}  // namespace bafq

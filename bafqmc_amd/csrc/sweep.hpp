@@ -1,0 +1,792 @@
+// The Markov-chain sweep of one chain, executed by one workgroup.
+//
+// Restates src/number_conserving/src/local_sweep.f90:55-161 (identical
+// orchestration in src/pairing/src/local_sweep.f90) for both solvers in one
+// "reduced" form:
+//   * number conserving: N = Ns, one row per site, the c flavour is the
+//     complex conjugate of the b flavour (src/number_conserving/src/obser_equal.f90:113-117);
+//   * Nambu pairing: the reference's 4Ns matrices are block diagonal on the
+//     sectors {b, c+} and {c, b+}, and for real t, real Delta, imaginary
+//     alpha_1 and real alpha_2 the second block is the complex conjugate of
+//     the first.  The solver propagates only the first block, N = 2 Ns, rows
+//     (b_1..b_Ns, c+_1..c+_Ns); det P_4x4 = |det P_2x2|^2.
+// HS diagonal factors of site i and channel nf are exp(s * alpha_nf * phi)
+// with s = +1 on every row except the c+ row of the total-density channel
+// (src/pairing/src/operator_Hubbard.f90:155-199).
+#pragma once
+
+#include "la.hpp"
+
+namespace bafq {
+
+// ---------------------------------------------------------------- layouts
+enum ObsIndex {
+    O_DENSITY_UP = 0, O_DENSITY_DO, O_DENSITY_TOTAL, O_DENSITY, O_KINETIC, O_DOUBLEOCC,
+    O_SQUAREOCC, O_LOCAL_NUMSQUARE, O_PAIR_EQUAL, O_NUM_UP, O_NUM_DO, O_NUMSQUARE_UP,
+    O_NUMSQUARE_DO, O_ONSITE_N2_UP, O_ONSITE_N2_DO, O_INTERACTION, O_PAIRING_E, O_CHEMICAL_E,
+    O_GRAND_E, O_ENERGY, O_SF_K_RE, O_SF_K_IM, O_PSF_RE, O_PSF_IM, O_DW_K_RE, O_DW_K_IM,
+    O_DEN_UPUP_RE, O_DEN_UPUP_IM, O_DEN_DODO_RE, O_DEN_DODO_IM, O_DEN_UPDO_RE, O_DEN_UPDO_IM,
+    O_NOBS, O_SITE0 = 40
+};
+enum StatIndex {
+    S_PROP_U1 = 0, S_ACC_U1, S_PROP_U2, S_ACC_U2, S_XMAX, S_XMEAN_SUM, S_NWRAP, S_LOGDET,
+    S_THERM_PROP_U1, S_THERM_ACC_U1, S_THERM_PROP_U2, S_THERM_ACC_U2, S_LEN = 16
+};
+
+struct ModelDev {
+    int kind;            // 0 number conserving, 1 Nambu pairing
+    int n, ns, nlx, nly, ltrot, nwrap, nst, has_k;
+    int active[2];
+    double u1, u2, mu, rdelta, shift_loc;
+    cd alpha[2];
+    const double* ep;    // e^{-dtau A}, n x n real row-major
+    const double* em;    // e^{+dtau A}
+    const cd* phasek;    // exp(i K . r_imj) indexed by imj
+    const int* bonds;    // ns x 3 forward bonds
+};
+
+struct ChainDev {
+    int model;
+    int* seed;
+    double* phi;                       // ltrot x ns x 2
+    cd *g, *ur, *vr, *ul, *vl;
+    double *dr, *dl;
+    cd *urlist, *vrlist, *ullist, *vllist;
+    double *drlist, *dllist;
+    cd *wa, *w1, *w2, *w3;
+    double* obs;
+    double* stats;
+};
+
+struct LaunchArgs {
+    const ModelDev* models;
+    const ChainDev* chains;
+    const int* ids;
+    int count;
+    int nsweep;
+    int measure;
+};
+
+// ---------------------------------------------------------------- RNG
+// seed <- 48828125 * seed mod 2^31, u = seed / 2^31 (src/common/random.f90:3-11).
+DEV double ranf(int& seed) {
+    const int64_t s = (48828125LL * (int64_t)seed) % 2147483648LL;
+    seed = (int)s;
+    return (double)s / 2147483648.0;
+}
+
+// ---------------------------------------------------------------- LDS plan
+struct Lds {
+    QrScratch qr;
+    double* dtmp;      // n
+    cd* f;             // n
+    cd* finv;          // n
+    cd* col;           // 2 x n
+    cd* row;           // 2 x n
+    double* red;       // nwaves x 64
+    cd* site;          // 6 x ns (pairing measurement)
+};
+
+HOSTDEV size_t lds_bytes(int n, int nwaves) {
+    size_t b = 0;
+    b += (size_t)n * sizeof(cd);        // tau
+    b += (size_t)n * sizeof(int) + 16;  // perm (padded)
+    b += (size_t)n * sizeof(double);    // nrm
+    b += 16 * sizeof(double);           // bcast
+    b += (size_t)n * sizeof(double);    // dtmp
+    b += 2 * (size_t)n * sizeof(cd);    // f, finv
+    b += 4 * (size_t)n * sizeof(cd);    // col, row
+    b += (size_t)nwaves * 64 * sizeof(double);
+    b += 6 * (size_t)n * sizeof(cd);
+    return b + 256;
+}
+
+DEV Lds carve(char* base, int n, int nwaves) {
+    Lds L;
+    char* p = base;
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += (bytes + 15) & ~(size_t)15;
+        return q;
+    };
+    L.qr.tau = (cd*)take((size_t)n * sizeof(cd));
+    L.qr.perm = (int*)take((size_t)n * sizeof(int) + 16);
+    L.qr.nrm = (double*)take((size_t)n * sizeof(double));
+    L.qr.bcast = (double*)take(16 * sizeof(double));
+    L.dtmp = (double*)take((size_t)n * sizeof(double));
+    L.f = (cd*)take((size_t)n * sizeof(cd));
+    L.finv = (cd*)take((size_t)n * sizeof(cd));
+    L.col = (cd*)take(2 * (size_t)n * sizeof(cd));
+    L.row = (cd*)take(2 * (size_t)n * sizeof(cd));
+    L.red = (double*)take((size_t)nwaves * 64 * sizeof(double));
+    L.site = (cd*)take(6 * (size_t)n * sizeof(cd));
+    return L;
+}
+
+// ---------------------------------------------------------------- helpers
+// Sum NV per-thread values over the workgroup; thread v < NV receives total v.
+template <int NV>
+DEV void wg_reduce(const Ctx& c, double (&v)[NV], double* red, double* out_add) {
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(c, v[k]);
+    if (c.lane() == 0)
+        for (int k = 0; k < NV; ++k) red[c.wave() * NV + k] = v[k];
+    c.sync();
+    if (c.tid() < NV) {
+        double s = 0.0;
+        for (int w = 0; w < c.nwaves(); ++w) s += red[w * NV + c.tid()];
+        out_add[c.tid()] += s;
+    }
+    c.sync();
+}
+
+// Row index of sector `sec` of site i.
+DEV int row_of(const ModelDev& m, int i, int sec) { return sec * m.ns + i; }
+DEV int nsec(const ModelDev& m) { return m.kind == 1 ? 2 : 1; }
+// Exponent sign of sector `sec` in channel nf (1: relative U1, 2: total U2).
+DEV double sec_sign(const ModelDev& m, int nf, int sec) {
+    return (m.kind == 1 && nf == 2 && sec == 1) ? -1.0 : 1.0;
+}
+
+// f[r] = exp(s_r alpha_nf phi_r), finv = 1 / f, for all rows.
+DEV void hs_factors(const Ctx& c, const ModelDev& m, const Lds& L, const double* phi_t, int nf) {
+    const int ns = m.ns, k = nsec(m);
+    for (int r = c.tid(); r < ns * k; r += c.nthreads()) {
+        const int sec = r / ns, i = r - sec * ns;
+        const double s = sec_sign(m, nf, sec);
+        const double ph = phi_t[i * 2 + (nf - 1)];
+        const cd a = m.alpha[nf - 1];
+        L.f[r] = cexp(cd{s * a.r * ph, s * a.i * ph});
+        L.finv[r] = cexp(cd{-s * a.r * ph, -s * a.i * ph});
+    }
+    c.sync();
+}
+
+// M <- diag(fl) M diag(fr), with fl/fr chosen from f/finv (or none).
+DEV void scale_mat(const Ctx& c, cd* M, int n, const cd* fl, const cd* fr) {
+    const int nn = n * n;
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int r = e / n, k = e - r * n;
+        cd x = M[e];
+        if (fl) x = fl[r] * x;
+        if (fr) x = x * fr[k];
+        M[e] = x;
+    }
+    c.sync();
+}
+
+// ---------------------------------------------------------------- Metropolis
+struct Decision {
+    bool acc;
+    double phi_new;
+    cd m[2][2];   // update kernel: G -= G[:,S] m (I_S - G[S,:])
+};
+
+// LocalU_metro: NC src/number_conserving/src/localU.f90:31-77 and
+// operator_Hubbard.f90:45-68; pairing src/pairing/src/localU.f90:33-113 and
+// src/pairing/src/operator_Hubbard.f90:90-137, in the reduced 2x2 form.
+DEV Decision metro(const ModelDev& m, int nf, double phi_old, int& seed, const cd (&gss)[2][2]) {
+    Decision d;
+    const double xflip = ranf(seed);
+    const double phi_new = phi_old + (xflip - 0.5) * fabs(m.shift_loc);
+    d.phi_new = phi_new;
+    const double rg = exp(-0.5 * phi_new * phi_new) / exp(-0.5 * phi_old * phi_old);
+    const cd a = m.alpha[nf - 1];
+    double ratio_abs;
+    if (m.kind == 0) {
+        const cd eo = cexp(cd{a.r * phi_old, a.i * phi_old});
+        const cd en = cexp(cd{a.r * phi_new, a.i * phi_new});
+        const cd delta = cdiv(en, eo) - cd{1.0, 0.0};
+        const cd den = cd{1.0, 0.0} + delta * (cd{1.0, 0.0} - gss[0][0]);
+        const cd rdet = cinv(den);
+        const cd z = (cd{rg, 0.0} * rdet) * conj(rdet);
+        ratio_abs = cabs(z);
+        d.m[0][0] = rdet * delta;
+        d.m[0][1] = d.m[1][0] = d.m[1][1] = cd{0.0, 0.0};
+    } else {
+        const double dphi = phi_new - phi_old;
+        const cd ep = cexp(cd{a.r * dphi, a.i * dphi});
+        const cd em = cexp(cd{-a.r * dphi, -a.i * dphi});
+        const cd d0 = ep - cd{1.0, 0.0};
+        const cd d1 = (nf == 2 ? em : ep) - cd{1.0, 0.0};
+        double rconst = 1.0;
+        if (nf == 2) {
+            const cd eo = cexp(cd{a.r * phi_old, a.i * phi_old});
+            const cd en = cexp(cd{a.r * phi_new, a.i * phi_new});
+            rconst = cdiv(eo, en).r;
+        }
+        const cd p00 = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - gss[0][0]);
+        const cd p01 = -(d0 * gss[0][1]);
+        const cd p10 = -(d1 * gss[1][0]);
+        const cd p11 = cd{1.0, 0.0} + d1 * (cd{1.0, 0.0} - gss[1][1]);
+        const cd det = p00 * p11 - p01 * p10;
+        ratio_abs = fabs(rg * rconst / cabs(det));
+        const cd idet = cinv(det);
+        d.m[0][0] = (p11 * idet) * d0;
+        d.m[0][1] = (-(p01 * idet)) * d1;
+        d.m[1][0] = (-(p10 * idet)) * d0;
+        d.m[1][1] = (p00 * idet) * d1;
+    }
+    const double u = ranf(seed);
+    d.acc = ratio_abs > u;
+    return d;
+}
+
+// Sequential local updates of one channel on time slice t (0-based), in
+// ascending (sweep right) or descending (sweep left) site order.  G must
+// already be in the frame in which the reference evaluates the ratio
+// (all sites' HS factors of this channel applied for sweep right; none for
+// sweep left) -- the per-site similarity transforms commute with the rank-k
+// updates and leave G_SS unchanged.
+DEV void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G,
+                   int t, int nf, bool ascending, int& seed) {
+    const int n = m.n, ns = m.ns, k = nsec(m);
+    double* phi_t = ch.phi + (size_t)t * ns * 2;
+    for (int q = 0; q < ns; ++q) {
+        const int i = ascending ? q : ns - 1 - q;
+        int S[2] = {i, ns + i};
+        cd gss[2][2];
+        for (int a = 0; a < k; ++a)
+            for (int b = 0; b < k; ++b) gss[a][b] = G[S[a] * n + S[b]];
+        if (k == 1) gss[0][1] = gss[1][0] = gss[1][1] = cd{0.0, 0.0};
+        const double phi_old = phi_t[i * 2 + nf - 1];
+        const Decision d = metro(m, nf, phi_old, seed, gss);
+        if (!d.acc) {
+            if (c.tid() == 0) ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += 1.0;
+            continue;
+        }
+        // snapshot column(s) and row(s) of the site
+        for (int r = c.tid(); r < n; r += c.nthreads())
+            for (int a = 0; a < k; ++a) {
+                L.col[a * n + r] = G[r * n + S[a]];
+                L.row[a * n + r] = G[S[a] * n + r];
+            }
+        c.sync();
+        // every thread has read phi_old and G_SS before this point
+        if (c.tid() == 0) {
+            ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += 1.0;
+            ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += 1.0;
+            phi_t[i * 2 + nf - 1] = d.phi_new;
+        }
+        const int nn = n * n;
+        for (int e = c.tid(); e < nn; e += c.nthreads()) {
+            const int r = e / n, col = e - r * n;
+            cd x = G[e];
+            for (int a = 0; a < k; ++a) {
+                cd w = cd{0.0, 0.0};
+                for (int b = 0; b < k; ++b) {
+                    cd rv = -L.row[b * n + col];
+                    if (col == S[b]) rv = rv + cd{1.0, 0.0};
+                    w = w + d.m[a][b] * rv;
+                }
+                x = x - L.col[a * n + r] * w;
+            }
+            G[e] = x;
+        }
+        c.sync();
+    }
+}
+
+// ---------------------------------------------------------------- stabilization
+// stab_UR (src/number_conserving/src/stabilization.f90:69-93).
+DEV void stab_ur(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+    const int n = m.n;
+    cd* A = ch.wa;
+    const int nn = n * n;
+    for (int e = c.tid(); e < nn; e += c.nthreads()) A[e] = ch.ur[e] * cd{ch.dr[e % n], 0.0};
+    c.sync();
+    qrp(c, A, n, L.qr, L.dtmp);
+    trmm_perm_rows(c, ch.w1, A, ch.vr, L.qr.perm, n);
+    mat_copy(c, ch.vr, ch.w1, n, false);
+    form_q(c, ch.ur, A, L.qr.tau, n);
+    vec_copy(c, ch.dr, L.dtmp, n);
+}
+
+// stab_UL (stabilization.f90:95-119).
+DEV void stab_ul(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+    const int n = m.n;
+    cd* A = ch.wa;
+    const int nn = n * n;
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int r = e / n, k = e - r * n;
+        A[e] = conj(ch.ul[k * n + r]) * cd{ch.dl[k], 0.0};
+    }
+    c.sync();
+    qrp(c, A, n, L.qr, L.dtmp);
+    perm_cols_times_vh(c, ch.w1, ch.vl, A, L.qr.perm, n);
+    mat_copy(c, ch.vl, ch.w1, n, false);
+    form_q(c, ch.w1, A, L.qr.tau, n);
+    mat_copy(c, ch.ul, ch.w1, n, true);
+    vec_copy(c, ch.dl, L.dtmp, n);
+}
+
+// stab_green (stabilization.f90:121-190): Gout = (1 - UR DR VR VL DL UL)^{-1}.
+// For nt > Ltrot/2 the reference decomposes M^H; then G^H = UR P V^{-1} D^{-1} Q^H UL
+// has the same structure as the nt <= Ltrot/2 branch with (UL^H, UR^H) -> (UR, UL).
+// Returns log|det G| = -sum log D.
+DEV double stab_green(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt,
+                      const cd* ur, const double* dr, const cd* vr, const cd* ul, const double* dl,
+                      const cd* vl, cd* Gout) {
+    const int n = m.n, nn = n * n;
+    const bool upper = nt > m.ltrot / 2;
+    cd* A = ch.wa;
+    gemm_cc(c, ch.w1, vr, false, vl, false, n);          // VR VL
+    gemm_cc(c, ch.w2, ur, true, ul, true, n);            // UR^H UL^H
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int r = e / n, k = e - r * n;
+        if (!upper) {
+            A[e] = ch.w2[e] - cd{dr[r] * dl[k], 0.0} * ch.w1[e];
+        } else {
+            const int t = k * n + r;
+            A[e] = conj(ch.w2[t] - cd{dr[k] * dl[r], 0.0} * ch.w1[t]);
+        }
+    }
+    c.sync();
+    qrp(c, A, n, L.qr, L.dtmp);
+    // X = Q^H Rt, Rt = UR^H (lower branch) or UL (upper branch)
+    mat_copy(c, ch.w1, upper ? ul : ur, n, !upper);
+    apply_qh_left(c, ch.w1, A, L.qr.tau, n);
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int r = e / n;
+        ch.w1[e] = ch.w1[e] * cd{1.0 / L.dtmp[r], 0.0};
+    }
+    c.sync();
+    trsm_upper_left(c, ch.w1, A, n);
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int r = e / n, k = e - r * n;
+        ch.w2[L.qr.perm[r] * n + k] = ch.w1[e];
+    }
+    c.sync();
+    // G' = Lf Y, Lf = UL^H (lower) or UR (upper); G = G' or G'^H
+    if (!upper) {
+        gemm_cc(c, Gout, ul, true, ch.w2, false, n);
+    } else {
+        gemm_cc(c, ch.w3, ur, false, ch.w2, false, n);
+        mat_copy(c, Gout, ch.w3, n, true);
+    }
+    double s = 0.0;
+    for (int k = c.lane(); k < n; k += 64) s += log(L.dtmp[k]);
+    s = wave_sum(c, s);
+    return -s;
+}
+
+DEV void store_drift_and_logdet(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L,
+                                const cd* Gnew, double logdet) {
+    const int nn = m.n * m.n;
+    double v[1] = {0.0};
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const cd d = Gnew[e] - ch.g[e];
+        v[0] += cabs(d);
+    }
+    v[0] = wave_sum(c, v[0]);
+    if (c.lane() == 0) L.red[c.wave()] = v[0];
+    c.sync();
+    if (c.tid() == 0) {
+        double s = 0.0;
+        for (int w = 0; w < c.nwaves(); ++w) s += L.red[w];
+        if (s > ch.stats[S_XMAX]) ch.stats[S_XMAX] = s;
+        ch.stats[S_XMEAN_SUM] += s;
+        ch.stats[S_NWRAP] += 1.0;
+        ch.stats[S_LOGDET] = logdet;
+    }
+    c.sync();
+}
+
+DEV cd* list_mat(cd* base, int st, int n) { return base + (size_t)st * n * n; }
+DEV double* list_vec(double* base, int st, int n) { return base + (size_t)st * n; }
+
+// Wrap_R (stabilization.f90:330-366), nt % Nwrap == 0, nt != 0.
+DEV void wrap_r(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
+    const int n = m.n, st = nt / m.nwrap;
+    stab_ur(c, m, ch, L);
+    const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, list_mat(ch.ullist, st, n),
+                                 list_vec(ch.dllist, st, n), list_mat(ch.vllist, st, n), ch.wa);
+    store_drift_and_logdet(c, m, ch, L, ch.wa, ld);
+    mat_copy(c, ch.g, ch.wa, n, false);
+    mat_copy(c, list_mat(ch.urlist, st, n), ch.ur, n, false);
+    mat_copy(c, list_mat(ch.vrlist, st, n), ch.vr, n, false);
+    vec_copy(c, list_vec(ch.drlist, st, n), ch.dr, n);
+}
+
+// Wrap_L (stabilization.f90:292-328), nt % Nwrap == 0, nt != Ltrot.
+DEV void wrap_l(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
+    const int n = m.n, st = nt / m.nwrap;
+    stab_ul(c, m, ch, L);
+    const double ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
+                                 list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, ch.wa);
+    store_drift_and_logdet(c, m, ch, L, ch.wa, ld);
+    mat_copy(c, ch.g, ch.wa, n, false);
+    mat_copy(c, list_mat(ch.ullist, st, n), ch.ul, n, false);
+    mat_copy(c, list_mat(ch.vllist, st, n), ch.vl, n, false);
+    vec_copy(c, list_vec(ch.dllist, st, n), ch.dl, n);
+}
+
+DEV void live_identity(const Ctx& c, int n, cd* u, double* d, cd* v) {
+    mat_identity(c, u, n);
+    mat_identity(c, v, n);
+    vec_fill(c, d, n, 1.0);
+}
+
+// ---------------------------------------------------------------- measurement
+DEV int imj_index(const ModelDev& m, int i, int j) {
+    const int xi = i % m.nlx, yi = i / m.nlx, xj = j % m.nlx, yj = j / m.nlx;
+    int dx = xi - xj, dy = yi - yj;
+    if (dx < 0) dx += m.nlx;
+    if (dy < 0) dy += m.nly;
+    return dy * m.nlx + dx;
+}
+
+// NC estimator, src/number_conserving/src/obser_equal.f90:95-198.
+DEV void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+    const int n = m.n, nn = n * n;
+    const cd* G = ch.g;
+    const double lq = (double)m.ns, norm = 1.0 / (lq * lq);
+    enum { NV = 22 };
+    double v[NV];
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    // v: 0 dens_up 1 dbl 2 sq 3 n2u 4 n2d 5 nsq_up 6 nsq_do 7-8 psf 9-10 sf 11-12 dw
+    //    13-14 den_upup 15-16 den_dodo 17-18 den_updo 19 kinetic
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int i = e / n, j = e - i * n;
+        const cd gij = G[e], gji = G[j * n + i];
+        const cd du_i = G[i * n + i] - cd{1.0, 0.0}, du_j = G[j * n + j] - cd{1.0, 0.0};
+        const cd dd_i = conj(G[i * n + i]) - cd{1.0, 0.0}, dd_j = conj(G[j * n + j]) - cd{1.0, 0.0};
+        const double dl = (i == j) ? 1.0 : 0.0;
+        const cd gupc = gji - cd{dl, 0.0};           // Grupc(i,j)
+        const cd gdoc = conj(gji) - cd{dl, 0.0};     // Grdoc(i,j)
+        const cd cu = du_i * du_j + gupc * gij;
+        const cd cdn = dd_i * dd_j + gdoc * conj(gij);
+        const cd cud = du_i * dd_j;
+        v[5] += cu.r;
+        v[6] += cdn.r;
+        const cd psf = gupc * gdoc;
+        v[7] += psf.r; v[8] += psf.i;
+        if (m.has_k) {
+            const cd ph = m.phasek[imj_index(m, i, j)];
+            const cd sf = ph * (gupc + gdoc);
+            v[9] += sf.r; v[10] += sf.i;
+            const cd dw = ph * (cu + cdn + cud + dd_i * du_j);
+            v[11] += dw.r; v[12] += dw.i;
+        }
+        v[13] += cu.r; v[14] += cu.i;
+        v[15] += cdn.r; v[16] += cdn.i;
+        v[17] += cud.r; v[18] += cud.i;
+        if (i == j) {
+            v[0] += du_i.r;
+            v[1] += (du_i * dd_i).r;
+            v[2] += (du_i * du_i + dd_i * dd_i).r;
+            v[3] += (cd{2.0, 0.0} * du_i * du_i + du_i).r;
+            v[4] += (cd{2.0, 0.0} * dd_i * dd_i + dd_i).r;
+            ch.obs[O_SITE0 + i] += (du_i + dd_i).r;
+        }
+    }
+    for (int e = c.tid(); e < m.ns * 3; e += c.nthreads()) {
+        const int i = e / 3, j = m.bonds[e];
+        const cd gij = G[i * n + j], gji = G[j * n + i];
+        // Grupc(i,j)+Grupc(j,i)+Grdoc(i,j)+Grdoc(j,i), i != j
+        v[19] += (gji + gij + conj(gji) + conj(gij)).r / lq;
+    }
+    double tot[NV];
+    for (int k = 0; k < NV; ++k) tot[k] = v[k];
+    for (int k = 0; k < NV; ++k) tot[k] = wave_sum(c, tot[k]);
+    if (c.lane() == 0)
+        for (int k = 0; k < NV; ++k) L.red[c.wave() * NV + k] = tot[k];
+    c.sync();
+    if (c.tid() == 0) {
+        double s[NV];
+        for (int k = 0; k < NV; ++k) {
+            s[k] = 0.0;
+            for (int w = 0; w < c.nwaves(); ++w) s[k] += L.red[w * NV + k];
+        }
+        double* o = ch.obs;
+        const double dens = s[0] / lq;
+        const double dbl = s[1] / lq;
+        const double inter = (m.u1 + m.u2) * (s[3] + s[4]) / lq + 2.0 * (m.u2 - m.u1) * dbl;
+        o[O_DENSITY_UP] += dens;
+        o[O_DENSITY_DO] += dens;
+        o[O_DENSITY_TOTAL] += 2.0 * dens;
+        o[O_DOUBLEOCC] += dbl;
+        o[O_SQUAREOCC] += s[2] / lq;
+        o[O_NUM_UP] += s[0];
+        o[O_NUM_DO] += s[0];
+        o[O_NUMSQUARE_UP] += s[5];
+        o[O_NUMSQUARE_DO] += s[6];
+        o[O_ONSITE_N2_UP] += s[3];
+        o[O_ONSITE_N2_DO] += s[4];
+        o[O_PSF_RE] += s[7] * norm; o[O_PSF_IM] += s[8] * norm;
+        o[O_SF_K_RE] += s[9] * norm; o[O_SF_K_IM] += s[10] * norm;
+        o[O_DW_K_RE] += s[11] * norm; o[O_DW_K_IM] += s[12] * norm;
+        o[O_DEN_UPUP_RE] += s[13] * norm; o[O_DEN_UPUP_IM] += s[14] * norm;
+        o[O_DEN_DODO_RE] += s[15] * norm; o[O_DEN_DODO_IM] += s[16] * norm;
+        o[O_DEN_UPDO_RE] += s[17] * norm; o[O_DEN_UPDO_IM] += s[18] * norm;
+        o[O_KINETIC] += s[19];
+        o[O_INTERACTION] += inter;
+        o[O_ENERGY] += s[19] + inter;
+        o[O_NOBS] += 1.0;
+    }
+    c.sync();
+}
+
+// Pairing estimator, src/pairing/src/obser_equal.f90:112-271, written in the
+// reduced block X (basis b, c+) with the 4Ns blocks
+//   G_b_b+ = Xbb, G_b_c = -Xbc, G_c+_b+ = Xcb, G_c+_c = -Xcc,
+//   G_c_c+ = conj Xbb, G_c_b = -conj Xbc, G_b+_c+ = conj Xcb, G_b+_b = -conj Xcc,
+// and the remaining eight blocks zero.
+DEV void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+    const int n = m.n, ns = m.ns;
+    const cd* G = ch.g;
+    const double nsd = (double)ns, norm = 1.0 / (nsd * nsd);
+    auto X = [&](int bi, int bj, int i, int j) { return G[(bi * ns + i) * n + bj * ns + j]; };
+    // per-site: nb, nc, Xbb_ii, Xbc_ii, Xcb_ii
+    cd* nb = L.site;
+    cd* ncv = L.site + ns;
+    cd* xbb = L.site + 2 * ns;
+    cd* xbc = L.site + 3 * ns;
+    cd* xcb = L.site + 4 * ns;
+    for (int i = c.tid(); i < ns; i += c.nthreads()) {
+        const cd xcc = X(1, 1, i, i);
+        nb[i] = -conj(xcc);
+        ncv[i] = -xcc;
+        xbb[i] = X(0, 0, i, i);
+        xbc[i] = X(0, 1, i, i);
+        xcb[i] = X(1, 0, i, i);
+    }
+    c.sync();
+    enum { NV = 24 };
+    double v[NV];
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    // v: 0 nb 1 nc 2 Dbc1 3 sq 4 Dbb1 5 Dcc1 6 pair 7 nsq_up 8 nsq_do 9-10 sf 11-12 dw
+    //    13-14 psf 15-16 den_upup 17-18 den_dodo 19-20 den_updo 21 kinetic
+    const int np = ns * ns;
+    for (int e = c.tid(); e < np; e += c.nthreads()) {
+        const int i = e / ns, j = e - i * ns;
+        const cd Xbb = X(0, 0, i, j), Xbc = X(0, 1, i, j), Xcb = X(1, 0, i, j), Xcc = X(1, 1, i, j);
+        const cd Dbb = nb[i] * nb[j] - conj(Xcc) * Xbb;
+        const cd Dcc = ncv[i] * ncv[j] - Xcc * conj(Xbb);
+        const cd Dbc = nb[i] * ncv[j] - conj(Xcb) * Xbc;
+        const cd Dcb = ncv[i] * nb[j] - Xcb * conj(Xbc);
+        v[7] += Dbb.r;
+        v[8] += Dcc.r;
+        v[15] += Dbb.r; v[16] += Dbb.i;
+        v[17] += Dcc.r; v[18] += Dcc.i;
+        v[19] += Dbc.r; v[20] += Dbc.i;
+        if (m.has_k) {
+            const cd ph = m.phasek[imj_index(m, i, j)];
+            const cd sf = ph * (-(conj(Xcc) + Xcc));
+            v[9] += sf.r; v[10] += sf.i;
+            const cd dw = ph * (Dbb + Dcc + Dbc + Dcb);
+            v[11] += dw.r; v[12] += dw.i;
+        }
+        const cd psf = -(conj(xcb[i]) * conj(xbc[j])) + conj(Xcc) * Xcc;
+        v[13] += psf.r; v[14] += psf.i;
+        if (i == j) {
+            const cd Dbb1 = nb[i] * nb[i] + nb[i] * xbb[i];
+            const cd Dcc1 = ncv[i] * ncv[i] + ncv[i] * conj(xbb[i]);
+            const cd Dbc1 = nb[i] * ncv[i] - conj(xcb[i]) * xbc[i];
+            v[0] += nb[i].r;
+            v[1] += ncv[i].r;
+            v[2] += Dbc1.r;
+            v[3] += (nb[i] * nb[i] + ncv[i] * ncv[i]).r;
+            v[4] += Dbb1.r;
+            v[5] += Dcc1.r;
+            v[6] += (-xbc[i] + conj(xcb[i])).r;
+            ch.obs[O_SITE0 + i] += (nb[i] + ncv[i]).r;
+        }
+    }
+    for (int e = c.tid(); e < ns * 3; e += c.nthreads()) {
+        const int i = e / 3, j = m.bonds[e];
+        // G_b+_b(i,j)+G_b+_b(j,i)+G_c+_c(i,j)+G_c+_c(j,i) = -2 Re(Xcc_ij + Xcc_ji)
+        v[21] += -2.0 * (X(1, 1, i, j) + X(1, 1, j, i)).r / nsd;
+    }
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(c, v[k]);
+    if (c.lane() == 0)
+        for (int k = 0; k < NV; ++k) L.red[c.wave() * NV + k] = v[k];
+    c.sync();
+    if (c.tid() == 0) {
+        double s[NV];
+        for (int k = 0; k < NV; ++k) {
+            s[k] = 0.0;
+            for (int w = 0; w < c.nwaves(); ++w) s[k] += L.red[w * NV + k];
+        }
+        double* o = ch.obs;
+        const double dens = (s[0] + s[1]) / nsd;
+        const double dbl = s[2] / nsd;
+        const double pair = s[6] / nsd;
+        const double inter = (m.u1 + m.u2) * (s[4] + s[5]) / nsd + 2.0 * (m.u2 - m.u1) * dbl;
+        const double pe = m.rdelta * pair;
+        const double che = -m.mu * dens;
+        const double kin = s[21];
+        o[O_DENSITY_UP] += s[0] / nsd;
+        o[O_DENSITY_DO] += s[1] / nsd;
+        o[O_DENSITY] += dens;
+        o[O_DENSITY_TOTAL] += dens;
+        o[O_NUM_UP] += s[0];
+        o[O_NUM_DO] += s[1];
+        o[O_DOUBLEOCC] += dbl;
+        o[O_SQUAREOCC] += s[3] / nsd;
+        o[O_LOCAL_NUMSQUARE] += (s[4] + s[5]) / nsd;
+        o[O_PAIR_EQUAL] += pair;
+        o[O_NUMSQUARE_UP] += s[7];
+        o[O_NUMSQUARE_DO] += s[8];
+        o[O_SF_K_RE] += s[9] * norm; o[O_SF_K_IM] += s[10] * norm;
+        o[O_DW_K_RE] += s[11] * norm; o[O_DW_K_IM] += s[12] * norm;
+        o[O_PSF_RE] += s[13] * norm; o[O_PSF_IM] += s[14] * norm;
+        o[O_DEN_UPUP_RE] += s[15] * norm; o[O_DEN_UPUP_IM] += s[16] * norm;
+        o[O_DEN_DODO_RE] += s[17] * norm; o[O_DEN_DODO_IM] += s[18] * norm;
+        o[O_DEN_UPDO_RE] += s[19] * norm; o[O_DEN_UPDO_IM] += s[20] * norm;
+        o[O_KINETIC] += kin;
+        o[O_ONSITE_N2_UP] += s[4];
+        o[O_ONSITE_N2_DO] += s[5];
+        o[O_INTERACTION] += inter;
+        o[O_PAIRING_E] += pe;
+        o[O_CHEMICAL_E] += che;
+        o[O_ENERGY] += kin + inter + pe;
+        o[O_GRAND_E] += kin + inter + pe + che;
+        o[O_NOBS] += 1.0;
+    }
+    c.sync();
+}
+
+DEV void measure(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+    if (m.kind == 0) measure_nc(c, m, ch, L);
+    else measure_pairing(c, m, ch, L);
+}
+
+// ---------------------------------------------------------------- sweeps
+// Local_sweep_pre (local_sweep.f90:80-94): UR = B(beta, 0) with checkpoints,
+// G(beta) from stab_green at nt = Ltrot.
+DEV void prepare_chain(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+    const int n = m.n;
+    live_identity(c, n, ch.ur, ch.dr, ch.vr);
+    live_identity(c, n, ch.ul, ch.dl, ch.vl);
+    live_identity(c, n, list_mat(ch.urlist, 0, n), list_vec(ch.drlist, 0, n), list_mat(ch.vrlist, 0, n));
+    live_identity(c, n, list_mat(ch.ullist, m.nst, n), list_vec(ch.dllist, m.nst, n),
+                  list_mat(ch.vllist, m.nst, n));
+    for (int nt = 1; nt <= m.ltrot; ++nt) {
+        const double* phi_t = ch.phi + (size_t)(nt - 1) * m.ns * 2;
+        for (int nf = 1; nf <= 2; ++nf) {
+            if (!m.active[nf - 1]) continue;
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, ch.ur, n, L.f, nullptr);
+        }
+        gemm_rc(c, ch.w1, m.ep, ch.ur, 0, n);
+        mat_copy(c, ch.ur, ch.w1, n, false);
+        if (nt % m.nwrap == 0) {
+            const int st = nt / m.nwrap;
+            stab_ur(c, m, ch, L);
+            mat_copy(c, list_mat(ch.urlist, st, n), ch.ur, n, false);
+            mat_copy(c, list_mat(ch.vrlist, st, n), ch.vr, n, false);
+            vec_copy(c, list_vec(ch.drlist, st, n), ch.dr, n);
+            if (nt == m.ltrot) {
+                const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, ch.ul, ch.dl, ch.vl, ch.g);
+                if (c.tid() == 0) ch.stats[S_LOGDET] = ld;
+                c.sync();
+            }
+        }
+    }
+}
+
+DEV void sweep_left(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed, bool meas) {
+    const int n = m.n;
+    live_identity(c, n, ch.ul, ch.dl, ch.vl);     // Wrap_L(Ltrot): L list slot Nst is the identity
+    for (int nt = m.ltrot; nt >= 1; --nt) {
+        if (nt % m.nwrap == 0 && nt != m.ltrot) wrap_l(c, m, ch, L, nt);
+        if (meas) measure(c, m, ch, L);
+        // propT_L: G <- e^{+dtau K} G e^{-dtau K}, UL <- UL e^{-dtau K}
+        gemm_rc(c, ch.w1, m.ep, ch.g, 1, n);
+        gemm_rc(c, ch.g, m.em, ch.w1, 0, n);
+        gemm_rc(c, ch.w1, m.ep, ch.ul, 1, n);
+        mat_copy(c, ch.ul, ch.w1, n, false);
+        const int t = nt - 1;
+        const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
+        for (int nf = 2; nf >= 1; --nf) {
+            if (!m.active[nf - 1]) continue;
+            site_loop(c, m, ch, L, ch.g, t, nf, false, seed);
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, ch.g, n, L.finv, L.f);      // G <- e^{-V} G e^{V}
+            scale_mat(c, ch.ul, n, nullptr, L.f);    // UL <- UL e^{V}
+        }
+    }
+    // Wrap_L(0)
+    wrap_l(c, m, ch, L, 0);
+}
+
+DEV void sweep_right(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed, bool meas) {
+    const int n = m.n;
+    live_identity(c, n, ch.ur, ch.dr, ch.vr);     // Wrap_R(0): R list slot 0 is the identity
+    for (int nt = 1; nt <= m.ltrot; ++nt) {
+        const int t = nt - 1;
+        const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
+        for (int nf = 1; nf <= 2; ++nf) {
+            if (!m.active[nf - 1]) continue;
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, ch.g, n, L.f, L.finv);      // G <- e^{V} G e^{-V} (old fields)
+            site_loop(c, m, ch, L, ch.g, t, nf, true, seed);
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, ch.ur, n, L.f, nullptr);    // UR <- e^{V} UR (new fields)
+        }
+        // propT_R: G <- e^{-dtau K} G e^{+dtau K}, UR <- e^{-dtau K} UR
+        gemm_rc(c, ch.w1, m.ep, ch.g, 0, n);
+        gemm_rc(c, ch.g, m.em, ch.w1, 1, n);
+        gemm_rc(c, ch.w1, m.ep, ch.ur, 0, n);
+        mat_copy(c, ch.ur, ch.w1, n, false);
+        if (nt % m.nwrap == 0) wrap_r(c, m, ch, L, nt);
+        if (meas) measure(c, m, ch, L);
+    }
+}
+
+// ---------------------------------------------------------------- kernel bodies
+DEV void body_prepare(const Ctx& c, const LaunchArgs& a) {
+    for (int q = c.block(); q < a.count; q += c.nblocks()) {
+        const ChainDev& ch = a.chains[a.ids[q]];
+        const ModelDev& m = a.models[ch.model];
+        const Lds L = carve(c.smem, m.n, c.nwaves());
+        prepare_chain(c, m, ch, L);
+    }
+}
+
+DEV void body_sweep(const Ctx& c, const LaunchArgs& a) {
+    for (int q = c.block(); q < a.count; q += c.nblocks()) {
+        const ChainDev& ch = a.chains[a.ids[q]];
+        const ModelDev& m = a.models[ch.model];
+        const Lds L = carve(c.smem, m.n, c.nwaves());
+        int seed = *ch.seed;
+        for (int s = 0; s < a.nsweep; ++s) {
+            sweep_left(c, m, ch, L, seed, a.measure != 0);
+            sweep_right(c, m, ch, L, seed, a.measure != 0);
+        }
+        c.sync();
+        if (c.tid() == 0) *ch.seed = seed;
+        c.sync();
+    }
+}
+
+// Local_sweep_therm (local_sweep.f90:64-78, localU.f90:115-156): Gaussian
+// weight only, one thread per chain.
+DEV void body_warmup(const Ctx& c, const LaunchArgs& a) {
+    const int q = c.block() * c.nthreads() + c.tid();
+    if (q >= a.count) return;
+    const ChainDev& ch = a.chains[a.ids[q]];
+    const ModelDev& m = a.models[ch.model];
+    int seed = *ch.seed;
+    for (int s = 0; s < a.nsweep; ++s)
+        for (int t = 0; t < m.ltrot; ++t)
+            for (int nf = 1; nf <= 2; ++nf) {
+                if (!m.active[nf - 1]) continue;
+                double* phi_t = ch.phi + (size_t)t * m.ns * 2;
+                for (int i = 0; i < m.ns; ++i) {
+                    const double po = phi_t[i * 2 + nf - 1];
+                    const double xflip = ranf(seed);
+                    const double pn = po + (xflip - 0.5) * fabs(m.shift_loc);
+                    const double rg = exp(-0.5 * pn * pn) / exp(-0.5 * po * po);
+                    const double u = ranf(seed);
+                    ch.stats[nf == 1 ? S_THERM_PROP_U1 : S_THERM_PROP_U2] += 1.0;
+                    if (fabs(rg) > u) {
+                        phi_t[i * 2 + nf - 1] = pn;
+                        ch.stats[nf == 1 ? S_THERM_ACC_U1 : S_THERM_ACC_U2] += 1.0;
+                    }
+                }
+            }
+    *ch.seed = seed;
+}
+
+}  // namespace bafq

@@ -1,0 +1,108 @@
+"""ctypes binding of the C ABI declared in ``include/bafqmc.h``.
+
+The HIP library ``_lib/libbafqmc.so`` is the product.  The emulator build
+``_lib/libbafqmc_emu.so`` runs the same kernels on CPU threads and is selected
+only explicitly (``load_library(emulator=True)`` or ``BAFQ_EMULATOR=1``), for
+the CPU test suite.  There is no other fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .build import HIP_LIB, EMU_LIB
+
+c_double_p = C.POINTER(C.c_double)
+c_int32_p = C.POINTER(C.c_int32)
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("nlx", C.c_int32), ("nly", C.c_int32),
+        ("ltrot", C.c_int32), ("nwrap", C.c_int32),
+        ("beta", C.c_double), ("u1", C.c_double), ("u2", C.c_double), ("mu", C.c_double),
+        ("rdelta", C.c_double), ("shift_loc", C.c_double),
+        ("expk_p", c_double_p), ("expk_m", c_double_p),
+    ]
+
+
+SYMBOLS = {
+    "bafq_create": (C.c_int, [C.POINTER(ModelDesc), C.c_int, c_int32_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "bafq_destroy": (None, [C.c_void_p]),
+    "bafq_last_error": (C.c_char_p, [C.c_void_p]),
+    "bafq_dims": (C.c_int, [C.c_void_p, C.c_int, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),
+    "bafq_set_fields": (C.c_int, [C.c_void_p, c_double_p, c_int32_p]),
+    "bafq_get_fields": (C.c_int, [C.c_void_p, c_double_p, c_int32_p]),
+    "bafq_warmup": (C.c_int, [C.c_void_p, C.c_int]),
+    "bafq_prepare": (C.c_int, [C.c_void_p]),
+    "bafq_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "bafq_read_bin": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    "bafq_get_green": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),
+    "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),
+    "bafq_debug_qrp": (C.c_int, [C.c_int, c_double_p, c_double_p, c_int32_p, c_double_p]),
+}
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(c_double_p)
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(c_int32_p)
+
+
+class NativeLibrary:
+    def __init__(self, path: Path):
+        path = Path(path)
+        if not path.exists():
+            raise FileNotFoundError(
+                f"native library {path} is missing: build it with `python -m bafqmc_amd.build`")
+        self.path = path
+        self.emulator = path.name == EMU_LIB.name
+        self.lib = C.CDLL(str(path))
+        for name, (res, args) in SYMBOLS.items():
+            fn = getattr(self.lib, name)
+            fn.restype = res
+            fn.argtypes = args
+
+    def __getattr__(self, name):
+        return getattr(self.lib, name)
+
+    def debug_gemm(self, a, b, op_a=0, op_b=0):
+        n = a.shape[0]
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        c = np.zeros((n, n), np.complex128)
+        rc = self.lib.bafq_debug_gemm(n, dptr(a.view(np.float64)), op_a, dptr(b.view(np.float64)), op_b,
+                                      dptr(c.view(np.float64)))
+        if rc:
+            raise RuntimeError("bafq_debug_gemm failed")
+        return c
+
+    def debug_qrp(self, a):
+        n = a.shape[0]
+        a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+        tau = np.zeros(n, np.complex128)
+        perm = np.zeros(n, np.int32)
+        d = np.zeros(n)
+        rc = self.lib.bafq_debug_qrp(n, dptr(a.view(np.float64)), dptr(tau.view(np.float64)), iptr(perm), dptr(d))
+        if rc:
+            raise RuntimeError("bafq_debug_qrp failed")
+        return a, tau, perm, d
+
+
+_CACHE = {}
+
+
+def load_library(emulator: bool | None = None) -> NativeLibrary:
+    """Load the HIP library (default) or, when asked explicitly, the CPU emulator."""
+    if emulator is None:
+        emulator = os.environ.get("BAFQ_EMULATOR", "0") == "1"
+    path = EMU_LIB if emulator else HIP_LIB
+    if path not in _CACHE:
+        _CACHE[path] = NativeLibrary(path)
+    return _CACHE[path]

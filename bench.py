@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Throughput of the batched BAFQMC sweep on one MI355X.
+
+Workload (default): 512 independent Nambu-pairing chains on a 6x6 triangular
+lattice (reduced Green matrices N = 2 Ns = 72), t=1, U1=1, U2=-0.5, Delta=0.5,
+beta=8, mu=-5, dtau=0.1 (Ltrot=80), Nwrap=10, shiftLoc=1.0, fields i.i.d. N(0,1)
+from per-chain seeds.  One step = one batched sweep (sweep_L + sweep_R of every
+chain, 2 Ltrot measurements each) followed by reading the bin.
+
+Metric: effective independent S_SF(K) samples/s = sum over chains of
+(bins/s) / (2 tau_int), tau_int measured on the per-chain Re S_SF(K) bin series.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 is not supported (one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+CONFIGS = {
+    "c2": dict(name="pairing 6x6 beta=8 Ltrot=80 U1=1 U2=-0.5 Delta=0.5, 512 chains",
+               p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=80, beta=8.0,
+                      nwrap=10, shift_loc=1.0), chains=512),
+    "c4": dict(name="number-conserving 9x9 beta=8 Ltrot=80 U1=1 U2=0, 256 chains",
+               p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=80, beta=8.0, nwrap=10,
+                      shift_loc=1.5), chains=256),
+    "c1": dict(name="number-conserving 3x3 beta=4 Ltrot=40 U1=1 U2=0, 512 chains",
+               p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=3, nly=3, ltrot=40, beta=4.0, nwrap=10,
+                      shift_loc=1.5), chains=512),
+}
+
+
+def flops_per_chain_sweep(n, ns, ltrot, nwrap, nch, c_meas=40.0):
+    """Algorithmic FP64 flops of one chain-sweep (complex MAC = 8 flops):
+    3 kinetic GEMMs per slice step (24 N^3), Ns rank-k Sherman-Morrison updates
+    per channel (8 N^3), 56 N^3 per stabilisation, measurement c_meas Ns^2."""
+    return (2 * ltrot * (24 + 8 * nch) * n ** 3 + 2 * (ltrot / nwrap) * 56 * n ** 3
+            + 2 * ltrot * c_meas * ns ** 2)
+
+
+def make_params(d):
+    from bafqmc_amd.model import Params
+    p = Params(pairing=d["pairing"], u1=d["u1"], u2=d["u2"], mu=d["mu"], rdelta=d.get("rdelta", 0.0),
+               nlx=d["nlx"], nly=d["nly"], ltrot=d["ltrot"], beta=d["beta"], nwrap=d["nwrap"],
+               shift_loc=d["shift_loc"], ini_type=2, ini_ampl=1.0)
+    p.nlx_therm, p.nly_therm, p.ltrot_therm = p.nlx, p.nly, p.ltrot
+    return p.validate()
+
+
+def cpu_baseline(cfg, tau, budget_s=12.0):
+    """The reference Fortran solver (oracle/_ref) on the host cores, bounded sample."""
+    d = cfg["p"]
+    cores = max(1, min(16, os.cpu_count() or 1))
+    try:
+        from oracle import run_ref
+        if not run_ref.available(d["pairing"]):
+            raise FileNotFoundError("reference binary not built")
+        from bafqmc_amd.analysis import seeds_for
+        # calibrate with one bin on one rank, then size the bounded run
+        pd = dict(d, nbin=1, nsweep=1, ini_type=2, ini_ampl=1.0)
+        dd, w1 = run_ref.run(pd, seeds_for(4242, 1))
+        run_ref.cleanup(dd)
+        nbin = max(1, int(budget_s / max(w1, 1e-3)))
+        pd["nbin"] = nbin
+        dd, wall = run_ref.run(pd, seeds_for(4242, cores), nranks=cores, timeout=600)
+        run_ref.cleanup(dd)
+        rate = cores * nbin / wall
+        return {"value": rate / (2.0 * tau), "unit": "effective S_SF(K) samples/s",
+                "cores": cores, "kind": "reference",
+                "sample": (f"reference Fortran (amdflang+MKL sequential) mpirun -np {cores}, one chain per rank, "
+                           f"{nbin} bins x 1 sweep of the same workload in {wall:.1f} s = {rate:.3f} chain-sweeps/s; "
+                           f"divided by 2*tau_int={2 * tau:.2f} from the GPU run (same Markov chain)")}
+    except Exception as e:  # noqa: BLE001 - fall back to the NumPy port of the oracle
+        from oracle import bafq_oracle as O
+        p = O.Params(**{k: d[k] for k in ("pairing", "u1", "u2", "mu", "nlx", "nly", "ltrot", "beta", "nwrap",
+                                             "shift_loc")}, rdelta=d.get("rdelta", 0.0))
+        rng = np.random.default_rng(1)
+        ch = O.Chain(p, rng.normal(size=(p.ltrot, p.nlx * p.nly, 2)), 12345)
+        t0 = time.perf_counter()
+        ch.pre()
+        n = 0
+        while time.perf_counter() - t0 < budget_s:
+            ch.bin()
+            n += 1
+        wall = time.perf_counter() - t0
+        rate = n / wall
+        return {"value": rate / (2.0 * tau), "unit": "effective S_SF(K) samples/s", "cores": 1, "kind": "port",
+                "sample": f"NumPy oracle, 1 chain, {n} sweeps in {wall:.1f} s ({rate:.3f} chain-sweeps/s); "
+                          f"reference binary unavailable: {e}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--chains", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X "
+              "(chains shard across GPUs by construction, but no multi-GPU path is built)", file=sys.stderr)
+        sys.exit(2)
+
+    from bafqmc_amd.model import ModelSpec, conf_set_batch
+    from bafqmc_amd.solver import BatchedSolver
+    from bafqmc_amd.analysis import seeds_for, tau_int
+
+    cfg = CONFIGS[args.config]
+    nchains = args.chains or cfg["chains"]
+    p = make_params(cfg["p"])
+    spec = ModelSpec(p)
+    phis, seeds = conf_set_batch(p, seeds_for(13579113, nchains))
+    s = BatchedSolver([spec], [0] * nchains)
+    s.set_fields(list(phis), seeds)
+    s.prepare()
+    for _ in range(args.warmup):
+        s.sweep(1)
+        s.read_bin()
+    sf = np.zeros((nchains, args.steps))
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.sweep(1)
+        kernel_ms.append(s.last_kernel_ms())
+        obs, _ = s.read_bin_raw()
+        sf[:, k] = obs[:, 20] / obs[:, 32]
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    tau, window, _ = tau_int(sf)
+    sweeps_per_s = nchains * args.steps / wall
+    value = nchains * (args.steps / wall) / (2.0 * tau)
+    nch = int(abs(p.u1) > 1e-10) + int(abs(p.u2) > 1e-10)
+    f_chain = flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch)
+    avg_ms = float(np.mean(kernel_ms))
+    achieved = f_chain * nchains / (avg_ms * 1e-3) / 1e12
+    out = {
+        "metric": "effective S_SF(K) samples/sec, plain batched sweep, 1 MI355X",
+        "value": value,
+        "unit": "effective S_SF(K) samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (HS fields i.i.d. N(0,1) from per-chain seeds)",
+        "config": {"workload": cfg["name"], "chains": nchains, "matrix_n": p.n, "ltrot": p.ltrot,
+                   "nwrap": p.nwrap, "sweeps_per_s": sweeps_per_s, "tau_int_sf_k": tau,
+                   "tau_window": window, "kernel": "k_sweep"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel_ms": avg_ms, "flops_per_launch": f_chain * nchains},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, tau)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

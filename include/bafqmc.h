@@ -1,0 +1,103 @@
+/*
+ * bafqmc.h -- C ABI of the MI355X solver for the BAFQMC Markov-chain sweep.
+ *
+ * The reference (YinkaiYu/BAFQMC) has no library API: its sweep is reached by
+ * running the executable `mpirun -np R bosonDQMC.out` in a run directory
+ * (src/number_conserving/run_paper.py:113,162; src/pairing/run_paper.py:100-101),
+ * one MPI rank per Markov chain.  Each entry point below replaces one step of
+ * that executable's main program (src/number_conserving/src/main.f90:22-83,
+ * identical structure in src/pairing/src/main.f90) for a whole batch of
+ * chains at once; the Python host (bafqmc_amd) keeps the file contract.
+ *
+ * Conventions: plain pointers and sizes, host memory on every pointer
+ * argument, row-major matrices, complex numbers as interleaved (re, im)
+ * doubles.  Every call returns 0 on success and a nonzero code otherwise, with
+ * a message from bafq_last_error().  One handle = one batch of chains on one
+ * GPU stream; calls on one handle must not overlap; distinct handles are
+ * independent.
+ */
+#ifndef BAFQMC_H
+#define BAFQMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bafq_handle bafq_handle;
+
+enum { BAFQ_NUMBER_CONSERVING = 0, BAFQ_PAIRING = 1 };
+
+/* One Hamiltonian + discretisation shared by a group of chains.
+ * Replaces read_input/Params_set (src/number_conserving/src/calc_basic.f90:48-106;
+ * pairing adds RDelta, src/pairing/src/calc_basic.f90:51) and the kinetic
+ * propagators of opT_set (src/number_conserving/src/non_interact.f90:74-109;
+ * src/pairing/src/non_interact.f90:121-178).  The solver works in the reduced
+ * form: N = Nlx*Nly (number conserving) or N = 2*Nlx*Nly (pairing, basis
+ * b_1..b_Ns, c+_1..c+_Ns).  expk_p / expk_m are the real N x N matrices
+ * exp(-dtau A) and exp(+dtau A) in that basis, built by the host. */
+typedef struct bafq_model_desc {
+    int32_t kind;
+    int32_t nlx, nly, ltrot, nwrap;
+    double beta, u1, u2, mu, rdelta, shift_loc;
+    const double* expk_p;
+    const double* expk_m;
+} bafq_model_desc;
+
+/* Model_init + Prop%make + WrList%make + Stabilize_init for every chain
+ * (src/number_conserving/src/main.f90:29-35).  chain_model[c] selects the
+ * model of chain c. */
+int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chain_model,
+                int n_chains, bafq_handle** out);
+void bafq_destroy(bafq_handle* h);
+const char* bafq_last_error(const bafq_handle* h);
+
+/* Sizes: matrix dimension N of a chain, doubles per chain in bafq_read_bin's
+ * obs block, doubles per chain in its stats block, fields per chain. */
+int bafq_dims(const bafq_handle* h, int chain, int32_t* n, int32_t* obs_stride,
+              int32_t* stats_stride, int32_t* n_fields);
+
+/* conf_in / conf_out (src/number_conserving/src/fields.f90:35-113,236-278):
+ * fields phi(ns, site, nt) in Fortran order, i.e. [nt][site][ns] per chain,
+ * concatenated over chains; one RNG seed per chain. */
+int bafq_set_fields(bafq_handle* h, const double* phi, const int32_t* seeds);
+int bafq_get_fields(bafq_handle* h, double* phi, int32_t* seeds);
+
+/* n calls of Local_sweep_therm (src/number_conserving/src/local_sweep.f90:64-78). */
+int bafq_warmup(bafq_handle* h, int n);
+
+/* Local_sweep_pre (local_sweep.f90:80-94): B(beta,0) checkpoints and G(beta). */
+int bafq_prepare(bafq_handle* h);
+
+/* n_sweeps x (sweep_L; sweep_R) = Local_sweep with Nsweep = n_sweeps
+ * (local_sweep.f90:136-161); measures every slice when measure != 0. */
+int bafq_sweep(bafq_handle* h, int n_sweeps, int measure);
+
+/* Per-chain observable sums, sample counts and counters accumulated since the
+ * previous call, then reset (replaces Obs_equal%ave + Fourier%preq's per-rank
+ * values, src/number_conserving/src/fourier_trans.f90:350-457; acceptance
+ * counters of AccCounter, operator_Hubbard.f90:104-129; the drift of
+ * compare_mat, stabilization.f90:256-266).  obs: n_chains * obs_stride,
+ * stats: n_chains * stats_stride. */
+int bafq_read_bin(bafq_handle* h, double* obs, double* stats);
+
+/* Current equal-time Green's function of one chain, N x N complex
+ * (Prop%Gr, src/number_conserving/src/process_matrix.f90:8-10). */
+int bafq_get_green(bafq_handle* h, int chain, double* g);
+
+/* Device time in milliseconds of the most recent warmup/prepare/sweep kernel,
+ * measured with HIP events on the stream the kernel ran on (replaces the
+ * SYSTEM_CLOCK wall time of src/number_conserving/src/main.f90:26-27,67-75). */
+int bafq_last_kernel_ms(const bafq_handle* h, double* ms);
+
+/* Self-test hooks with no counterpart in the reference: run the device
+ * complex GEMM C = op(A) op(B) (op 0: none, 1: conjugate transpose) and the
+ * column-pivoted QR on host matrices, for the numerics tests. */
+int bafq_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
+int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,113 @@
+"""Shared test helpers: oracle/solver runners and comparison utilities."""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+from oracle import bafq_oracle as O
+from bafqmc_amd.model import Params, ModelSpec
+from bafqmc_amd.solver import BatchedSolver
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def fixed_fields(lt, ns):
+    t = np.arange(1, lt + 1, dtype=float)[:, None]
+    s = np.arange(1, ns + 1, dtype=float)[None, :]
+    dens = 0.23 + 0.31 * np.sin(0.71 * t + 0.37 * s) + 0.09 * np.cos(0.53 * t * s)
+    rel = -0.19 + 0.28 * np.cos(0.43 * t - 0.61 * s) + 0.13 * np.sin(0.29 * t * s)
+    return np.stack((rel, dens), axis=-1)
+
+
+def oracle_params(d: dict) -> O.Params:
+    keys = ("pairing", "u1", "u2", "mu", "nlx", "nly", "ltrot", "beta", "nwrap", "shift_loc", "rdelta")
+    return O.Params(**{k: d.get(k, 0.0 if k == "rdelta" else None) for k in keys})
+
+
+def solver_params(d: dict) -> Params:
+    p = Params(pairing=d["pairing"], u1=d["u1"], u2=d["u2"], mu=d["mu"], rdelta=d.get("rdelta", 0.0),
+               nlx=d["nlx"], nly=d["nly"], ltrot=d["ltrot"], beta=d["beta"], nwrap=d["nwrap"],
+               shift_loc=d["shift_loc"], nbin=d.get("nbin", 1), ini_type=d.get("ini_type", 2),
+               ini_ampl=d.get("ini_ampl", 1.0))
+    p.nlx_therm, p.nly_therm, p.ltrot_therm = p.nlx, p.nly, p.ltrot
+    return p.validate()
+
+
+def reduced(g4, ns):
+    """{b, c+} block of a 4Ns Nambu matrix (rows/cols b_1..b_Ns, c+_1..c+_Ns)."""
+    idx = [*range(ns), *range(3 * ns, 4 * ns)]
+    return g4[np.ix_(idx, idx)]
+
+
+def rel_err(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    scale = max(float(np.max(np.abs(b))), 1e-300)
+    return float(np.max(np.abs(a - b))) / scale
+
+
+def compare_bins(mine: dict, ref: dict, keys=None):
+    """Worst relative error over observables (complex: scale by |value|)."""
+    worst, where = 0.0, None
+    for k, v in ref.items():
+        if keys is not None and k not in keys:
+            continue
+        if k not in mine or k.startswith("_"):
+            continue
+        e = rel_err(mine[k], v)
+        if e > worst:
+            worst, where = e, k
+    return worst, where
+
+
+def load_golden(name):
+    z = np.load(GOLDEN / f"{name}.npz", allow_pickle=False)
+    params = json.loads(str(z["params"]))
+    return z, params
+
+
+def golden_row(z, name, b):
+    """Row b of an output file as an observable value (complex for 2 columns)."""
+    row = z["out_" + name][b]
+    if row.size == 2 and name in ("sf_K", "psf_Gamma", "dw_K", "den_upup_sub11", "den_dodo_sub11", "den_updo"):
+        return complex(row[0], row[1])
+    if name == "density_site_total":
+        return row
+    return float(row[0])
+
+
+def run_solver(lib, params: Params, phis, seeds, nbin=1, nsweep=1, prepare=True):
+    s = BatchedSolver([ModelSpec(params)], [0] * len(seeds), lib=lib)
+    s.set_fields(phis, seeds)
+    if prepare:
+        s.prepare()
+    bins = []
+    for _ in range(nbin):
+        s.sweep(nsweep)
+        bins.append(s.read_bin())
+    return s, bins
+
+
+def chain_average(bins_of_chains, key):
+    vals = [b[key] for b in bins_of_chains]
+    return sum(vals) / len(vals)
+
+
+# Observables are sums of products of Green-function entries of magnitude O(1);
+# low-density estimators (density = G_ii - 1) inherit absolute, not relative,
+# round-off of G.  Tolerance: |a - b| <= OBS_RTOL |b| + OBS_ATOL.
+OBS_RTOL = 1e-9
+OBS_ATOL = 1e-12
+
+
+def assert_obs_close(v, ref, label, rtol=OBS_RTOL, atol=OBS_ATOL):
+    """Elementwise |v - ref| <= rtol |ref| + atol; complex values use the modulus
+    (their imaginary parts are round-off in both codes)."""
+    v = np.asarray(v)
+    ref = np.asarray(ref)
+    err = np.abs(v - ref)
+    bound = rtol * np.abs(ref) + atol
+    if not np.all(err <= bound):
+        raise AssertionError(f"{label}: |diff| {np.max(err):.3e} > bound {np.min(bound):.3e}; "
+                             f"got {v!r} want {ref!r}")

@@ -1,0 +1,195 @@
+"""Parity of the HIP solver (through the C ABI) with the oracle and the reference.
+
+G is compared on the scale of max|G| (1e-10, the bar stated for the sweep);
+observables with tests.helpers.assert_obs_close.  Pairing runs compare the
+solver's reduced {b, c+} block with the oracle's full 4Ns matrices.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import bafq_oracle as O
+from bafqmc_amd.model import ModelSpec, conf_set_batch
+from bafqmc_amd.solver import BatchedSolver
+from tests.helpers import (fixed_fields, load_golden, golden_row, oracle_params, solver_params, reduced,
+                           rel_err, assert_obs_close)
+
+pytestmark = pytest.mark.gpu
+G_TOL = 1e-10
+
+
+@pytest.mark.parametrize("n", [9, 18, 36, 72, 81, 144])
+def test_device_gemm(hip_lib, n):
+    rng = np.random.default_rng(n)
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    b = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    for oa in (0, 1):
+        for ob in (0, 1):
+            A = a.conj().T if oa else a
+            B = b.conj().T if ob else b
+            c = hip_lib.debug_gemm(a, b, oa, ob)
+            assert rel_err(c, A @ B) < 1e-13
+
+
+@pytest.mark.parametrize("n", [9, 18, 72, 81, 144])
+def test_device_qrp(hip_lib, n):
+    rng = np.random.default_rng(100 + n)
+    # graded columns, like the stabilisation matrices
+    a = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) * np.logspace(0, 8, n)[None, :]
+    r, tau, perm, d = hip_lib.debug_qrp(a)
+    R = np.triu(r) * d[:, None]
+    Q = np.eye(n, dtype=complex)
+    for k in range(n - 1, -1, -1):
+        v = np.zeros(n, complex)
+        v[k] = 1.0
+        v[k + 1:] = r[k + 1:, k]
+        Q = (np.eye(n) - tau[k] * np.outer(v, v.conj())) @ Q
+    assert sorted(perm.tolist()) == list(range(n))
+    assert rel_err(Q @ R, a[:, perm]) < 1e-13
+    assert np.max(np.abs(Q.conj().T @ Q - np.eye(n))) < 1e-13
+    assert np.all(np.diff(d) <= 1e-12 * d[0])          # pivoting orders |R_kk|
+    # same |R_kk| as LAPACK's pivoted QR
+    import scipy.linalg as sla
+    _, rl, _ = sla.qr(a, pivoting=True)
+    np.testing.assert_allclose(d, np.abs(np.diag(rl)), rtol=1e-9)
+
+
+CASES = [
+    dict(pairing=False, u1=0.7, u2=-0.1, mu=-5.0, nlx=3, nly=3, ltrot=6, beta=0.6, nwrap=2, shift_loc=0.0),
+    dict(pairing=False, u1=0.7, u2=-0.1, mu=-5.0, nlx=3, nly=3, ltrot=6, beta=0.6, nwrap=2, shift_loc=1.5),
+    dict(pairing=True, u1=0.7, u2=-0.1, mu=-5.0, rdelta=0.3, nlx=3, nly=3, ltrot=6, beta=0.6, nwrap=2,
+         shift_loc=0.0),
+    dict(pairing=True, u1=0.7, u2=-0.1, mu=-5.0, rdelta=0.3, nlx=3, nly=3, ltrot=6, beta=0.6, nwrap=3,
+         shift_loc=1.5),
+    dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=6, nly=6, ltrot=20, beta=2.0, nwrap=10, shift_loc=1.5),
+    dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=20, beta=2.0, nwrap=10,
+         shift_loc=1.0),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_sweep_matches_oracle(hip_lib, case):
+    check_sweep_vs_oracle(hip_lib, case)
+
+
+def check_sweep_vs_oracle(lib, case):
+    hip_lib = lib
+    d = CASES[case]
+    print(f"case {case}: {d}", flush=True)
+    ns = d["nlx"] * d["nly"]
+    phi = fixed_fields(d["ltrot"], ns)
+    ch = O.Chain(oracle_params(d), phi, 1357911)
+    ch.pre()
+    s = BatchedSolver([ModelSpec(solver_params(d))], [0], lib=hip_lib)
+    s.set_fields([phi], [1357911])
+    s.prepare()
+    go = reduced(ch.prop.g, ns) if d["pairing"] else ch.prop.g
+    assert rel_err(s.green(0), go) < G_TOL
+    for b in range(2):
+        ref = ch.bin()
+        s.sweep(1)
+        mine = s.read_bin()[0]
+        for k, v in ref.items():
+            assert_obs_close(mine[k], v, f"case {case} bin {b} {k}")
+        go = reduced(ch.prop.g, ns) if d["pairing"] else ch.prop.g
+        assert rel_err(s.green(0), go) < G_TOL
+        f, sd = s.get_fields()
+        np.testing.assert_allclose(f[0], ch.phi, rtol=0, atol=1e-13)
+        assert sd[0] == ch.rng.seed
+        st = mine["_stats"]
+        assert (st["acc_u1"], st["acc_u2"]) == (ch.acc[0, 1], ch.acc[1, 1])
+    s.close()
+
+
+GOLDEN = ["nc_fixed_3x3_w1", "nc_fixed_3x3_w6", "pairing_fixed_3x3_w2", "nc_traj_3x3_c1",
+          "nc_traj_3x3_u2", "pairing_traj_3x3", "pairing_traj_6x6_c2", "nc_traj_9x9_c4"]
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_sweep_matches_reference_fortran(hip_lib, name):
+    check_golden(hip_lib, name)
+
+
+def check_golden(lib, name):
+    """Chain-averaged per-bin outputs and final fields/seeds of the reference binaries."""
+    hip_lib = lib
+    z, pd = load_golden(name)
+    print(f"golden {name}", flush=True)
+    p = solver_params(pd)
+    seeds = [int(x) for x in z["seeds"]]
+    if "fields_in" in z.files:
+        phis, seeds_run = list(z["fields_in"]), seeds
+    else:
+        phis, seeds_run = conf_set_batch(p, seeds)
+        phis = list(phis)
+    s = BatchedSolver([ModelSpec(p)], [0] * len(seeds), lib=hip_lib)
+    s.set_fields(phis, [int(x) for x in seeds_run])
+    s.prepare()
+    for b in range(pd["nbin"]):
+        s.sweep(1)
+        per = s.read_bin()
+        for key in z.files:
+            if not key.startswith("out_"):
+                continue
+            k = key[4:]
+            if k not in per[0]:
+                continue
+            mine = sum(c[k] for c in per) / len(per)
+            assert_obs_close(mine, golden_row(z, k, b), f"{name} bin {b} {k}")
+    f, sd = s.get_fields()
+    for c in range(len(seeds)):
+        np.testing.assert_allclose(f[c], z["fields_out"][c], rtol=0, atol=1e-12)
+        assert sd[c] == int(z["seeds_out"][c])
+    if not pd["pairing"]:
+        # log_weight = -1/2 sum phi^2 + 2 log|det G| (pole_diagnostics.f90:130-131)
+        for c in range(len(seeds)):
+            sv = np.linalg.svd(s.green(c), compute_uv=False)
+            lw = -0.5 * float(np.sum(f[c] ** 2)) + 2.0 * float(np.sum(np.log(sv)))
+            if len(seeds) == 1:
+                np.testing.assert_allclose(lw, z["out_log_weight"][-1][0], rtol=1e-10)
+    s.close()
+
+
+def test_uneven_batch_equals_single_chains(hip_lib):
+    """Chains of different lattices/solvers in one launch evolve exactly as alone."""
+    specs = [CASES[1], CASES[3], CASES[4], CASES[5]]
+    models = [ModelSpec(solver_params(d)) for d in specs]
+    chain_model = [0, 1, 2, 3, 2, 0]
+    phis = [fixed_fields(specs[m]["ltrot"], specs[m]["nlx"] * specs[m]["nly"]) for m in chain_model]
+    seeds = [1111 + 7 * c for c in range(len(chain_model))]
+    s = BatchedSolver(models, chain_model, lib=hip_lib)
+    s.set_fields(phis, seeds)
+    s.prepare()
+    s.sweep(1)
+    batch = s.read_bin()
+    gb = [s.green(c) for c in range(len(chain_model))]
+    s.close()
+    for c, m in enumerate(chain_model):
+        s1 = BatchedSolver([models[m]], [0], lib=hip_lib)
+        s1.set_fields([phis[c]], [seeds[c]])
+        s1.prepare()
+        s1.sweep(1)
+        one = s1.read_bin()[0]
+        for k in ("density_total", "energy_density", "sf_K", "dw_K"):
+            assert batch[c][k] == one[k]
+        assert np.array_equal(gb[c], s1.green(0))
+        s1.close()
+
+
+@pytest.mark.parametrize("pairing,delta", [(False, 0.0), (True, 0.2)])
+def test_gaussian_closed_form_gpu(hip_lib, pairing, delta):
+    d = dict(pairing=pairing, u1=0.0, u2=0.0, mu=-5.0, rdelta=delta, nlx=3, nly=3, ltrot=100, beta=1.0,
+             nwrap=10, shift_loc=1.0)
+    s = BatchedSolver([ModelSpec(solver_params(d))], [0], lib=hip_lib)
+    s.set_fields([np.zeros((100, 9, 2))], [24681357])
+    s.prepare()
+    s.sweep(1)
+    b = s.read_bin()[0]
+    exact = O.gaussian_closed_form(delta, 1.0, -5.0)
+    for k, v in exact.items():
+        if k == "pair_equal" and not pairing:
+            continue
+        got = b[k].real if isinstance(b[k], complex) else b[k]
+        assert abs(got - v) <= 2e-9 + 2e-8 * abs(v), (k, got, v)
+    s.close()

@@ -1,0 +1,206 @@
+"""Pin the CPU oracle before trusting it (CPU only).
+
+* RNG checkpoints of src/common/test_numerics.f90:45-56;
+* closed-form Gaussian/Bogoliubov limit (tests/physics/test_gaussian_solvers.py:44-77),
+  NC and pairing, Delta -> -Delta symmetry, NC == pairing at Delta = 0;
+* fixed interacting fields against independent dense products
+  (tests/physics/test_fixed_field_solver.py:55-119) for three Nwrap;
+* determinant ratios against brute-force determinants (NC rank 1, pairing 4x4 block);
+* golden outputs of the reference Fortran binaries (tests/golden/*.npz).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import bafq_oracle as O
+from tests.helpers import fixed_fields, load_golden, golden_row, oracle_params, assert_obs_close
+
+ABS_TOL, REL_TOL = 2.0e-9, 2.0e-8
+
+
+def test_ranf_checkpoints():
+    r = O.Ranf(1)
+    r()
+    assert r.seed == 48828125
+    r()
+    assert r.seed == 52882121
+    expected = r.seed
+    for _ in range(10000):
+        expected = (48828125 * expected) % 2 ** 31
+        v = r()
+        assert r.seed == expected
+        assert 0.0 <= v < 1.0
+
+
+def _gaussian_run(pairing, delta, nwrap, nbin=2):
+    p = O.Params(pairing=pairing, u1=0.0, u2=0.0, mu=-5.0, nlx=3, nly=3, ltrot=100, beta=1.0,
+                 nwrap=nwrap, shift_loc=1.0, rdelta=delta)
+    ch = O.Chain(p, np.zeros((100, 9, 2)), 24681357)
+    ch.pre()
+    return [ch.bin() for _ in range(nbin)]
+
+
+@pytest.mark.parametrize("pairing,delta,nwrap", [(False, 0.0, 10), (False, 0.0, 2), (True, 0.0, 10),
+                                                 (True, 0.2, 10), (True, -0.2, 10), (True, 0.2, 2)])
+def test_gaussian_closed_form(pairing, delta, nwrap):
+    exact = O.gaussian_closed_form(delta, 1.0, -5.0)
+    for b in _gaussian_run(pairing, delta, nwrap):
+        for k, v in exact.items():
+            if k == "pair_equal" and not pairing:
+                continue
+            got = b[k].real if isinstance(b[k], complex) else b[k]
+            assert math.isclose(got, v, abs_tol=ABS_TOL, rel_tol=REL_TOL), (k, got, v)
+            if isinstance(b[k], complex):
+                assert abs(b[k].imag) < ABS_TOL
+
+
+def test_gaussian_symmetries():
+    plus = _gaussian_run(True, 0.2, 10, 1)[0]
+    minus = _gaussian_run(True, -0.2, 10, 1)[0]
+    nc = _gaussian_run(False, 0.0, 10, 1)[0]
+    zero = _gaussian_run(True, 0.0, 10, 1)[0]
+    for k in ("density_total", "energy_density", "sf_K", "dw_K"):
+        assert abs(plus[k] - minus[k]) < ABS_TOL
+        assert abs(zero[k] - nc[k]) < ABS_TOL
+    assert abs(plus["pair_equal"] + minus["pair_equal"]) < ABS_TOL
+    assert plus["pair_equal"] < -0.01
+
+
+def _dense_reference(ch):
+    """Independent dense traces of tests/physics/test_fixed_field_solver.py:55-119."""
+    n, lt = ch.n, ch.p.ltrot
+    latt = ch.latt
+    hop = latt.hopping()
+    greens = [O.dense_green(ch, tau)[0] for tau in range(1, lt + 1)]
+    dens, kin, dbl, ons = [], [], [], []
+    for g in greens:
+        occ = np.diag(g) - 1.0
+        dens.append(2.0 * occ.real)
+        kin.append(2.0 * np.trace(hop @ (g - np.eye(n))).real / n)
+        dbl.append(float(np.mean(np.abs(occ) ** 2)))
+        ons.append(float(np.sum(2.0 * occ ** 2 + occ).real))
+    onsite = float(np.mean(ons))
+    d = float(np.mean(dbl))
+    kinetic = float(np.mean(kin))
+    inter = 2.0 * (ch.p.u1 + ch.p.u2) * onsite / n + 2.0 * (ch.p.u2 - ch.p.u1) * d
+    _, prod = O.dense_green(ch, lt)
+    _, logabs = np.linalg.slogdet(np.eye(n) - prod)
+    return {"density_site_total": np.mean(dens, axis=0), "kinetic": kinetic, "doubleOcc": d,
+            "onsite_n2_up": onsite, "onsite_n2_do": onsite, "interaction_energy_density": inter,
+            "energy_density": kinetic + inter,
+            "log_weight": -0.5 * float(np.sum(ch.phi ** 2)) - 2.0 * logabs}
+
+
+@pytest.mark.parametrize("nwrap", [1, 2, 6])
+def test_fixed_field_dense(nwrap):
+    p = O.Params(pairing=False, u1=0.7, u2=-0.1, mu=-5.0, nlx=3, nly=3, ltrot=6, beta=0.6, nwrap=nwrap,
+                 shift_loc=0.0)
+    phi = fixed_fields(6, 9)
+    ch = O.Chain(p, phi, 1357911)
+    ref = _dense_reference(ch)
+    ch.pre()
+    for _ in range(2):
+        b = ch.bin()
+        for k, v in ref.items():
+            got = ch.log_weight() if k == "log_weight" else b[k]
+            np.testing.assert_allclose(got, v, rtol=2e-10, atol=2e-10, err_msg=k)
+    np.testing.assert_array_equal(ch.phi, phi)
+
+
+def test_nc_ratio_bruteforce():
+    """r_det = det(1-B)/det(1-B') for B' = (1 + Delta E_ii) B (localU.f90:55-59)."""
+    p = O.Params(pairing=False, u1=0.8, u2=-0.3, mu=-4.0, nlx=3, nly=3, ltrot=4, beta=1.0, nwrap=2,
+                 shift_loc=1.0)
+    rng = np.random.default_rng(7)
+    ch = O.Chain(p, rng.normal(size=(4, 9, 2)), 5)
+    for tau in range(1, 5):
+        g, prod = O.dense_green(ch, tau)
+        for i in (0, 4, 8):
+            for al in (ch.alpha[0], ch.alpha[1]):
+                dphi = rng.normal()
+                delta = np.exp(al * dphi) - 1.0
+                e = np.zeros((9, 9)); e[i, i] = 1.0
+                prod2 = (np.eye(9) + delta * e) @ prod
+                brute = np.linalg.det(np.eye(9) - prod) / np.linalg.det(np.eye(9) - prod2)
+                local = 1.0 / (1.0 + delta * (1.0 - g[i, i]))
+                assert abs(brute - local) < 1e-11 * abs(local)
+
+
+def test_pairing_ratio_bruteforce_and_block_structure():
+    """|det P|^{-1/2} equals |det(1-B4)/det(1-B4')|^{1/2} (pairing localU.f90:59-66), and the
+    4Ns Green function is block diagonal on {b,c+} + {c,b+} with the second block the conjugate."""
+    p = O.Params(pairing=True, u1=0.8, u2=-0.3, mu=-4.0, nlx=3, nly=3, ltrot=4, beta=1.0, nwrap=2,
+                 shift_loc=1.0, rdelta=0.4)
+    rng = np.random.default_rng(11)
+    ch = O.Chain(p, rng.normal(size=(4, 9, 2)), 5)
+    ns = 9
+    b1 = [*range(ns), *range(3 * ns, 4 * ns)]
+    b2 = [*range(ns, 2 * ns), *range(2 * ns, 3 * ns)]
+    for tau in (1, 3):
+        g, prod = O.dense_green(ch, tau)
+        off = [*b2]
+        assert np.max(np.abs(g[np.ix_(b1, off)])) < 1e-12 * np.max(np.abs(g))
+        np.testing.assert_allclose(g[np.ix_(b2, b2)], np.conj(g[np.ix_(b1, b1)]), rtol=0, atol=1e-12 * np.abs(g).max())
+        for i in (0, 5):
+            for nf in (1, 2):
+                dphi = rng.normal()
+                al = ch.alpha[nf - 1]
+                signs = ch._sector_signs(nf)
+                dvec = np.array([np.exp(s * al * dphi) - 1.0 for s in signs])
+                isec = ch._rows(i)
+                dm = np.zeros((4 * ns, 4 * ns), complex)
+                for k, r in enumerate(isec):
+                    dm[r, r] = dvec[k]
+                prod2 = (np.eye(4 * ns) + dm) @ prod
+                brute = abs(np.linalg.det(np.eye(4 * ns) - prod) / np.linalg.det(np.eye(4 * ns) - prod2)) ** 0.5
+                gss = g[np.ix_(isec, isec)]
+                pb = np.eye(4) + dvec[:, None] * (np.eye(4) - gss)
+                local = abs(complex(np.linalg.det(pb)) ** (-0.5))
+                assert abs(brute - local) < 1e-10 * local
+
+
+GOLDEN_CASES = ["nc_fixed_3x3_w1", "nc_fixed_3x3_w2", "nc_fixed_3x3_w6", "pairing_fixed_3x3_w2",
+                "nc_traj_3x3_c1", "nc_traj_3x3_u2", "pairing_traj_3x3"]
+
+
+def oracle_run_golden(name):
+    """Run the oracle on a golden case; returns (per-bin chain-averaged dicts, chains)."""
+    z, pd = load_golden(name)
+    seeds = [int(s) for s in z["seeds"]]
+    p = oracle_params(pd)
+    ns = p.nlx * p.nly
+    chains = []
+    for c, s in enumerate(seeds):
+        if "fields_in" in z:
+            phi, seed = z["fields_in"][c], s
+        else:
+            r = O.Ranf(s)
+            phi = O.conf_set(r, pd.get("ini_type", 2), pd.get("ini_ampl", 1.0), (0.0, 0.0), ns, p.ltrot)
+            seed = r.seed
+        ch = O.Chain(p, phi, seed)
+        ch.pre()
+        chains.append(ch)
+    bins = []
+    for _ in range(pd["nbin"]):
+        per = [ch.bin() for ch in chains]
+        bins.append({k: sum(b[k] for b in per) / len(per) for k in per[0]})
+    return z, pd, bins, chains
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_matches_reference_fortran(name):
+    z, pd, bins, chains = oracle_run_golden(name)
+    for b, vals in enumerate(bins):
+        for k, v in vals.items():
+            if "out_" + k not in z.files:
+                continue
+            ref = golden_row(z, k, b)
+            assert_obs_close(v, ref, f"{k} bin {b}")
+    for c, ch in enumerate(chains):
+        assert ch.rng.seed == int(z["seeds_out"][c])
+        np.testing.assert_allclose(ch.phi, z["fields_out"][c], rtol=0, atol=1e-13)
+    if not pd["pairing"] and len(chains) == 1:
+        np.testing.assert_allclose(chains[0].log_weight(), z["out_log_weight"][-1][0], rtol=1e-10)
