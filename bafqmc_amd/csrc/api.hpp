@@ -28,6 +28,7 @@ const char* rt_error();
 int rt_block_threads();        // threads per workgroup for the sweep kernels
 int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
 int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
+int rt_debug_mfma(double* out);
 
 constexpr double kZero = 1.0e-10;   // src/number_conserving/src/calc_basic.f90:5
 
@@ -362,5 +363,7 @@ int bafq_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b,
 int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
     return bafq::rt_debug_qrp(n, a, tau, perm, d);
 }
+
+int bafq_debug_mfma_probe(double* out) { return bafq::rt_debug_mfma(out); }
 
 }  // extern "C"

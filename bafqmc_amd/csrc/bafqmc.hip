@@ -50,6 +50,25 @@ __global__ void __launch_bounds__(512) k_debug_qrp(cd* A, int n, cd* tau, int* p
     }
 }
 
+// Lane-layout probe of the FP64 MFMA: workgroup p (one wavefront) multiplies
+// A = one-hot at lane p by B = (1000 + lane) and writes the 4 accumulators.
+__global__ void __launch_bounds__(64) k_debug_mfma(double* out) {
+    Ctx c{nullptr};
+    const int p = blockIdx.x, l = threadIdx.x;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    acc = c.mfma(l == p ? 1.0 : 0.0, 1000.0 + l, acc);
+    for (int r = 0; r < 4; ++r) out[(p * 64 + l) * 4 + r] = acc[r];
+}
+
+int rt_debug_mfma(double* out) {
+    double* d = nullptr;
+    if (rt_malloc((void**)&d, sizeof(double) * 64 * 64 * 4)) return 1;
+    hipLaunchKernelGGL(k_debug_mfma, dim3(64), dim3(64), 0, 0, d);
+    int rc = check(hipGetLastError(), "k_debug_mfma") || rt_sync() || rt_d2h(out, d, sizeof(double) * 64 * 64 * 4);
+    rt_free(d);
+    return rc;
+}
+
 int rt_malloc(void** p, size_t bytes) { return check(hipMalloc(p, bytes), "hipMalloc"); }
 void rt_free(void* p) { (void)hipFree(p); }
 int rt_h2d(void* dst, const void* src, size_t bytes) {

@@ -81,6 +81,16 @@ int rt_launch(KernelId k, const LaunchArgs& a, int grid, int block, size_t smem,
     return 0;
 }
 
+int rt_debug_mfma(double* out) {
+    for (int p = 0; p < 64; ++p)
+        run_groups(1, 64, 0, [&](EmuCtx& c) {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            acc = c.mfma(c.lane() == p ? 1.0 : 0.0, 1000.0 + c.lane(), acc);
+            for (int r = 0; r < 4; ++r) out[(p * 64 + c.lane()) * 4 + r] = acc[r];
+        });
+    return 0;
+}
+
 int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c) {
     run_groups(1, rt_block_threads(), 0, [&](EmuCtx& ctx) {
         gemm_cc(ctx, (cd*)c, (const cd*)a, op_a != 0, (const cd*)b, op_b != 0, n);

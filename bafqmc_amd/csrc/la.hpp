@@ -108,7 +108,7 @@ DEV void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool o
             acci = c.mfma(a.i, b.r, acci);
         }
         for (int r = 0; r < 4; ++r) {
-            const int row = 16 * ti + 4 * lk + r, col = 16 * tj + li;
+            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
             if (row < n && col < n) C[row * n + col] = cd{accr[r], acci[r]};
         }
     }
@@ -143,7 +143,7 @@ DEV void gemm_rc(const Ctx& c, cd* C, const double* E, const cd* X, int side, in
             }
         }
         for (int r = 0; r < 4; ++r) {
-            const int row = 16 * ti + 4 * lk + r, col = 16 * tj + li;
+            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
             if (row < n && col < n) C[row * n + col] = cd{accr[r], acci[r]};
         }
     }

@@ -11,9 +11,10 @@
 //     from the 64 lanes' operands.  It exists so the same kernels can be run
 //     and bounds-checked on a machine without a GPU.
 //
-// MFMA f64 16x16x4 operand layout (CDNA3/CDNA4 ISA): lane l holds
+// MFMA f64 16x16x4 operand layout (CDNA3/CDNA4): lane l holds
 //   A[i][k] with i = l % 16, k = l / 16;  B[k][j] with j = l % 16, k = l / 16;
-//   D[i][j] in accumulator register r with i = 4 * (l / 16) + r, j = l % 16.
+//   D[i][j] in accumulator register r with i = l / 16 + 4 * r, j = l % 16
+// (checked on gfx950 by bafq_debug_mfma_probe / tests/test_gpu_parity.py).
 #pragma once
 
 #include <cstdint>
@@ -43,6 +44,9 @@ struct d4 {
 namespace bafq {
 
 constexpr int kWave = 64;
+
+// Row of the 16x16 MFMA f64 result held by lane `lane` in accumulator register r.
+HOSTDEV int mfma_row(int lane, int r) { return (lane >> 4) + 4 * r; }
 
 #ifndef BAFQ_EMU
 struct HipCtx {
@@ -105,7 +109,7 @@ struct EmuCtx {
         wsync();
         const int l = lane(), j = l & 15;
         for (int r = 0; r < 4; ++r) {
-            const int i = 4 * (l >> 4) + r;
+            const int i = mfma_row(l, r);
             double s = c[r];
             for (int k = 0; k < 4; ++k) s += A[i + 16 * k] * B[j + 16 * k];
             c[r] = s;

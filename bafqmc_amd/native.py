@@ -44,6 +44,7 @@ SYMBOLS = {
     "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),
     "bafq_debug_qrp": (C.c_int, [C.c_int, c_double_p, c_double_p, c_int32_p, c_double_p]),
+    "bafq_debug_mfma_probe": (C.c_int, [c_double_p]),
 }
 
 
@@ -82,6 +83,12 @@ class NativeLibrary:
         if rc:
             raise RuntimeError("bafq_debug_gemm failed")
         return c
+
+    def debug_mfma_probe(self):
+        out = np.zeros((64, 64, 4))
+        if self.lib.bafq_debug_mfma_probe(dptr(out)):
+            raise RuntimeError("bafq_debug_mfma_probe failed")
+        return out
 
     def debug_qrp(self, a):
         n = a.shape[0]

@@ -96,6 +96,10 @@ int bafq_last_kernel_ms(const bafq_handle* h, double* ms);
  * column-pivoted QR on host matrices, for the numerics tests. */
 int bafq_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
 int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
+/* Lane layout of the FP64 MFMA 16x16x4 tile operation: out[p][lane][r] =
+ * accumulator r of `lane` for A = one-hot at lane p, B = 1000 + lane
+ * (64 x 64 x 4 doubles). */
+int bafq_debug_mfma_probe(double* out);
 
 #ifdef __cplusplus
 }

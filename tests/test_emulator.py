@@ -9,7 +9,11 @@ import numpy as np
 import pytest
 
 from tests.helpers import rel_err
-from tests.test_gpu_parity import check_sweep_vs_oracle, check_golden
+from tests.test_gpu_parity import check_sweep_vs_oracle, check_golden, check_mfma_layout
+
+
+def test_emu_mfma_layout(emu_lib):
+    check_mfma_layout(emu_lib)
 
 
 def test_emu_gemm_qrp(emu_lib):

@@ -19,6 +19,31 @@ pytestmark = pytest.mark.gpu
 G_TOL = 1e-10
 
 
+def check_mfma_layout(lib):
+    """FP64 MFMA 16x16x4: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, result
+    D[i][j] in lane j + 16 (i % 4), accumulator register i // 4."""
+    out = lib.debug_mfma_probe()
+    seen = {}
+    for p in range(64):
+        ip, kp = p % 16, p // 16
+        for lane in range(64):
+            for r in range(4):
+                row, col = lane // 16 + 4 * r, lane % 16
+                want = 1000.0 + col + 16 * kp if row == ip else 0.0
+                v = out[p, lane, r]
+                if v != 0.0:
+                    q = int(round(v - 1000.0))
+                    seen[(lane, r)] = (ip, q % 16)
+                if v != want:
+                    table = {k: seen.get(k) for k in sorted(seen)}
+                    raise AssertionError(f"MFMA layout differs at p={p} lane={lane} r={r}: {v} != {want}; "
+                                         f"(lane, reg) -> (row, col) seen so far: {table}")
+
+
+def test_mfma_layout(hip_lib):
+    check_mfma_layout(hip_lib)
+
+
 @pytest.mark.parametrize("n", [9, 18, 36, 72, 81, 144])
 def test_device_gemm(hip_lib, n):
     rng = np.random.default_rng(n)
