@@ -15,6 +15,7 @@
 namespace bafq {
 
 enum KernelId { K_PREPARE = 0, K_SWEEP = 1, K_WARMUP = 2 };
+enum PathId { P_GLOBAL = 0, P_LDS = 1 };
 
 // back-end hooks (0 on success)
 int rt_malloc(void** p, size_t bytes);
@@ -22,7 +23,8 @@ void rt_free(void* p);
 int rt_h2d(void* dst, const void* src, size_t bytes);
 int rt_d2h(void* dst, const void* src, size_t bytes);
 int rt_memset(void* dst, int v, size_t bytes);
-int rt_launch(KernelId k, const LaunchArgs& a, int grid, int block, size_t smem, double* ms);
+int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms);
+double rt_clock_hz();          // rate of Ctx::clock()
 int rt_sync();
 const char* rt_error();
 int rt_block_threads();        // threads per workgroup for the sweep kernels
@@ -51,7 +53,9 @@ struct bafq_handle {
     double* d_stats = nullptr;
     size_t phi_total = 0;
     std::vector<size_t> phi_off;
-    size_t max_lds = 0;
+    size_t max_lds[2] = {0, 0};
+    int* d_path_ids[2] = {nullptr, nullptr};
+    int path_count[2] = {0, 0};
     double last_ms = 0.0;
 };
 
@@ -148,15 +152,37 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
     Bump b;
     const size_t off_models = b.take(sizeof(ModelDev) * n_models);
     std::vector<size_t> off_ep(n_models), off_em(n_models), off_ph(n_models), off_bd(n_models);
+    std::vector<size_t> off_ez(n_models);
+    std::vector<std::vector<double>> ez(n_models);
     for (int k = 0; k < n_models; ++k) {
         const int n = h->models[k].n, ns = h->models[k].ns;
         off_ep[k] = b.take(sizeof(double) * n * n);
         off_em[k] = b.take(sizeof(double) * n * n);
+        // MFMA operand order of e^{-+dtau A} (left: [ti][kk][lane] = E[16ti+l%16][4kk+l/16],
+        // right: [tj][kk][lane] = E[4kk+l/16][16tj+l%16]), zero padded.
+        const int nt = (n + 15) / 16, nk = (n + 3) / 4;
+        const size_t per = (size_t)nt * nk * 64;
+        ez[k].assign(4 * per, 0.0);
+        for (int which = 0; which < 4; ++which) {
+            const double* E = (which == 0 || which == 2) ? models[k].expk_p : models[k].expk_m;
+            const bool left = which < 2;
+            double* dst = ez[k].data() + which * per;
+            for (int t = 0; t < nt; ++t)
+                for (int kk = 0; kk < nk; ++kk)
+                    for (int l = 0; l < 64; ++l) {
+                        const int i = left ? 16 * t + (l & 15) : 4 * kk + (l >> 4);
+                        const int j = left ? 4 * kk + (l >> 4) : 16 * t + (l & 15);
+                        dst[((size_t)t * nk + kk) * 64 + l] = (i < n && j < n) ? E[(size_t)i * n + j] : 0.0;
+                    }
+        }
+        off_ez[k] = b.take(sizeof(double) * 4 * per);
         off_ph[k] = b.take(sizeof(cd) * ns);
         off_bd[k] = b.take(sizeof(int) * 3 * ns);
     }
     const size_t off_chains = b.take(sizeof(ChainDev) * n_chains);
     const size_t off_ids = b.take(sizeof(int) * n_chains);
+    const size_t off_pids0 = b.take(sizeof(int) * n_chains);
+    const size_t off_pids1 = b.take(sizeof(int) * n_chains);
     const size_t off_seeds = b.take(sizeof(int) * n_chains);
     int max_ns = 0;
     for (int c = 0; c < n_chains; ++c) {
@@ -202,6 +228,9 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
         rt_h2d(base + off_bd[k], bonds[k].data(), sizeof(int) * 3 * m.ns);
         m.ep = (const double*)(base + off_ep[k]);
         m.em = (const double*)(base + off_em[k]);
+        rt_h2d(base + off_ez[k], ez[k].data(), sizeof(double) * ez[k].size());
+        for (int w = 0; w < 4; ++w)
+            m.ez[w] = (const double*)(base + off_ez[k]) + (size_t)w * (ez[k].size() / 4);
         m.phasek = (const cd*)(base + off_ph[k]);
         m.bonds = (const int*)(base + off_bd[k]);
     }
@@ -214,8 +243,8 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
     h->d_phi = (double*)(base + off_phi);
     rt_h2d(h->d_models, h->models.data(), sizeof(ModelDev) * n_models);
     h->chains.resize(n_chains);
-    std::vector<int> ids(n_chains);
-    int nw = rt_block_threads() / 64;
+    std::vector<int> ids(n_chains), pids[2];
+    const int nw_glb = rt_block_threads() / 64;
     for (int c = 0; c < n_chains; ++c) {
         ChainDev& ch = h->chains[c];
         const COff& o = co[c];
@@ -236,10 +265,22 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
         h->chain_n.push_back(m.n);
         h->chain_ns.push_back(m.ns);
         h->chain_lt.push_back(m.ltrot);
-        h->max_lds = std::max(h->max_lds, lds_bytes(m.n, nw));
+        // BAFQ_FORCE_GLOBAL=1 runs every chain on the global-memory path (tests of that path)
+        const char* fg = std::getenv("BAFQ_FORCE_GLOBAL");
+        const bool force_global = fg && fg[0] == '1';
+        const int path = (m.n <= kLdsMaxN && !force_global) ? P_LDS : P_GLOBAL;
+        pids[path].push_back(c);
+        const size_t lb = path == P_LDS ? lds_bytes(m.n, kLdsWaves, true) : lds_bytes(m.n, nw_glb, false);
+        h->max_lds[path] = std::max(h->max_lds[path], lb);
     }
     rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * n_chains);
     rt_h2d(h->d_ids, ids.data(), sizeof(int) * n_chains);
+    h->d_path_ids[P_GLOBAL] = (int*)(base + off_pids0);
+    h->d_path_ids[P_LDS] = (int*)(base + off_pids1);
+    for (int p = 0; p < 2; ++p) {
+        h->path_count[p] = (int)pids[p].size();
+        if (!pids[p].empty()) rt_h2d(h->d_path_ids[p], pids[p].data(), sizeof(int) * pids[p].size());
+    }
     if (rt_sync()) { *out = h; return fail(h, rt_error()); }
     *out = h;
     return 0;
@@ -287,21 +328,26 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
     LaunchArgs a;
     a.models = h->d_models;
     a.chains = h->d_chains;
-    a.ids = h->d_ids;
-    a.count = (int)h->chains.size();
     a.nsweep = nsweep;
     a.measure = measure;
-    int grid, block;
-    size_t smem = 0;
+    double total_ms = 0.0;
     if (k == K_WARMUP) {
-        block = 64;
-        grid = (a.count + block - 1) / block;
+        a.ids = h->d_ids;
+        a.count = (int)h->chains.size();
+        const int block = 64, grid = (a.count + block - 1) / block;
+        if (rt_launch(k, P_GLOBAL, a, grid, block, 0, &total_ms)) return fail(h, rt_error());
     } else {
-        block = rt_block_threads();
-        grid = a.count;
-        smem = h->max_lds;
+        for (int p = 0; p < 2; ++p) {
+            if (h->path_count[p] == 0) continue;
+            a.ids = h->d_path_ids[p];
+            a.count = h->path_count[p];
+            const int block = p == P_LDS ? kLdsWaves * 64 : rt_block_threads();
+            double ms = 0.0;
+            if (rt_launch(k, (PathId)p, a, a.count, block, h->max_lds[p], &ms)) return fail(h, rt_error());
+            total_ms += ms;
+        }
     }
-    if (rt_launch(k, a, grid, block, smem, &h->last_ms)) return fail(h, rt_error());
+    h->last_ms = total_ms;
     if (rt_sync()) return fail(h, rt_error());
     return 0;
 }
@@ -337,6 +383,11 @@ int bafq_read_bin(bafq_handle* h, double* obs, double* stats) {
     for (size_t c = 0; c < nc; ++c)
         for (int k = 0; k < S_LEN; ++k)
             if (k != S_LOGDET && k != S_XMAX) st[c * S_LEN + k] = 0.0;
+    if (stats) {
+        const double hz = rt_clock_hz();
+        for (size_t c = 0; c < nc; ++c)
+            for (int k = S_T_SITE; k < S_T_SITE + kNumTimers; ++k) stats[c * S_LEN + k] /= hz;
+    }
     if (rt_h2d(h->d_stats, st.data(), sizeof(double) * S_LEN * nc)) return fail(h, rt_error());
     return rt_sync() ? fail(h, rt_error()) : 0;
 }

@@ -19,13 +19,25 @@ static int check(hipError_t e, const char* what) {
 __global__ void __launch_bounds__(512) k_prepare(LaunchArgs a) {
     extern __shared__ __align__(16) char smem[];
     Ctx c{smem};
-    body_prepare(c, a);
+    body_prepare<false>(c, a);
 }
 
 __global__ void __launch_bounds__(512) k_sweep(LaunchArgs a) {
     extern __shared__ __align__(16) char smem[];
     Ctx c{smem};
-    body_sweep(c, a);
+    body_sweep<false>(c, a);
+}
+
+__global__ void __launch_bounds__(512) k_prepare_lds(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem};
+    body_prepare<true>(c, a);
+}
+
+__global__ void __launch_bounds__(512) k_sweep_lds(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem};
+    body_sweep<true>(c, a);
 }
 
 __global__ void __launch_bounds__(64) k_warmup(LaunchArgs a) {
@@ -41,7 +53,7 @@ __global__ void __launch_bounds__(512) k_debug_gemm(cd* C, const cd* A, int oa, 
 __global__ void __launch_bounds__(512) k_debug_qrp(cd* A, int n, cd* tau, int* perm, double* d) {
     extern __shared__ __align__(16) char smem[];
     Ctx c{smem};
-    const Lds L = carve(smem, n, c.nwaves());
+    const Lds L = carve(smem, n, c.nwaves(), false);
     qrp(c, A, n, L.qr, L.dtmp);
     for (int k = c.tid(); k < n; k += c.nthreads()) {
         tau[k] = L.qr.tau[k];
@@ -88,28 +100,33 @@ int rt_block_threads() {
     return b;
 }
 
-int rt_launch(KernelId k, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
+double rt_clock_hz() {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || khz <= 0) return 1.0e8;
+    return 1000.0 * khz;
+}
+
+int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
     static hipEvent_t ev[2] = {nullptr, nullptr};
     if (!ev[0]) {
         if (check(hipEventCreate(&ev[0]), "hipEventCreate") || check(hipEventCreate(&ev[1]), "hipEventCreate"))
             return 1;
     }
-    if (smem > 65536 &&
-        check(hipFuncSetAttribute(k == K_PREPARE ? (const void*)k_prepare : (const void*)k_sweep,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
-              "hipFuncSetAttribute"))
+    const void* fn = nullptr;
+    if (k == K_PREPARE) fn = path == P_LDS ? (const void*)k_prepare_lds : (const void*)k_prepare;
+    if (k == K_SWEEP) fn = path == P_LDS ? (const void*)k_sweep_lds : (const void*)k_sweep;
+    if (fn && smem > 65536 &&
+        check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem), "hipFuncSetAttribute"))
         return 1;
     if (check(hipEventRecord(ev[0], 0), "hipEventRecord")) return 1;
-    switch (k) {
-    case K_PREPARE:
-        hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(block), smem, 0, a);
-        break;
-    case K_SWEEP:
-        hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(block), smem, 0, a);
-        break;
-    case K_WARMUP:
+    if (k == K_WARMUP) {
         hipLaunchKernelGGL(k_warmup, dim3(grid), dim3(block), 0, 0, a);
-        break;
+    } else if (k == K_PREPARE) {
+        if (path == P_LDS) hipLaunchKernelGGL(k_prepare_lds, dim3(grid), dim3(block), smem, 0, a);
+        else hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(block), smem, 0, a);
+    } else {
+        if (path == P_LDS) hipLaunchKernelGGL(k_sweep_lds, dim3(grid), dim3(block), smem, 0, a);
+        else hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(block), smem, 0, a);
     }
     if (check(hipGetLastError(), "kernel launch")) return 1;
     if (check(hipEventRecord(ev[1], 0), "hipEventRecord") || check(hipEventSynchronize(ev[1]), "hipEventSynchronize"))
@@ -145,7 +162,7 @@ int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
     int rc = rt_h2d(da, a, bytes);
     if (!rc) {
         const int block = rt_block_threads();
-        hipLaunchKernelGGL(k_debug_qrp, dim3(1), dim3(block), lds_bytes(n, block / 64), 0, da, n, dt, dp, dd);
+        hipLaunchKernelGGL(k_debug_qrp, dim3(1), dim3(block), lds_bytes(n, block / 64, false), 0, da, n, dt, dp, dd);
         rc = check(hipGetLastError(), "k_debug_qrp") || rt_sync() || rt_d2h(a, da, bytes) ||
              rt_d2h(tau, dt, sizeof(cd) * n) || rt_d2h(perm, dp, sizeof(int) * n) || rt_d2h(d, dd, sizeof(double) * n);
     }

@@ -59,7 +59,9 @@ int rt_block_threads() {
     return b;
 }
 
-int rt_launch(KernelId k, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
+double rt_clock_hz() { return 1.0e9; }
+
+int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
     if (ms) *ms = 0.0;
     if (k == K_WARMUP) {
         // one thread per chain; no workgroup cooperation
@@ -75,8 +77,13 @@ int rt_launch(KernelId k, const LaunchArgs& a, int grid, int block, size_t smem,
         return 0;
     }
     run_groups(grid, block, smem, [&](EmuCtx& c) {
-        if (k == K_PREPARE) body_prepare(c, a);
-        else body_sweep(c, a);
+        if (k == K_PREPARE) {
+            if (path == P_LDS) body_prepare<true>(c, a);
+            else body_prepare<false>(c, a);
+        } else {
+            if (path == P_LDS) body_sweep<true>(c, a);
+            else body_sweep<false>(c, a);
+        }
     });
     return 0;
 }
@@ -100,8 +107,8 @@ int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, d
 
 int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
     const int block = rt_block_threads();
-    run_groups(1, block, lds_bytes(n, block / 64), [&](EmuCtx& ctx) {
-        const Lds L = carve(ctx.smem, n, ctx.nwaves());
+    run_groups(1, block, lds_bytes(n, block / 64, false), [&](EmuCtx& ctx) {
+        const Lds L = carve(ctx.smem, n, ctx.nwaves(), false);
         qrp(ctx, (cd*)a, n, L.qr, L.dtmp);
         for (int k = ctx.tid(); k < n; k += ctx.nthreads()) {
             ((cd*)tau)[k] = L.qr.tau[k];

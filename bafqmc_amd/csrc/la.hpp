@@ -83,13 +83,23 @@ DEV void vec_copy(const Ctx& c, double* dst, const double* src, int n) {
     c.sync();
 }
 
+// M[r][k] *= d[k] (real column scaling).
+DEV void scale_mat_cols_real(const Ctx& c, cd* M, int n, const double* d) {
+    const int nn = n * n;
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int k = e % n;
+        M[e] = cd{M[e].r * d[k], M[e].i * d[k]};
+    }
+    c.sync();
+}
+
 // --------------------------------------------------------------- GEMMs
 // C = op(A) * op(B), op in {N, C (conjugate transpose)}; C must not alias.
 DEV cd load_op(const cd* M, bool conj_t, int row, int col, int n) {
     return conj_t ? conj(M[col * n + row]) : M[row * n + col];
 }
 
-DEV void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool opB, int n) {
+DEVFN void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool opB, int n) {
     const int nt = (n + 15) >> 4;
     const int lane = c.lane();
     const int li = lane & 15, lk = lane >> 4;
@@ -116,7 +126,7 @@ DEV void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool o
 }
 
 // Real-by-complex: side 0: C = E * X ; side 1: C = X * E.  E real row-major.
-DEV void gemm_rc(const Ctx& c, cd* C, const double* E, const cd* X, int side, int n) {
+DEVFN void gemm_rc(const Ctx& c, cd* C, const double* E, const cd* X, int side, int n) {
     const int nt = (n + 15) >> 4;
     const int lane = c.lane();
     const int li = lane & 15, lk = lane >> 4;
@@ -167,7 +177,7 @@ constexpr int kGroup = 8;   // lanes cooperating on one column
 // D[k] = |R_kk|, perm[k] = original index of column k, tau[k] the scalar
 // factors with H(k) = I - tau_k v_k v_k^H, Q = H(0) H(1) ... H(n-1).
 // Column norms are recomputed exactly after every step (no downdating).
-DEV void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
+DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     // initial squared column norms
@@ -300,7 +310,7 @@ DEV cd refl(const cd* A, int n, int r, int k) {
 }
 
 // Q = H(0) ... H(n-1) formed explicitly (ZUNGQR).
-DEV void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n) {
+DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n) {
     mat_identity(c, Q, n);
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
@@ -327,7 +337,7 @@ DEV void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n) {
 }
 
 // X <- Q^H X (ZUNMQR 'L','C').
-DEV void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n) {
+DEVFN void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n) {
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     for (int k = 0; k < n; ++k) {
@@ -353,7 +363,7 @@ DEV void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n) {
 }
 
 // X <- V^{-1} X, V = upper triangle of A (ZTRSM 'L','U','N','N').
-DEV void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n) {
+DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n) {
     const int tid = c.tid(), nth = c.nthreads();
     for (int r = n - 1; r >= 0; --r) {
         const cd inv = cinv(A[r * n + r]);
@@ -369,7 +379,7 @@ DEV void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n) {
 }
 
 // Y = V * X[perm, :]   (ZLAPMR forward + ZTRMM 'L','U','N').
-DEV void trmm_perm_rows(const Ctx& c, cd* Y, const cd* A, const cd* X, const int* perm, int n) {
+DEVFN void trmm_perm_rows(const Ctx& c, cd* Y, const cd* A, const cd* X, const int* perm, int n) {
     const int nn = n * n;
     for (int e = c.tid(); e < nn; e += c.nthreads()) {
         const int r = e / n, j = e - r * n;
@@ -384,7 +394,7 @@ DEV void trmm_perm_rows(const Ctx& c, cd* Y, const cd* A, const cd* X, const int
 }
 
 // Y = X[:, perm] * V^H  (ZLAPMT forward + ZTRMM 'R','U','C').
-DEV void perm_cols_times_vh(const Ctx& c, cd* Y, const cd* X, const cd* A, const int* perm, int n) {
+DEVFN void perm_cols_times_vh(const Ctx& c, cd* Y, const cd* X, const cd* A, const int* perm, int n) {
     const int nn = n * n;
     for (int e = c.tid(); e < nn; e += c.nthreads()) {
         const int r = e / n, j = e - r * n;

@@ -23,12 +23,13 @@
 #ifndef BAFQ_EMU
 #include <hip/hip_runtime.h>
 #define DEV __device__ __forceinline__
-#define DEVFN __device__
+#define DEVFN __device__ __noinline__
 #define HOSTDEV __host__ __device__ inline
 typedef double d4 __attribute__((ext_vector_type(4)));
 #else
 #include <atomic>
 #include <barrier>
+#include <chrono>
 #include <cstring>
 #include <vector>
 #define DEV inline
@@ -65,6 +66,8 @@ struct HipCtx {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
     DEV int atomic_add(int* p, int v) const { return atomicAdd(p, v); }
+    // constant-rate device clock (hipDeviceAttributeWallClockRate kHz)
+    DEV uint64_t clock() const { return wall_clock64(); }
 };
 using Ctx = HipCtx;
 #else
@@ -119,6 +122,11 @@ struct EmuCtx {
     }
     int atomic_add(int* p, int v) const {
         return reinterpret_cast<std::atomic<int>*>(p)->fetch_add(v);
+    }
+    // nanoseconds
+    uint64_t clock() const {
+        return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                   std::chrono::steady_clock::now().time_since_epoch()).count();
     }
 };
 using Ctx = EmuCtx;

@@ -13,6 +13,14 @@
 // HS diagonal factors of site i and channel nf are exp(s * alpha_nf * phi)
 // with s = +1 on every row except the c+ row of the total-density channel
 // (src/pairing/src/operator_Hubbard.f90:155-199).
+//
+// Two execution paths share the numerics:
+//   * LDS path (N <= kLdsMaxN, 512 threads): G lives in LDS for the whole
+//     sweep; the kinetic propagation runs on FP64 MFMA from LDS with the
+//     propagators pre-ordered as MFMA operands; the B(tau) products of the U
+//     factors are applied at each stabilisation point (the fields of those
+//     slices are final by then), in LDS.
+//   * global path (any N): G and all factors in global memory (L2/MALL).
 #pragma once
 
 #include "la.hpp"
@@ -30,8 +38,15 @@ enum ObsIndex {
 };
 enum StatIndex {
     S_PROP_U1 = 0, S_ACC_U1, S_PROP_U2, S_ACC_U2, S_XMAX, S_XMEAN_SUM, S_NWRAP, S_LOGDET,
-    S_THERM_PROP_U1, S_THERM_ACC_U1, S_THERM_PROP_U2, S_THERM_ACC_U2, S_LEN = 16
+    S_THERM_PROP_U1, S_THERM_ACC_U1, S_THERM_PROP_U2, S_THERM_ACC_U2,
+    // device-clock ticks spent per phase (thread 0 of the chain's workgroup)
+    S_T_SITE, S_T_KIN, S_T_MEAS, S_T_WRAP, S_T_HS, S_LEN = 20
 };
+constexpr int kNumTimers = 5;
+
+constexpr int kLdsWaves = 8;      // LDS path: 512 threads
+constexpr int kLdsMaxN = 96;      // LDS path: N <= 96 (G in LDS, <= 36 MFMA tiles)
+constexpr int kMaxTiles = 5;      // ceil(36 / 8) output tiles per wavefront
 
 struct ModelDev {
     int kind;            // 0 number conserving, 1 Nambu pairing
@@ -41,6 +56,9 @@ struct ModelDev {
     cd alpha[2];
     const double* ep;    // e^{-dtau A}, n x n real row-major
     const double* em;    // e^{+dtau A}
+    // MFMA-operand order: [0] e^{-dtau A} as left factor ([ti][kk][64]),
+    // [1] e^{+dtau A} left, [2] e^{-dtau A} right ([tj][kk][64]), [3] e^{+dtau A} right
+    const double* ez[4];
     const cd* phasek;    // exp(i K . r_imj) indexed by imj
     const int* bonds;    // ns x 3 forward bonds
 };
@@ -68,14 +86,37 @@ struct LaunchArgs {
 };
 
 // ---------------------------------------------------------------- RNG
+constexpr int64_t kMod = 2147483648LL;
+constexpr int64_t kMul = 48828125LL;
 // seed <- 48828125 * seed mod 2^31, u = seed / 2^31 (src/common/random.f90:3-11).
 DEV double ranf(int& seed) {
-    const int64_t s = (48828125LL * (int64_t)seed) % 2147483648LL;
+    const int64_t s = (kMul * (int64_t)seed) % kMod;
     seed = (int)s;
     return (double)s / 2147483648.0;
 }
+// 48828125^k mod 2^31 (jump ahead k draws).
+DEV int64_t lcg_pow(int k) {
+    int64_t r = 1, b = kMul;
+    while (k) {
+        if (k & 1) r = (r * b) % kMod;
+        b = (b * b) % kMod;
+        k >>= 1;
+    }
+    return r;
+}
+DEV int lcg_jump(int seed, int k) { return (int)((lcg_pow(k) * (int64_t)seed) % kMod); }
 
 // ---------------------------------------------------------------- LDS plan
+// One proposal of the local Metropolis step, fixed before the site loop:
+// the RNG draws of a site do not depend on earlier accept decisions.
+struct Proposal {
+    double phi_new;
+    double ratio0;   // Gaussian ratio (times the Nambu normal-ordering constant)
+    double u;        // the uniform the ratio is compared with
+    double pad;
+    cd d0, d1;       // e^{s alpha (phi'-phi)} - 1 of the site's rows
+};
+
 struct Lds {
     QrScratch qr;
     double* dtmp;      // n
@@ -85,28 +126,34 @@ struct Lds {
     cd* row;           // 2 x n
     double* red;       // nwaves x 64
     cd* site;          // 6 x ns (pairing measurement)
+    Proposal* prop;    // ns
+    cd* mat;           // n x n (LDS path only)
 };
 
-HOSTDEV size_t lds_bytes(int n, int nwaves) {
+HOSTDEV size_t lds_align(size_t b) { return (b + 15) & ~(size_t)15; }
+
+HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat) {
     size_t b = 0;
-    b += (size_t)n * sizeof(cd);        // tau
-    b += (size_t)n * sizeof(int) + 16;  // perm (padded)
-    b += (size_t)n * sizeof(double);    // nrm
-    b += 16 * sizeof(double);           // bcast
-    b += (size_t)n * sizeof(double);    // dtmp
-    b += 2 * (size_t)n * sizeof(cd);    // f, finv
-    b += 4 * (size_t)n * sizeof(cd);    // col, row
-    b += (size_t)nwaves * 64 * sizeof(double);
-    b += 6 * (size_t)n * sizeof(cd);
-    return b + 256;
+    b += lds_align((size_t)n * sizeof(cd));           // tau
+    b += lds_align((size_t)n * sizeof(int) + 16);     // perm
+    b += lds_align((size_t)n * sizeof(double));       // nrm
+    b += lds_align(16 * sizeof(double));              // bcast
+    b += lds_align((size_t)n * sizeof(double));       // dtmp
+    b += 2 * lds_align((size_t)n * sizeof(cd));       // f, finv
+    b += 2 * lds_align(2 * (size_t)n * sizeof(cd));   // col, row
+    b += lds_align((size_t)nwaves * 64 * sizeof(double));
+    b += lds_align(6 * (size_t)n * sizeof(cd));
+    b += lds_align((size_t)n * sizeof(Proposal));
+    if (with_mat) b += lds_align((size_t)n * n * sizeof(cd));
+    return b + 64;
 }
 
-DEV Lds carve(char* base, int n, int nwaves) {
+DEV Lds carve(char* base, int n, int nwaves, bool with_mat) {
     Lds L;
     char* p = base;
     auto take = [&](size_t bytes) {
         char* q = p;
-        p += (bytes + 15) & ~(size_t)15;
+        p += lds_align(bytes);
         return q;
     };
     L.qr.tau = (cd*)take((size_t)n * sizeof(cd));
@@ -120,27 +167,23 @@ DEV Lds carve(char* base, int n, int nwaves) {
     L.row = (cd*)take(2 * (size_t)n * sizeof(cd));
     L.red = (double*)take((size_t)nwaves * 64 * sizeof(double));
     L.site = (cd*)take(6 * (size_t)n * sizeof(cd));
+    L.prop = (Proposal*)take((size_t)n * sizeof(Proposal));
+    L.mat = with_mat ? (cd*)take((size_t)n * n * sizeof(cd)) : nullptr;
     return L;
 }
 
-// ---------------------------------------------------------------- helpers
-// Sum NV per-thread values over the workgroup; thread v < NV receives total v.
-template <int NV>
-DEV void wg_reduce(const Ctx& c, double (&v)[NV], double* red, double* out_add) {
-    for (int k = 0; k < NV; ++k) v[k] = wave_sum(c, v[k]);
-    if (c.lane() == 0)
-        for (int k = 0; k < NV; ++k) red[c.wave() * NV + k] = v[k];
-    c.sync();
-    if (c.tid() < NV) {
-        double s = 0.0;
-        for (int w = 0; w < c.nwaves(); ++w) s += red[w * NV + c.tid()];
-        out_add[c.tid()] += s;
+// ---------------------------------------------------------------- timers
+struct Timer {
+    uint64_t t0;
+    DEV void start(const Ctx& c) { t0 = c.clock(); }
+    DEV void stop(const Ctx& c, const ChainDev& ch, int slot) {
+        const uint64_t t1 = c.clock();
+        if (c.tid() == 0) ch.stats[slot] += (double)(t1 - t0);
+        t0 = t1;
     }
-    c.sync();
-}
+};
 
-// Row index of sector `sec` of site i.
-DEV int row_of(const ModelDev& m, int i, int sec) { return sec * m.ns + i; }
+// ---------------------------------------------------------------- helpers
 DEV int nsec(const ModelDev& m) { return m.kind == 1 ? 2 : 1; }
 // Exponent sign of sector `sec` in channel nf (1: relative U1, 2: total U2).
 DEV double sec_sign(const ModelDev& m, int nf, int sec) {
@@ -175,98 +218,104 @@ DEV void scale_mat(const Ctx& c, cd* M, int n, const cd* fl, const cd* fr) {
 }
 
 // ---------------------------------------------------------------- Metropolis
-struct Decision {
-    bool acc;
-    double phi_new;
-    cd m[2][2];   // update kernel: G -= G[:,S] m (I_S - G[S,:])
-};
-
-// LocalU_metro: NC src/number_conserving/src/localU.f90:31-77 and
-// operator_Hubbard.f90:45-68; pairing src/pairing/src/localU.f90:33-113 and
-// src/pairing/src/operator_Hubbard.f90:90-137, in the reduced 2x2 form.
-DEV Decision metro(const ModelDev& m, int nf, double phi_old, int& seed, const cd (&gss)[2][2]) {
-    Decision d;
-    const double xflip = ranf(seed);
-    const double phi_new = phi_old + (xflip - 0.5) * fabs(m.shift_loc);
-    d.phi_new = phi_new;
-    const double rg = exp(-0.5 * phi_new * phi_new) / exp(-0.5 * phi_old * phi_old);
+// Proposals of one channel on one slice (LocalU_metro's draws, NC
+// src/number_conserving/src/localU.f90:44-61 and operator_Hubbard.f90:45-68;
+// pairing src/pairing/src/localU.f90:46-71 and operator_Hubbard.f90:90-137).
+// Site i is visited at position p (ascending: p = i, descending: p = Ns-1-i)
+// and consumes draws 2p (shift) and 2p+1 (acceptance) of the chain stream.
+DEVFN void make_proposals(const Ctx& c, const ModelDev& m, const Lds& L, const double* phi_t, int nf,
+                        bool ascending, int seed) {
     const cd a = m.alpha[nf - 1];
-    double ratio_abs;
-    if (m.kind == 0) {
-        const cd eo = cexp(cd{a.r * phi_old, a.i * phi_old});
-        const cd en = cexp(cd{a.r * phi_new, a.i * phi_new});
-        const cd delta = cdiv(en, eo) - cd{1.0, 0.0};
-        const cd den = cd{1.0, 0.0} + delta * (cd{1.0, 0.0} - gss[0][0]);
-        const cd rdet = cinv(den);
-        const cd z = (cd{rg, 0.0} * rdet) * conj(rdet);
-        ratio_abs = cabs(z);
-        d.m[0][0] = rdet * delta;
-        d.m[0][1] = d.m[1][0] = d.m[1][1] = cd{0.0, 0.0};
-    } else {
-        const double dphi = phi_new - phi_old;
-        const cd ep = cexp(cd{a.r * dphi, a.i * dphi});
-        const cd em = cexp(cd{-a.r * dphi, -a.i * dphi});
-        const cd d0 = ep - cd{1.0, 0.0};
-        const cd d1 = (nf == 2 ? em : ep) - cd{1.0, 0.0};
-        double rconst = 1.0;
-        if (nf == 2) {
+    for (int i = c.tid(); i < m.ns; i += c.nthreads()) {
+        const int pos = ascending ? i : m.ns - 1 - i;
+        int s = lcg_jump(seed, 2 * pos);
+        const double xflip = ranf(s);
+        const double u = ranf(s);
+        const double phi_old = phi_t[i * 2 + nf - 1];
+        const double phi_new = phi_old + (xflip - 0.5) * fabs(m.shift_loc);
+        const double rg = exp(-0.5 * phi_new * phi_new) / exp(-0.5 * phi_old * phi_old);
+        Proposal P;
+        P.phi_new = phi_new;
+        P.u = u;
+        P.pad = 0.0;
+        if (m.kind == 0) {
             const cd eo = cexp(cd{a.r * phi_old, a.i * phi_old});
             const cd en = cexp(cd{a.r * phi_new, a.i * phi_new});
-            rconst = cdiv(eo, en).r;
+            P.d0 = cdiv(en, eo) - cd{1.0, 0.0};
+            P.d1 = cd{0.0, 0.0};
+            P.ratio0 = rg;
+        } else {
+            const double dphi = phi_new - phi_old;
+            const cd ep = cexp(cd{a.r * dphi, a.i * dphi});
+            const cd em = cexp(cd{-a.r * dphi, -a.i * dphi});
+            P.d0 = ep - cd{1.0, 0.0};
+            P.d1 = (nf == 2 ? em : ep) - cd{1.0, 0.0};
+            double rconst = 1.0;
+            if (nf == 2) {
+                const cd eo = cexp(cd{a.r * phi_old, a.i * phi_old});
+                const cd en = cexp(cd{a.r * phi_new, a.i * phi_new});
+                rconst = cdiv(eo, en).r;
+            }
+            P.ratio0 = rg * rconst;
         }
-        const cd p00 = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - gss[0][0]);
-        const cd p01 = -(d0 * gss[0][1]);
-        const cd p10 = -(d1 * gss[1][0]);
-        const cd p11 = cd{1.0, 0.0} + d1 * (cd{1.0, 0.0} - gss[1][1]);
-        const cd det = p00 * p11 - p01 * p10;
-        ratio_abs = fabs(rg * rconst / cabs(det));
-        const cd idet = cinv(det);
-        d.m[0][0] = (p11 * idet) * d0;
-        d.m[0][1] = (-(p01 * idet)) * d1;
-        d.m[1][0] = (-(p10 * idet)) * d0;
-        d.m[1][1] = (p00 * idet) * d1;
+        L.prop[i] = P;
     }
-    const double u = ranf(seed);
-    d.acc = ratio_abs > u;
-    return d;
+    c.sync();
 }
 
-// Sequential local updates of one channel on time slice t (0-based), in
-// ascending (sweep right) or descending (sweep left) site order.  G must
-// already be in the frame in which the reference evaluates the ratio
-// (all sites' HS factors of this channel applied for sweep right; none for
-// sweep left) -- the per-site similarity transforms commute with the rank-k
-// updates and leave G_SS unchanged.
-DEV void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G,
-                   int t, int nf, bool ascending, int& seed) {
+// Sequential Metropolis decisions and Sherman-Morrison updates of one channel
+// on time slice t (0-based) in ascending (sweep right) or descending (sweep
+// left) site order.  G must be in the frame in which the reference evaluates
+// the ratio (all sites' HS factors of this channel applied for sweep right;
+// none for sweep left) -- the per-site similarity transforms commute with the
+// rank-k updates and leave G_SS unchanged.  Advances `seed` by 2 Ns draws.
+DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t,
+                   int nf, bool ascending, int& seed) {
     const int n = m.n, ns = m.ns, k = nsec(m);
     double* phi_t = ch.phi + (size_t)t * ns * 2;
+    make_proposals(c, m, L, phi_t, nf, ascending, seed);
+    seed = lcg_jump(seed, 2 * ns);
+    int nacc = 0;
     for (int q = 0; q < ns; ++q) {
         const int i = ascending ? q : ns - 1 - q;
-        int S[2] = {i, ns + i};
-        cd gss[2][2];
-        for (int a = 0; a < k; ++a)
-            for (int b = 0; b < k; ++b) gss[a][b] = G[S[a] * n + S[b]];
-        if (k == 1) gss[0][1] = gss[1][0] = gss[1][1] = cd{0.0, 0.0};
-        const double phi_old = phi_t[i * 2 + nf - 1];
-        const Decision d = metro(m, nf, phi_old, seed, gss);
-        if (!d.acc) {
-            if (c.tid() == 0) ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += 1.0;
-            continue;
+        const int S[2] = {i, ns + i};
+        const Proposal P = L.prop[i];
+        cd mk[2][2];
+        bool acc;
+        if (k == 1) {
+            const cd g00 = G[i * n + i];
+            const cd den = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
+            const cd rdet = cinv(den);
+            const cd z = (cd{P.ratio0, 0.0} * rdet) * conj(rdet);
+            acc = cabs(z) > P.u;
+            mk[0][0] = rdet * P.d0;
+            mk[0][1] = mk[1][0] = mk[1][1] = cd{0.0, 0.0};
+        } else {
+            const cd g00 = G[S[0] * n + S[0]], g01 = G[S[0] * n + S[1]];
+            const cd g10 = G[S[1] * n + S[0]], g11 = G[S[1] * n + S[1]];
+            const cd p00 = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
+            const cd p01 = -(P.d0 * g01);
+            const cd p10 = -(P.d1 * g10);
+            const cd p11 = cd{1.0, 0.0} + P.d1 * (cd{1.0, 0.0} - g11);
+            const cd det = p00 * p11 - p01 * p10;
+            acc = fabs(P.ratio0 / cabs(det)) > P.u;
+            const cd idet = cinv(det);
+            mk[0][0] = (p11 * idet) * P.d0;
+            mk[0][1] = (-(p01 * idet)) * P.d1;
+            mk[1][0] = (-(p10 * idet)) * P.d0;
+            mk[1][1] = (p00 * idet) * P.d1;
         }
-        // snapshot column(s) and row(s) of the site
+        if (!acc) continue;
+        ++nacc;
+        // snapshot the column(s) and row(s) of the site
         for (int r = c.tid(); r < n; r += c.nthreads())
             for (int a = 0; a < k; ++a) {
                 L.col[a * n + r] = G[r * n + S[a]];
                 L.row[a * n + r] = G[S[a] * n + r];
             }
         c.sync();
-        // every thread has read phi_old and G_SS before this point
-        if (c.tid() == 0) {
-            ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += 1.0;
-            ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += 1.0;
-            phi_t[i * 2 + nf - 1] = d.phi_new;
-        }
+        // every thread has read G_SS before this point
+        if (c.tid() == 0) phi_t[i * 2 + nf - 1] = P.phi_new;
         const int nn = n * n;
         for (int e = c.tid(); e < nn; e += c.nthreads()) {
             const int r = e / n, col = e - r * n;
@@ -276,7 +325,7 @@ DEV void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Ld
                 for (int b = 0; b < k; ++b) {
                     cd rv = -L.row[b * n + col];
                     if (col == S[b]) rv = rv + cd{1.0, 0.0};
-                    w = w + d.m[a][b] * rv;
+                    w = w + mk[a][b] * rv;
                 }
                 x = x - L.col[a * n + r] * w;
             }
@@ -284,16 +333,68 @@ DEV void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Ld
         }
         c.sync();
     }
+    if (c.tid() == 0) {
+        ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
+        ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
+    }
+}
+
+// ---------------------------------------------------------------- MFMA GEMM in LDS
+// M (LDS, n x n complex) <- E M (left) or M E (right) with E real, read in
+// MFMA operand order from global memory (coalesced 512 B per k-step).  Each
+// wavefront keeps its output tiles in registers until every read of M is done.
+DEVFN void lds_gemm(const Ctx& c, cd* M, const double* Ez, int n, bool left) {
+    const int nt = (n + 15) >> 4, nk = (n + 3) >> 2, ntt = nt * nt;
+    const int lane = c.lane(), li = lane & 15, lk = lane >> 4, w = c.wave(), nw = c.nwaves();
+    d4 ar[kMaxTiles], ai[kMaxTiles];
+    for (int q = 0; q < kMaxTiles; ++q) {
+        ar[q] = d4{0.0, 0.0, 0.0, 0.0};
+        ai[q] = d4{0.0, 0.0, 0.0, 0.0};
+        const int t = w + q * nw;
+        if (t < ntt) {
+            const int ti = t / nt, tj = t - ti * nt;
+            if (left) {
+                const int col = 16 * tj + li;
+                const double* ez = Ez + (size_t)ti * nk * 64 + lane;
+                for (int kk = 0; kk < nk; ++kk) {
+                    const int k = 4 * kk + lk;
+                    const double a = ez[(size_t)kk * 64];
+                    const cd b = (k < n && col < n) ? M[k * n + col] : cd{0.0, 0.0};
+                    ar[q] = c.mfma(a, b.r, ar[q]);
+                    ai[q] = c.mfma(a, b.i, ai[q]);
+                }
+            } else {
+                const int row = 16 * ti + li;
+                const double* ez = Ez + (size_t)tj * nk * 64 + lane;
+                for (int kk = 0; kk < nk; ++kk) {
+                    const int k = 4 * kk + lk;
+                    const cd a = (row < n && k < n) ? M[row * n + k] : cd{0.0, 0.0};
+                    const double b = ez[(size_t)kk * 64];
+                    ar[q] = c.mfma(a.r, b, ar[q]);
+                    ai[q] = c.mfma(a.i, b, ai[q]);
+                }
+            }
+        }
+    }
+    c.sync();
+    for (int q = 0; q < kMaxTiles; ++q) {
+        const int t = w + q * nw;
+        if (t < ntt) {
+            const int ti = t / nt, tj = t - ti * nt;
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
+                if (row < n && col < n) M[row * n + col] = cd{ar[q][r], ai[q][r]};
+            }
+        }
+    }
+    c.sync();
 }
 
 // ---------------------------------------------------------------- stabilization
+// QRP of A (already holding U D or U^H D) and the new factors; A may be LDS.
 // stab_UR (src/number_conserving/src/stabilization.f90:69-93).
-DEV void stab_ur(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+DEVFN void stab_ur_from(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* A) {
     const int n = m.n;
-    cd* A = ch.wa;
-    const int nn = n * n;
-    for (int e = c.tid(); e < nn; e += c.nthreads()) A[e] = ch.ur[e] * cd{ch.dr[e % n], 0.0};
-    c.sync();
     qrp(c, A, n, L.qr, L.dtmp);
     trmm_perm_rows(c, ch.w1, A, ch.vr, L.qr.perm, n);
     mat_copy(c, ch.vr, ch.w1, n, false);
@@ -302,15 +403,8 @@ DEV void stab_ur(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds&
 }
 
 // stab_UL (stabilization.f90:95-119).
-DEV void stab_ul(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+DEVFN void stab_ul_from(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* A) {
     const int n = m.n;
-    cd* A = ch.wa;
-    const int nn = n * n;
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
-        const int r = e / n, k = e - r * n;
-        A[e] = conj(ch.ul[k * n + r]) * cd{ch.dl[k], 0.0};
-    }
-    c.sync();
     qrp(c, A, n, L.qr, L.dtmp);
     perm_cols_times_vh(c, ch.w1, ch.vl, A, L.qr.perm, n);
     mat_copy(c, ch.vl, ch.w1, n, false);
@@ -319,16 +413,26 @@ DEV void stab_ul(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds&
     vec_copy(c, ch.dl, L.dtmp, n);
 }
 
+// A <- U diag(D) (conj_t = false) or U^H diag(D) (conj_t = true).
+DEV void load_ud(const Ctx& c, cd* A, const cd* U, const double* D, int n, bool conj_t) {
+    const int nn = n * n;
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int r = e / n, k = e - r * n;
+        const cd u = conj_t ? conj(U[k * n + r]) : U[e];
+        A[e] = u * cd{D[k], 0.0};
+    }
+    c.sync();
+}
+
 // stab_green (stabilization.f90:121-190): Gout = (1 - UR DR VR VL DL UL)^{-1}.
 // For nt > Ltrot/2 the reference decomposes M^H; then G^H = UR P V^{-1} D^{-1} Q^H UL
 // has the same structure as the nt <= Ltrot/2 branch with (UL^H, UR^H) -> (UR, UL).
-// Returns log|det G| = -sum log D.
-DEV double stab_green(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt,
+// A is the QR work matrix (LDS or global).  Returns log|det G| = -sum log D.
+DEVFN double stab_green(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt,
                       const cd* ur, const double* dr, const cd* vr, const cd* ul, const double* dl,
-                      const cd* vl, cd* Gout) {
+                      const cd* vl, cd* A, cd* Gout) {
     const int n = m.n, nn = n * n;
     const bool upper = nt > m.ltrot / 2;
-    cd* A = ch.wa;
     gemm_cc(c, ch.w1, vr, false, vl, false, n);          // VR VL
     gemm_cc(c, ch.w2, ur, true, ul, true, n);            // UR^H UL^H
     for (int e = c.tid(); e < nn; e += c.nthreads()) {
@@ -369,16 +473,15 @@ DEV double stab_green(const Ctx& c, const ModelDev& m, const ChainDev& ch, const
     return -s;
 }
 
+// Drift of the wrapped G against the recomputed one (compare_mat,
+// stabilization.f90:256-266) and the last log|det G|.
 DEV void store_drift_and_logdet(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L,
-                                const cd* Gnew, double logdet) {
+                                const cd* Gnew, const cd* Gold, double logdet) {
     const int nn = m.n * m.n;
-    double v[1] = {0.0};
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
-        const cd d = Gnew[e] - ch.g[e];
-        v[0] += cabs(d);
-    }
-    v[0] = wave_sum(c, v[0]);
-    if (c.lane() == 0) L.red[c.wave()] = v[0];
+    double v = 0.0;
+    for (int e = c.tid(); e < nn; e += c.nthreads()) v += cabs(Gnew[e] - Gold[e]);
+    v = wave_sum(c, v);
+    if (c.lane() == 0) L.red[c.wave()] = v;
     c.sync();
     if (c.tid() == 0) {
         double s = 0.0;
@@ -394,36 +497,24 @@ DEV void store_drift_and_logdet(const Ctx& c, const ModelDev& m, const ChainDev&
 DEV cd* list_mat(cd* base, int st, int n) { return base + (size_t)st * n * n; }
 DEV double* list_vec(double* base, int st, int n) { return base + (size_t)st * n; }
 
-// Wrap_R (stabilization.f90:330-366), nt % Nwrap == 0, nt != 0.
-DEV void wrap_r(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
-    const int n = m.n, st = nt / m.nwrap;
-    stab_ur(c, m, ch, L);
-    const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, list_mat(ch.ullist, st, n),
-                                 list_vec(ch.dllist, st, n), list_mat(ch.vllist, st, n), ch.wa);
-    store_drift_and_logdet(c, m, ch, L, ch.wa, ld);
-    mat_copy(c, ch.g, ch.wa, n, false);
+DEV void live_identity(const Ctx& c, int n, cd* u, double* d, cd* v) {
+    mat_identity(c, u, n);
+    mat_identity(c, v, n);
+    vec_fill(c, d, n, 1.0);
+}
+
+DEV void store_right(const Ctx& c, const ModelDev& m, const ChainDev& ch, int st) {
+    const int n = m.n;
     mat_copy(c, list_mat(ch.urlist, st, n), ch.ur, n, false);
     mat_copy(c, list_mat(ch.vrlist, st, n), ch.vr, n, false);
     vec_copy(c, list_vec(ch.drlist, st, n), ch.dr, n);
 }
 
-// Wrap_L (stabilization.f90:292-328), nt % Nwrap == 0, nt != Ltrot.
-DEV void wrap_l(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
-    const int n = m.n, st = nt / m.nwrap;
-    stab_ul(c, m, ch, L);
-    const double ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
-                                 list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, ch.wa);
-    store_drift_and_logdet(c, m, ch, L, ch.wa, ld);
-    mat_copy(c, ch.g, ch.wa, n, false);
+DEV void store_left(const Ctx& c, const ModelDev& m, const ChainDev& ch, int st) {
+    const int n = m.n;
     mat_copy(c, list_mat(ch.ullist, st, n), ch.ul, n, false);
     mat_copy(c, list_mat(ch.vllist, st, n), ch.vl, n, false);
     vec_copy(c, list_vec(ch.dllist, st, n), ch.dl, n);
-}
-
-DEV void live_identity(const Ctx& c, int n, cd* u, double* d, cd* v) {
-    mat_identity(c, u, n);
-    mat_identity(c, v, n);
-    vec_fill(c, d, n, 1.0);
 }
 
 // ---------------------------------------------------------------- measurement
@@ -436,9 +527,8 @@ DEV int imj_index(const ModelDev& m, int i, int j) {
 }
 
 // NC estimator, src/number_conserving/src/obser_equal.f90:95-198.
-DEV void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+DEVFN void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const cd* G) {
     const int n = m.n, nn = n * n;
-    const cd* G = ch.g;
     const double lq = (double)m.ns, norm = 1.0 / (lq * lq);
     enum { NV = 22 };
     double v[NV];
@@ -485,11 +575,9 @@ DEV void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const L
         // Grupc(i,j)+Grupc(j,i)+Grdoc(i,j)+Grdoc(j,i), i != j
         v[19] += (gji + gij + conj(gji) + conj(gij)).r / lq;
     }
-    double tot[NV];
-    for (int k = 0; k < NV; ++k) tot[k] = v[k];
-    for (int k = 0; k < NV; ++k) tot[k] = wave_sum(c, tot[k]);
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(c, v[k]);
     if (c.lane() == 0)
-        for (int k = 0; k < NV; ++k) L.red[c.wave() * NV + k] = tot[k];
+        for (int k = 0; k < NV; ++k) L.red[c.wave() * NV + k] = v[k];
     c.sync();
     if (c.tid() == 0) {
         double s[NV];
@@ -531,12 +619,10 @@ DEV void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const L
 //   G_b_b+ = Xbb, G_b_c = -Xbc, G_c+_b+ = Xcb, G_c+_c = -Xcc,
 //   G_c_c+ = conj Xbb, G_c_b = -conj Xbc, G_b+_c+ = conj Xcb, G_b+_b = -conj Xcc,
 // and the remaining eight blocks zero.
-DEV void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+DEVFN void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const cd* G) {
     const int n = m.n, ns = m.ns;
-    const cd* G = ch.g;
     const double nsd = (double)ns, norm = 1.0 / (nsd * nsd);
     auto X = [&](int bi, int bj, int i, int j) { return G[(bi * ns + i) * n + bj * ns + j]; };
-    // per-site: nb, nc, Xbb_ii, Xbc_ii, Xcb_ii
     cd* nb = L.site;
     cd* ncv = L.site + ns;
     cd* xbb = L.site + 2 * ns;
@@ -646,21 +732,42 @@ DEV void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
     c.sync();
 }
 
-DEV void measure(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
-    if (m.kind == 0) measure_nc(c, m, ch, L);
-    else measure_pairing(c, m, ch, L);
+DEV void measure(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const cd* G) {
+    if (m.kind == 0) measure_nc(c, m, ch, L, G);
+    else measure_pairing(c, m, ch, L, G);
 }
 
-// ---------------------------------------------------------------- sweeps
-// Local_sweep_pre (local_sweep.f90:80-94): UR = B(beta, 0) with checkpoints,
-// G(beta) from stab_green at nt = Ltrot.
-DEV void prepare_chain(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+// ====================================================================== global path
+// Wrap_R (stabilization.f90:330-366), nt % Nwrap == 0, nt != 0.
+DEVFN void wrap_r_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
+    const int n = m.n, st = nt / m.nwrap;
+    load_ud(c, ch.wa, ch.ur, ch.dr, n, false);
+    stab_ur_from(c, m, ch, L, ch.wa);
+    const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, list_mat(ch.ullist, st, n),
+                                 list_vec(ch.dllist, st, n), list_mat(ch.vllist, st, n), ch.wa, ch.wa);
+    store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
+    mat_copy(c, ch.g, ch.wa, n, false);
+    store_right(c, m, ch, st);
+}
+
+// Wrap_L (stabilization.f90:292-328), nt % Nwrap == 0, nt != Ltrot.
+DEVFN void wrap_l_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
+    const int n = m.n, st = nt / m.nwrap;
+    load_ud(c, ch.wa, ch.ul, ch.dl, n, true);
+    stab_ul_from(c, m, ch, L, ch.wa);
+    const double ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
+                                 list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, ch.wa, ch.wa);
+    store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
+    mat_copy(c, ch.g, ch.wa, n, false);
+    store_left(c, m, ch, st);
+}
+
+DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
     const int n = m.n;
     live_identity(c, n, ch.ur, ch.dr, ch.vr);
     live_identity(c, n, ch.ul, ch.dl, ch.vl);
-    live_identity(c, n, list_mat(ch.urlist, 0, n), list_vec(ch.drlist, 0, n), list_mat(ch.vrlist, 0, n));
-    live_identity(c, n, list_mat(ch.ullist, m.nst, n), list_vec(ch.dllist, m.nst, n),
-                  list_mat(ch.vllist, m.nst, n));
+    store_right(c, m, ch, 0);
+    store_left(c, m, ch, m.nst);
     for (int nt = 1; nt <= m.ltrot; ++nt) {
         const double* phi_t = ch.phi + (size_t)(nt - 1) * m.ns * 2;
         for (int nf = 1; nf <= 2; ++nf) {
@@ -671,48 +778,59 @@ DEV void prepare_chain(const Ctx& c, const ModelDev& m, const ChainDev& ch, cons
         gemm_rc(c, ch.w1, m.ep, ch.ur, 0, n);
         mat_copy(c, ch.ur, ch.w1, n, false);
         if (nt % m.nwrap == 0) {
-            const int st = nt / m.nwrap;
-            stab_ur(c, m, ch, L);
-            mat_copy(c, list_mat(ch.urlist, st, n), ch.ur, n, false);
-            mat_copy(c, list_mat(ch.vrlist, st, n), ch.vr, n, false);
-            vec_copy(c, list_vec(ch.drlist, st, n), ch.dr, n);
-            if (nt == m.ltrot) {
-                const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, ch.ul, ch.dl, ch.vl, ch.g);
-                if (c.tid() == 0) ch.stats[S_LOGDET] = ld;
-                c.sync();
-            }
+            load_ud(c, ch.wa, ch.ur, ch.dr, n, false);
+            stab_ur_from(c, m, ch, L, ch.wa);
+            store_right(c, m, ch, nt / m.nwrap);
         }
     }
+    const double ld = stab_green(c, m, ch, L, m.ltrot, ch.ur, ch.dr, ch.vr, ch.ul, ch.dl, ch.vl, ch.wa, ch.g);
+    if (c.tid() == 0) ch.stats[S_LOGDET] = ld;
+    c.sync();
 }
 
-DEV void sweep_left(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed, bool meas) {
+DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
+                           bool meas) {
     const int n = m.n;
+    Timer tm;
     live_identity(c, n, ch.ul, ch.dl, ch.vl);     // Wrap_L(Ltrot): L list slot Nst is the identity
+    tm.start(c);
     for (int nt = m.ltrot; nt >= 1; --nt) {
-        if (nt % m.nwrap == 0 && nt != m.ltrot) wrap_l(c, m, ch, L, nt);
-        if (meas) measure(c, m, ch, L);
+        if (nt % m.nwrap == 0 && nt != m.ltrot) {
+            wrap_l_global(c, m, ch, L, nt);
+            tm.stop(c, ch, S_T_WRAP);
+        }
+        if (meas) {
+            measure(c, m, ch, L, ch.g);
+            tm.stop(c, ch, S_T_MEAS);
+        }
         // propT_L: G <- e^{+dtau K} G e^{-dtau K}, UL <- UL e^{-dtau K}
         gemm_rc(c, ch.w1, m.ep, ch.g, 1, n);
         gemm_rc(c, ch.g, m.em, ch.w1, 0, n);
         gemm_rc(c, ch.w1, m.ep, ch.ul, 1, n);
         mat_copy(c, ch.ul, ch.w1, n, false);
+        tm.stop(c, ch, S_T_KIN);
         const int t = nt - 1;
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
             site_loop(c, m, ch, L, ch.g, t, nf, false, seed);
+            tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.g, n, L.finv, L.f);      // G <- e^{-V} G e^{V}
             scale_mat(c, ch.ul, n, nullptr, L.f);    // UL <- UL e^{V}
+            tm.stop(c, ch, S_T_HS);
         }
     }
-    // Wrap_L(0)
-    wrap_l(c, m, ch, L, 0);
+    wrap_l_global(c, m, ch, L, 0);
+    tm.stop(c, ch, S_T_WRAP);
 }
 
-DEV void sweep_right(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed, bool meas) {
+DEV void sweep_right_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
+                            bool meas) {
     const int n = m.n;
+    Timer tm;
     live_identity(c, n, ch.ur, ch.dr, ch.vr);     // Wrap_R(0): R list slot 0 is the identity
+    tm.start(c);
     for (int nt = 1; nt <= m.ltrot; ++nt) {
         const int t = nt - 1;
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
@@ -720,41 +838,209 @@ DEV void sweep_right(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
             if (!m.active[nf - 1]) continue;
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.g, n, L.f, L.finv);      // G <- e^{V} G e^{-V} (old fields)
+            tm.stop(c, ch, S_T_HS);
             site_loop(c, m, ch, L, ch.g, t, nf, true, seed);
+            tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.ur, n, L.f, nullptr);    // UR <- e^{V} UR (new fields)
+            tm.stop(c, ch, S_T_HS);
         }
         // propT_R: G <- e^{-dtau K} G e^{+dtau K}, UR <- e^{-dtau K} UR
         gemm_rc(c, ch.w1, m.ep, ch.g, 0, n);
         gemm_rc(c, ch.g, m.em, ch.w1, 1, n);
         gemm_rc(c, ch.w1, m.ep, ch.ur, 0, n);
         mat_copy(c, ch.ur, ch.w1, n, false);
-        if (nt % m.nwrap == 0) wrap_r(c, m, ch, L, nt);
-        if (meas) measure(c, m, ch, L);
+        tm.stop(c, ch, S_T_KIN);
+        if (nt % m.nwrap == 0) {
+            wrap_r_global(c, m, ch, L, nt);
+            tm.stop(c, ch, S_T_WRAP);
+        }
+        if (meas) {
+            measure(c, m, ch, L, ch.g);
+            tm.stop(c, ch, S_T_MEAS);
+        }
+    }
+}
+
+// ====================================================================== LDS path
+// Right factor products of slices [t0, t1] (1-based, ascending) applied to the
+// matrix in LDS: M <- B_t1 ... B_t0 M, B_t = e^{-dtau K} e^{V_1(t)} e^{V_2(t)}
+// in the reference's order (propU_R U1 then U2, then propT_R; multiply.f90:80-86).
+DEVFN void apply_slices_right(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* M,
+                            int t0, int t1) {
+    for (int nt = t0; nt <= t1; ++nt) {
+        const double* phi_t = ch.phi + (size_t)(nt - 1) * m.ns * 2;
+        for (int nf = 1; nf <= 2; ++nf) {
+            if (!m.active[nf - 1]) continue;
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, M, m.n, L.f, nullptr);
+        }
+        lds_gemm(c, M, m.ez[0], m.n, true);
+    }
+}
+
+// Left factor products of slices t1 down to t0 (sweep-left order):
+// M <- M B_t1 ... B_t0 with M B = (M e^{-dtau K}) e^{V_2} e^{V_1} (propT_L, then U2, U1).
+DEVFN void apply_slices_left(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* M,
+                           int t0, int t1) {
+    for (int nt = t1; nt >= t0; --nt) {
+        lds_gemm(c, M, m.ez[2], m.n, false);
+        const double* phi_t = ch.phi + (size_t)(nt - 1) * m.ns * 2;
+        for (int nf = 2; nf >= 1; --nf) {
+            if (!m.active[nf - 1]) continue;
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, M, m.n, nullptr, L.f);
+        }
+    }
+}
+
+// Wrap_R at nt with G in LDS (L.mat): UR <- B(nt, nt-Nwrap] UR, stab_UR, stab_green.
+DEVFN void wrap_r_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
+    const int n = m.n, st = nt / m.nwrap;
+    mat_copy(c, ch.g, L.mat, n, false);                 // wrapped G, for the drift check
+    mat_copy(c, L.mat, ch.ur, n, false);
+    apply_slices_right(c, m, ch, L, L.mat, nt - m.nwrap + 1, nt);
+    scale_mat_cols_real(c, L.mat, n, ch.dr);
+    stab_ur_from(c, m, ch, L, L.mat);
+    const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, list_mat(ch.ullist, st, n),
+                                 list_vec(ch.dllist, st, n), list_mat(ch.vllist, st, n), L.mat, ch.wa);
+    store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
+    mat_copy(c, L.mat, ch.wa, n, false);
+    store_right(c, m, ch, st);
+}
+
+// Wrap_L at nt with G in LDS: UL <- UL B(nt+Nwrap, nt], stab_UL, stab_green.
+DEVFN void wrap_l_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
+    const int n = m.n, st = nt / m.nwrap;
+    mat_copy(c, ch.g, L.mat, n, false);
+    mat_copy(c, L.mat, ch.ul, n, false);
+    apply_slices_left(c, m, ch, L, L.mat, nt + 1, nt + m.nwrap);
+    mat_copy(c, ch.ul, L.mat, n, false);
+    load_ud(c, L.mat, ch.ul, ch.dl, n, true);
+    stab_ul_from(c, m, ch, L, L.mat);
+    const double ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
+                                 list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, L.mat, ch.wa);
+    store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
+    mat_copy(c, L.mat, ch.wa, n, false);
+    store_left(c, m, ch, st);
+}
+
+DEV void prepare_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+    const int n = m.n;
+    live_identity(c, n, ch.ur, ch.dr, ch.vr);
+    live_identity(c, n, ch.ul, ch.dl, ch.vl);
+    store_right(c, m, ch, 0);
+    store_left(c, m, ch, m.nst);
+    for (int st = 1; st <= m.nst; ++st) {
+        mat_copy(c, L.mat, ch.ur, n, false);
+        apply_slices_right(c, m, ch, L, L.mat, (st - 1) * m.nwrap + 1, st * m.nwrap);
+        scale_mat_cols_real(c, L.mat, n, ch.dr);
+        stab_ur_from(c, m, ch, L, L.mat);
+        store_right(c, m, ch, st);
+    }
+    const double ld = stab_green(c, m, ch, L, m.ltrot, ch.ur, ch.dr, ch.vr, ch.ul, ch.dl, ch.vl, L.mat, ch.g);
+    if (c.tid() == 0) ch.stats[S_LOGDET] = ld;
+    c.sync();
+}
+
+DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
+                        bool meas) {
+    const int n = m.n;
+    cd* G = L.mat;
+    Timer tm;
+    live_identity(c, n, ch.ul, ch.dl, ch.vl);     // Wrap_L(Ltrot): L list slot Nst is the identity
+    tm.start(c);
+    for (int nt = m.ltrot; nt >= 1; --nt) {
+        if (nt % m.nwrap == 0 && nt != m.ltrot) {
+            wrap_l_lds(c, m, ch, L, nt);
+            tm.stop(c, ch, S_T_WRAP);
+        }
+        if (meas) {
+            measure(c, m, ch, L, G);
+            tm.stop(c, ch, S_T_MEAS);
+        }
+        // propT_L: G <- e^{+dtau K} (G e^{-dtau K}); UL's product is applied at the next wrap
+        lds_gemm(c, G, m.ez[2], n, false);
+        lds_gemm(c, G, m.ez[1], n, true);
+        tm.stop(c, ch, S_T_KIN);
+        const int t = nt - 1;
+        const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
+        for (int nf = 2; nf >= 1; --nf) {
+            if (!m.active[nf - 1]) continue;
+            site_loop(c, m, ch, L, G, t, nf, false, seed);
+            tm.stop(c, ch, S_T_SITE);
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, G, n, L.finv, L.f);         // G <- e^{-V} G e^{V}
+            tm.stop(c, ch, S_T_HS);
+        }
+    }
+    wrap_l_lds(c, m, ch, L, 0);
+    tm.stop(c, ch, S_T_WRAP);
+}
+
+DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
+                         bool meas) {
+    const int n = m.n;
+    cd* G = L.mat;
+    Timer tm;
+    live_identity(c, n, ch.ur, ch.dr, ch.vr);     // Wrap_R(0): R list slot 0 is the identity
+    tm.start(c);
+    for (int nt = 1; nt <= m.ltrot; ++nt) {
+        const int t = nt - 1;
+        const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
+        for (int nf = 1; nf <= 2; ++nf) {
+            if (!m.active[nf - 1]) continue;
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, G, n, L.f, L.finv);         // G <- e^{V} G e^{-V} (old fields)
+            tm.stop(c, ch, S_T_HS);
+            site_loop(c, m, ch, L, G, t, nf, true, seed);
+            tm.stop(c, ch, S_T_SITE);
+        }
+        // propT_R: G <- (e^{-dtau K} G) e^{+dtau K}; UR's product is applied at the wrap
+        lds_gemm(c, G, m.ez[0], n, true);
+        lds_gemm(c, G, m.ez[3], n, false);
+        tm.stop(c, ch, S_T_KIN);
+        if (nt % m.nwrap == 0) {
+            wrap_r_lds(c, m, ch, L, nt);
+            tm.stop(c, ch, S_T_WRAP);
+        }
+        if (meas) {
+            measure(c, m, ch, L, G);
+            tm.stop(c, ch, S_T_MEAS);
+        }
     }
 }
 
 // ---------------------------------------------------------------- kernel bodies
+template <bool kLds>
 DEV void body_prepare(const Ctx& c, const LaunchArgs& a) {
     for (int q = c.block(); q < a.count; q += c.nblocks()) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves());
-        prepare_chain(c, m, ch, L);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds);
+        if (kLds) prepare_lds(c, m, ch, L);
+        else prepare_global(c, m, ch, L);
     }
 }
 
+template <bool kLds>
 DEV void body_sweep(const Ctx& c, const LaunchArgs& a) {
     for (int q = c.block(); q < a.count; q += c.nblocks()) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves());
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds);
         int seed = *ch.seed;
+        if (kLds) mat_copy(c, L.mat, ch.g, m.n, false);
         for (int s = 0; s < a.nsweep; ++s) {
-            sweep_left(c, m, ch, L, seed, a.measure != 0);
-            sweep_right(c, m, ch, L, seed, a.measure != 0);
+            if (kLds) {
+                sweep_left_lds(c, m, ch, L, seed, a.measure != 0);
+                sweep_right_lds(c, m, ch, L, seed, a.measure != 0);
+            } else {
+                sweep_left_global(c, m, ch, L, seed, a.measure != 0);
+                sweep_right_global(c, m, ch, L, seed, a.measure != 0);
+            }
         }
-        c.sync();
+        if (kLds) mat_copy(c, ch.g, L.mat, m.n, false);
         if (c.tid() == 0) *ch.seed = seed;
         c.sync();
     }

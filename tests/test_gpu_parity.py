@@ -90,6 +90,10 @@ CASES = [
     dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=6, nly=6, ltrot=20, beta=2.0, nwrap=10, shift_loc=1.5),
     dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=20, beta=2.0, nwrap=10,
          shift_loc=1.0),
+    # beyond the LDS-resident size (N > 96): global-memory path
+    dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=12, nly=12, ltrot=10, beta=1.0, nwrap=5, shift_loc=1.5),
+    dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12, ltrot=4, beta=0.4, nwrap=2,
+         shift_loc=1.0),
 ]
 
 
@@ -174,6 +178,13 @@ def check_golden(lib, name):
             if len(seeds) == 1:
                 np.testing.assert_allclose(lw, z["out_log_weight"][-1][0], rtol=1e-10)
     s.close()
+
+
+@pytest.mark.parametrize("name", ["nc_traj_3x3_u2", "pairing_traj_3x3", "pairing_traj_6x6_c2"])
+def test_global_memory_path_matches_reference(hip_lib, name, monkeypatch):
+    """The global-memory path (used for N > 96) on the golden cases."""
+    monkeypatch.setenv("BAFQ_FORCE_GLOBAL", "1")
+    check_golden(hip_lib, name)
 
 
 def test_uneven_batch_equals_single_chains(hip_lib):
