@@ -270,29 +270,39 @@ DEVFN void make_proposals(const Ctx& c, const ModelDev& m, const Lds& L, const d
 // none for sweep left) -- the per-site similarity transforms commute with the
 // rank-k updates and leave G_SS unchanged.  Advances `seed` by 2 Ns draws.
 DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t,
-                   int nf, bool ascending, int& seed) {
-    const int n = m.n, ns = m.ns, k = nsec(m);
+                     int nf, bool ascending, int& seed) {
+    const int n = m.n, ns = m.ns;
+    const bool pairing = m.kind == 1;
+    const int tid = c.tid(), nth = c.nthreads();
     double* phi_t = ch.phi + (size_t)t * ns * 2;
     make_proposals(c, m, L, phi_t, nf, ascending, seed);
     seed = lcg_jump(seed, 2 * ns);
+    cd* col0 = L.col;
+    cd* col1 = L.col + n;
+    cd* row0 = L.row;
+    cd* row1 = L.row + n;
+    // fixed-column mapping of the rank-k update: thread -> column cc, rows r0, r0 + rstep, ...
+    const bool fixed_col = n <= nth;
+    const int rstep = fixed_col ? nth / n : 1;
+    const int cc = fixed_col ? tid % n : 0;
+    const int r0 = fixed_col ? tid / n : 0;
     int nacc = 0;
     for (int q = 0; q < ns; ++q) {
         const int i = ascending ? q : ns - 1 - q;
-        const int S[2] = {i, ns + i};
+        const int s0 = i, s1 = ns + i;
         const Proposal P = L.prop[i];
-        cd mk[2][2];
+        cd m00, m01 = cd{0.0, 0.0}, m10 = cd{0.0, 0.0}, m11 = cd{0.0, 0.0};
         bool acc;
-        if (k == 1) {
-            const cd g00 = G[i * n + i];
+        if (!pairing) {
+            const cd g00 = G[s0 * n + s0];
             const cd den = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
             const cd rdet = cinv(den);
             const cd z = (cd{P.ratio0, 0.0} * rdet) * conj(rdet);
             acc = cabs(z) > P.u;
-            mk[0][0] = rdet * P.d0;
-            mk[0][1] = mk[1][0] = mk[1][1] = cd{0.0, 0.0};
+            m00 = rdet * P.d0;
         } else {
-            const cd g00 = G[S[0] * n + S[0]], g01 = G[S[0] * n + S[1]];
-            const cd g10 = G[S[1] * n + S[0]], g11 = G[S[1] * n + S[1]];
+            const cd g00 = G[s0 * n + s0], g01 = G[s0 * n + s1];
+            const cd g10 = G[s1 * n + s0], g11 = G[s1 * n + s1];
             const cd p00 = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
             const cd p01 = -(P.d0 * g01);
             const cd p10 = -(P.d1 * g10);
@@ -300,40 +310,47 @@ DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
             const cd det = p00 * p11 - p01 * p10;
             acc = fabs(P.ratio0 / cabs(det)) > P.u;
             const cd idet = cinv(det);
-            mk[0][0] = (p11 * idet) * P.d0;
-            mk[0][1] = (-(p01 * idet)) * P.d1;
-            mk[1][0] = (-(p10 * idet)) * P.d0;
-            mk[1][1] = (p00 * idet) * P.d1;
+            m00 = (p11 * idet) * P.d0;
+            m01 = (-(p01 * idet)) * P.d1;
+            m10 = (-(p10 * idet)) * P.d0;
+            m11 = (p00 * idet) * P.d1;
         }
         if (!acc) continue;
         ++nacc;
         // snapshot the column(s) and row(s) of the site
-        for (int r = c.tid(); r < n; r += c.nthreads())
-            for (int a = 0; a < k; ++a) {
-                L.col[a * n + r] = G[r * n + S[a]];
-                L.row[a * n + r] = G[S[a] * n + r];
+        for (int r = tid; r < n; r += nth) {
+            col0[r] = G[r * n + s0];
+            row0[r] = G[s0 * n + r];
+            if (pairing) {
+                col1[r] = G[r * n + s1];
+                row1[r] = G[s1 * n + r];
             }
+        }
         c.sync();
         // every thread has read G_SS before this point
-        if (c.tid() == 0) phi_t[i * 2 + nf - 1] = P.phi_new;
-        const int nn = n * n;
-        for (int e = c.tid(); e < nn; e += c.nthreads()) {
-            const int r = e / n, col = e - r * n;
-            cd x = G[e];
-            for (int a = 0; a < k; ++a) {
-                cd w = cd{0.0, 0.0};
-                for (int b = 0; b < k; ++b) {
-                    cd rv = -L.row[b * n + col];
-                    if (col == S[b]) rv = rv + cd{1.0, 0.0};
-                    w = w + mk[a][b] * rv;
-                }
-                x = x - L.col[a * n + r] * w;
+        if (tid == 0) phi_t[i * 2 + nf - 1] = P.phi_new;
+        // G -= G[:,S] M (I_S - G[S,:])
+        for (int cb = fixed_col ? cc : tid; cb < n; cb += fixed_col ? n : nth) {
+            if (fixed_col && r0 >= rstep) break;
+            cd e0 = -row0[cb];
+            if (cb == s0) e0 = e0 + cd{1.0, 0.0};
+            cd w0 = m00 * e0, w1 = cd{0.0, 0.0};
+            if (pairing) {
+                cd e1 = -row1[cb];
+                if (cb == s1) e1 = e1 + cd{1.0, 0.0};
+                w0 = w0 + m01 * e1;
+                w1 = m10 * e0 + m11 * e1;
             }
-            G[e] = x;
+            for (int r = fixed_col ? r0 : 0; r < n; r += rstep) {
+                cd x = G[r * n + cb];
+                x = x - col0[r] * w0;
+                if (pairing) x = x - col1[r] * w1;
+                G[r * n + cb] = x;
+            }
         }
         c.sync();
     }
-    if (c.tid() == 0) {
+    if (tid == 0) {
         ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
         ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
     }
@@ -347,6 +364,7 @@ DEVFN void lds_gemm(const Ctx& c, cd* M, const double* Ez, int n, bool left) {
     const int nt = (n + 15) >> 4, nk = (n + 3) >> 2, ntt = nt * nt;
     const int lane = c.lane(), li = lane & 15, lk = lane >> 4, w = c.wave(), nw = c.nwaves();
     d4 ar[kMaxTiles], ai[kMaxTiles];
+#pragma unroll
     for (int q = 0; q < kMaxTiles; ++q) {
         ar[q] = d4{0.0, 0.0, 0.0, 0.0};
         ai[q] = d4{0.0, 0.0, 0.0, 0.0};
@@ -377,6 +395,7 @@ DEVFN void lds_gemm(const Ctx& c, cd* M, const double* Ez, int n, bool left) {
         }
     }
     c.sync();
+#pragma unroll
     for (int q = 0; q < kMaxTiles; ++q) {
         const int t = w + q * nw;
         if (t < ntt) {
