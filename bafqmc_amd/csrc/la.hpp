@@ -43,6 +43,32 @@ HOSTDEV cd cexp(cd z) {
     return cd{e * cos(z.i), e * sin(z.i)};
 }
 
+// ------------------------------------------------------------ LDS pointers
+// Pointers known to address LDS are cast to the LDS address space so the
+// compiler emits ds_read/ds_write instead of flat accesses.  Complex values are
+// moved through their components (no struct copy across address spaces).
+#ifndef BAFQ_EMU
+#define LDS_AS __attribute__((address_space(3)))
+#else
+#define LDS_AS
+#endif
+template <class T>
+DEV LDS_AS T* as_lds(T* p) { return (LDS_AS T*)(p); }
+#ifndef BAFQ_EMU
+#define GLB_AS __attribute__((address_space(1)))
+#else
+#define GLB_AS
+#endif
+template <class T>
+DEV GLB_AS T* as_global(T* p) { return (GLB_AS T*)(p); }
+template <class P>
+DEV cd ldc(P p, int i) { return cd{p[i].r, p[i].i}; }
+template <class P>
+DEV void stc(P p, int i, cd v) {
+    p[i].r = v.r;
+    p[i].i = v.i;
+}
+
 // ------------------------------------------------------------ reductions
 DEV double wave_sum(const Ctx& c, double v) {
     for (int m = 32; m >= 1; m >>= 1) v += c.shfl_xor(v, m);

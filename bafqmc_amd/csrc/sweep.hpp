@@ -190,30 +190,51 @@ DEV double sec_sign(const ModelDev& m, int nf, int sec) {
     return (m.kind == 1 && nf == 2 && sec == 1) ? -1.0 : 1.0;
 }
 
-// f[r] = exp(s_r alpha_nf phi_r), finv = 1 / f, for all rows.
+// f[r] = exp(s_r alpha_nf phi_r), finv = 1 / f, for all rows (LDS).
 DEV void hs_factors(const Ctx& c, const ModelDev& m, const Lds& L, const double* phi_t, int nf) {
     const int ns = m.ns, k = nsec(m);
+    auto f = as_lds(L.f);
+    auto finv = as_lds(L.finv);
     for (int r = c.tid(); r < ns * k; r += c.nthreads()) {
         const int sec = r / ns, i = r - sec * ns;
         const double s = sec_sign(m, nf, sec);
         const double ph = phi_t[i * 2 + (nf - 1)];
         const cd a = m.alpha[nf - 1];
-        L.f[r] = cexp(cd{s * a.r * ph, s * a.i * ph});
-        L.finv[r] = cexp(cd{-s * a.r * ph, -s * a.i * ph});
+        stc(f, r, cexp(cd{s * a.r * ph, s * a.i * ph}));
+        stc(finv, r, cexp(cd{-s * a.r * ph, -s * a.i * ph}));
     }
     c.sync();
 }
 
-// M <- diag(fl) M diag(fr), with fl/fr chosen from f/finv (or none).
-DEV void scale_mat(const Ctx& c, cd* M, int n, const cd* fl, const cd* fr) {
+// M <- diag(fl) M diag(fr), fl/fr in LDS (or null); M in LDS or global (pointer type MP).
+template <class MP>
+DEV void scale_mat_t(const Ctx& c, MP M, int n, const cd* fl_, const cd* fr_) {
+    auto fl = as_lds(fl_);
+    auto fr = as_lds(fr_);
     const int nn = n * n;
     for (int e = c.tid(); e < nn; e += c.nthreads()) {
         const int r = e / n, k = e - r * n;
-        cd x = M[e];
-        if (fl) x = fl[r] * x;
-        if (fr) x = x * fr[k];
-        M[e] = x;
+        cd x = ldc(M, e);
+        if (fl_) x = ldc(fl, r) * x;
+        if (fr_) x = x * ldc(fr, k);
+        stc(M, e, x);
     }
+    c.sync();
+}
+DEV void scale_mat(const Ctx& c, cd* M, int n, const cd* fl, const cd* fr) { scale_mat_t(c, M, n, fl, fr); }
+DEV void scale_mat_lds(const Ctx& c, cd* M, int n, const cd* fl, const cd* fr) {
+    scale_mat_t(c, as_lds(M), n, fl, fr);
+}
+
+// dst <- src between LDS and global memory (n x n complex).
+DEV void lds_from_global(const Ctx& c, cd* dst, const cd* src, int n) {
+    auto d = as_lds(dst);
+    for (int e = c.tid(); e < n * n; e += c.nthreads()) stc(d, e, src[e]);
+    c.sync();
+}
+DEV void global_from_lds(const Ctx& c, cd* dst, const cd* src, int n) {
+    auto s = as_lds(src);
+    for (int e = c.tid(); e < n * n; e += c.nthreads()) dst[e] = ldc(s, e);
     c.sync();
 }
 
@@ -258,7 +279,13 @@ DEVFN void make_proposals(const Ctx& c, const ModelDev& m, const Lds& L, const d
             }
             P.ratio0 = rg * rconst;
         }
-        L.prop[i] = P;
+        auto pp = as_lds(L.prop) + i;
+        pp->phi_new = P.phi_new;
+        pp->ratio0 = P.ratio0;
+        pp->u = P.u;
+        pp->pad = 0.0;
+        pp->d0.r = P.d0.r; pp->d0.i = P.d0.i;
+        pp->d1.r = P.d1.r; pp->d1.i = P.d1.i;
     }
     c.sync();
 }
@@ -269,7 +296,8 @@ DEVFN void make_proposals(const Ctx& c, const ModelDev& m, const Lds& L, const d
 // the ratio (all sites' HS factors of this channel applied for sweep right;
 // none for sweep left) -- the per-site similarity transforms commute with the
 // rank-k updates and leave G_SS unchanged.  Advances `seed` by 2 Ns draws.
-DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t,
+template <class GP>
+DEV void site_loop_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, GP G, int t,
                      int nf, bool ascending, int& seed) {
     const int n = m.n, ns = m.ns;
     const bool pairing = m.kind == 1;
@@ -277,10 +305,11 @@ DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
     double* phi_t = ch.phi + (size_t)t * ns * 2;
     make_proposals(c, m, L, phi_t, nf, ascending, seed);
     seed = lcg_jump(seed, 2 * ns);
-    cd* col0 = L.col;
-    cd* col1 = L.col + n;
-    cd* row0 = L.row;
-    cd* row1 = L.row + n;
+    auto col0 = as_lds(L.col);
+    auto col1 = as_lds(L.col) + n;
+    auto row0 = as_lds(L.row);
+    auto row1 = as_lds(L.row) + n;
+    auto props = as_lds(L.prop);
     // fixed-column mapping of the rank-k update: thread -> column cc, rows r0, r0 + rstep, ...
     const bool fixed_col = n <= nth;
     const int rstep = fixed_col ? nth / n : 1;
@@ -290,19 +319,27 @@ DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
     for (int q = 0; q < ns; ++q) {
         const int i = ascending ? q : ns - 1 - q;
         const int s0 = i, s1 = ns + i;
-        const Proposal P = L.prop[i];
+        Proposal P;
+        {
+            auto pp = props + i;
+            P.phi_new = pp->phi_new;
+            P.ratio0 = pp->ratio0;
+            P.u = pp->u;
+            P.d0 = cd{pp->d0.r, pp->d0.i};
+            P.d1 = cd{pp->d1.r, pp->d1.i};
+        }
         cd m00, m01 = cd{0.0, 0.0}, m10 = cd{0.0, 0.0}, m11 = cd{0.0, 0.0};
         bool acc;
         if (!pairing) {
-            const cd g00 = G[s0 * n + s0];
+            const cd g00 = ldc(G, s0 * n + s0);
             const cd den = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
             const cd rdet = cinv(den);
             const cd z = (cd{P.ratio0, 0.0} * rdet) * conj(rdet);
             acc = cabs(z) > P.u;
             m00 = rdet * P.d0;
         } else {
-            const cd g00 = G[s0 * n + s0], g01 = G[s0 * n + s1];
-            const cd g10 = G[s1 * n + s0], g11 = G[s1 * n + s1];
+            const cd g00 = ldc(G, s0 * n + s0), g01 = ldc(G, s0 * n + s1);
+            const cd g10 = ldc(G, s1 * n + s0), g11 = ldc(G, s1 * n + s1);
             const cd p00 = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
             const cd p01 = -(P.d0 * g01);
             const cd p10 = -(P.d1 * g10);
@@ -319,11 +356,11 @@ DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
         ++nacc;
         // snapshot the column(s) and row(s) of the site
         for (int r = tid; r < n; r += nth) {
-            col0[r] = G[r * n + s0];
-            row0[r] = G[s0 * n + r];
+            stc(col0, r, ldc(G, r * n + s0));
+            stc(row0, r, ldc(G, s0 * n + r));
             if (pairing) {
-                col1[r] = G[r * n + s1];
-                row1[r] = G[s1 * n + r];
+                stc(col1, r, ldc(G, r * n + s1));
+                stc(row1, r, ldc(G, s1 * n + r));
             }
         }
         c.sync();
@@ -332,20 +369,20 @@ DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
         // G -= G[:,S] M (I_S - G[S,:])
         for (int cb = fixed_col ? cc : tid; cb < n; cb += fixed_col ? n : nth) {
             if (fixed_col && r0 >= rstep) break;
-            cd e0 = -row0[cb];
+            cd e0 = -ldc(row0, cb);
             if (cb == s0) e0 = e0 + cd{1.0, 0.0};
             cd w0 = m00 * e0, w1 = cd{0.0, 0.0};
             if (pairing) {
-                cd e1 = -row1[cb];
+                cd e1 = -ldc(row1, cb);
                 if (cb == s1) e1 = e1 + cd{1.0, 0.0};
                 w0 = w0 + m01 * e1;
                 w1 = m10 * e0 + m11 * e1;
             }
             for (int r = fixed_col ? r0 : 0; r < n; r += rstep) {
-                cd x = G[r * n + cb];
-                x = x - col0[r] * w0;
-                if (pairing) x = x - col1[r] * w1;
-                G[r * n + cb] = x;
+                cd x = ldc(G, r * n + cb);
+                x = x - ldc(col0, r) * w0;
+                if (pairing) x = x - ldc(col1, r) * w1;
+                stc(G, r * n + cb, x);
             }
         }
         c.sync();
@@ -356,11 +393,21 @@ DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
     }
 }
 
+DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t, int nf,
+                     bool ascending, int& seed) {
+    site_loop_t(c, m, ch, L, G, t, nf, ascending, seed);
+}
+DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t, int nf,
+                         bool ascending, int& seed) {
+    site_loop_t(c, m, ch, L, as_lds(G), t, nf, ascending, seed);
+}
+
 // ---------------------------------------------------------------- MFMA GEMM in LDS
 // M (LDS, n x n complex) <- E M (left) or M E (right) with E real, read in
 // MFMA operand order from global memory (coalesced 512 B per k-step).  Each
 // wavefront keeps its output tiles in registers until every read of M is done.
-DEVFN void lds_gemm(const Ctx& c, cd* M, const double* Ez, int n, bool left) {
+DEVFN void lds_gemm(const Ctx& c, cd* M_, const double* Ez, int n, bool left) {
+    auto M = as_lds(M_);
     const int nt = (n + 15) >> 4, nk = (n + 3) >> 2, ntt = nt * nt;
     const int lane = c.lane(), li = lane & 15, lk = lane >> 4, w = c.wave(), nw = c.nwaves();
     d4 ar[kMaxTiles], ai[kMaxTiles];
@@ -373,20 +420,20 @@ DEVFN void lds_gemm(const Ctx& c, cd* M, const double* Ez, int n, bool left) {
             const int ti = t / nt, tj = t - ti * nt;
             if (left) {
                 const int col = 16 * tj + li;
-                const double* ez = Ez + (size_t)ti * nk * 64 + lane;
+                auto ez = as_global(Ez + (size_t)ti * nk * 64 + lane);
                 for (int kk = 0; kk < nk; ++kk) {
                     const int k = 4 * kk + lk;
                     const double a = ez[(size_t)kk * 64];
-                    const cd b = (k < n && col < n) ? M[k * n + col] : cd{0.0, 0.0};
+                    const cd b = (k < n && col < n) ? ldc(M, k * n + col) : cd{0.0, 0.0};
                     ar[q] = c.mfma(a, b.r, ar[q]);
                     ai[q] = c.mfma(a, b.i, ai[q]);
                 }
             } else {
                 const int row = 16 * ti + li;
-                const double* ez = Ez + (size_t)tj * nk * 64 + lane;
+                auto ez = as_global(Ez + (size_t)tj * nk * 64 + lane);
                 for (int kk = 0; kk < nk; ++kk) {
                     const int k = 4 * kk + lk;
-                    const cd a = (row < n && k < n) ? M[row * n + k] : cd{0.0, 0.0};
+                    const cd a = (row < n && k < n) ? ldc(M, row * n + k) : cd{0.0, 0.0};
                     const double b = ez[(size_t)kk * 64];
                     ar[q] = c.mfma(a.r, b, ar[q]);
                     ai[q] = c.mfma(a.i, b, ai[q]);
@@ -402,7 +449,7 @@ DEVFN void lds_gemm(const Ctx& c, cd* M, const double* Ez, int n, bool left) {
             const int ti = t / nt, tj = t - ti * nt;
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
-                if (row < n && col < n) M[row * n + col] = cd{ar[q][r], ai[q][r]};
+                if (row < n && col < n) stc(M, row * n + col, cd{ar[q][r], ai[q][r]});
             }
         }
     }
@@ -546,8 +593,10 @@ DEV int imj_index(const ModelDev& m, int i, int j) {
 }
 
 // NC estimator, src/number_conserving/src/obser_equal.f90:95-198.
-DEVFN void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const cd* G) {
+template <class GP>
+DEV void measure_nc_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, GP G) {
     const int n = m.n, nn = n * n;
+    auto red = as_lds(L.red);
     const double lq = (double)m.ns, norm = 1.0 / (lq * lq);
     enum { NV = 22 };
     double v[NV];
@@ -556,9 +605,10 @@ DEVFN void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const
     //    13-14 den_upup 15-16 den_dodo 17-18 den_updo 19 kinetic
     for (int e = c.tid(); e < nn; e += c.nthreads()) {
         const int i = e / n, j = e - i * n;
-        const cd gij = G[e], gji = G[j * n + i];
-        const cd du_i = G[i * n + i] - cd{1.0, 0.0}, du_j = G[j * n + j] - cd{1.0, 0.0};
-        const cd dd_i = conj(G[i * n + i]) - cd{1.0, 0.0}, dd_j = conj(G[j * n + j]) - cd{1.0, 0.0};
+        const cd gij = ldc(G, e), gji = ldc(G, j * n + i);
+        const cd gii = ldc(G, i * n + i), gjj = ldc(G, j * n + j);
+        const cd du_i = gii - cd{1.0, 0.0}, du_j = gjj - cd{1.0, 0.0};
+        const cd dd_i = conj(gii) - cd{1.0, 0.0}, dd_j = conj(gjj) - cd{1.0, 0.0};
         const double dl = (i == j) ? 1.0 : 0.0;
         const cd gupc = gji - cd{dl, 0.0};           // Grupc(i,j)
         const cd gdoc = conj(gji) - cd{dl, 0.0};     // Grdoc(i,j)
@@ -590,19 +640,19 @@ DEVFN void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const
     }
     for (int e = c.tid(); e < m.ns * 3; e += c.nthreads()) {
         const int i = e / 3, j = m.bonds[e];
-        const cd gij = G[i * n + j], gji = G[j * n + i];
+        const cd gij = ldc(G, i * n + j), gji = ldc(G, j * n + i);
         // Grupc(i,j)+Grupc(j,i)+Grdoc(i,j)+Grdoc(j,i), i != j
         v[19] += (gji + gij + conj(gji) + conj(gij)).r / lq;
     }
     for (int k = 0; k < NV; ++k) v[k] = wave_sum(c, v[k]);
     if (c.lane() == 0)
-        for (int k = 0; k < NV; ++k) L.red[c.wave() * NV + k] = v[k];
+        for (int k = 0; k < NV; ++k) red[c.wave() * NV + k] = v[k];
     c.sync();
     if (c.tid() == 0) {
         double s[NV];
         for (int k = 0; k < NV; ++k) {
             s[k] = 0.0;
-            for (int w = 0; w < c.nwaves(); ++w) s[k] += L.red[w * NV + k];
+            for (int w = 0; w < c.nwaves(); ++w) s[k] += red[w * NV + k];
         }
         double* o = ch.obs;
         const double dens = s[0] / lq;
@@ -638,22 +688,24 @@ DEVFN void measure_nc(const Ctx& c, const ModelDev& m, const ChainDev& ch, const
 //   G_b_b+ = Xbb, G_b_c = -Xbc, G_c+_b+ = Xcb, G_c+_c = -Xcc,
 //   G_c_c+ = conj Xbb, G_c_b = -conj Xbc, G_b+_c+ = conj Xcb, G_b+_b = -conj Xcc,
 // and the remaining eight blocks zero.
-DEVFN void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const cd* G) {
+template <class GP>
+DEV void measure_pairing_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, GP G) {
     const int n = m.n, ns = m.ns;
     const double nsd = (double)ns, norm = 1.0 / (nsd * nsd);
-    auto X = [&](int bi, int bj, int i, int j) { return G[(bi * ns + i) * n + bj * ns + j]; };
-    cd* nb = L.site;
-    cd* ncv = L.site + ns;
-    cd* xbb = L.site + 2 * ns;
-    cd* xbc = L.site + 3 * ns;
-    cd* xcb = L.site + 4 * ns;
+    auto X = [&](int bi, int bj, int i, int j) { return ldc(G, (bi * ns + i) * n + bj * ns + j); };
+    auto red = as_lds(L.red);
+    auto nbp = as_lds(L.site);
+    auto ncp = as_lds(L.site) + ns;
+    auto xbbp = as_lds(L.site) + 2 * ns;
+    auto xbcp = as_lds(L.site) + 3 * ns;
+    auto xcbp = as_lds(L.site) + 4 * ns;
     for (int i = c.tid(); i < ns; i += c.nthreads()) {
         const cd xcc = X(1, 1, i, i);
-        nb[i] = -conj(xcc);
-        ncv[i] = -xcc;
-        xbb[i] = X(0, 0, i, i);
-        xbc[i] = X(0, 1, i, i);
-        xcb[i] = X(1, 0, i, i);
+        stc(nbp, i, -conj(xcc));
+        stc(ncp, i, -xcc);
+        stc(xbbp, i, X(0, 0, i, i));
+        stc(xbcp, i, X(0, 1, i, i));
+        stc(xcbp, i, X(1, 0, i, i));
     }
     c.sync();
     enum { NV = 24 };
@@ -665,10 +717,11 @@ DEVFN void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
     for (int e = c.tid(); e < np; e += c.nthreads()) {
         const int i = e / ns, j = e - i * ns;
         const cd Xbb = X(0, 0, i, j), Xbc = X(0, 1, i, j), Xcb = X(1, 0, i, j), Xcc = X(1, 1, i, j);
-        const cd Dbb = nb[i] * nb[j] - conj(Xcc) * Xbb;
-        const cd Dcc = ncv[i] * ncv[j] - Xcc * conj(Xbb);
-        const cd Dbc = nb[i] * ncv[j] - conj(Xcb) * Xbc;
-        const cd Dcb = ncv[i] * nb[j] - Xcb * conj(Xbc);
+        const cd nbi = ldc(nbp, i), nbj = ldc(nbp, j), nci = ldc(ncp, i), ncj = ldc(ncp, j);
+        const cd Dbb = nbi * nbj - conj(Xcc) * Xbb;
+        const cd Dcc = nci * ncj - Xcc * conj(Xbb);
+        const cd Dbc = nbi * ncj - conj(Xcb) * Xbc;
+        const cd Dcb = nci * nbj - Xcb * conj(Xbc);
         v[7] += Dbb.r;
         v[8] += Dcc.r;
         v[15] += Dbb.r; v[16] += Dbb.i;
@@ -681,20 +734,21 @@ DEVFN void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
             const cd dw = ph * (Dbb + Dcc + Dbc + Dcb);
             v[11] += dw.r; v[12] += dw.i;
         }
-        const cd psf = -(conj(xcb[i]) * conj(xbc[j])) + conj(Xcc) * Xcc;
+        const cd psf = -(conj(ldc(xcbp, i)) * conj(ldc(xbcp, j))) + conj(Xcc) * Xcc;
         v[13] += psf.r; v[14] += psf.i;
         if (i == j) {
-            const cd Dbb1 = nb[i] * nb[i] + nb[i] * xbb[i];
-            const cd Dcc1 = ncv[i] * ncv[i] + ncv[i] * conj(xbb[i]);
-            const cd Dbc1 = nb[i] * ncv[i] - conj(xcb[i]) * xbc[i];
-            v[0] += nb[i].r;
-            v[1] += ncv[i].r;
+            const cd xbbi = ldc(xbbp, i), xbci = ldc(xbcp, i), xcbi = ldc(xcbp, i);
+            const cd Dbb1 = nbi * nbi + nbi * xbbi;
+            const cd Dcc1 = nci * nci + nci * conj(xbbi);
+            const cd Dbc1 = nbi * nci - conj(xcbi) * xbci;
+            v[0] += nbi.r;
+            v[1] += nci.r;
             v[2] += Dbc1.r;
-            v[3] += (nb[i] * nb[i] + ncv[i] * ncv[i]).r;
+            v[3] += (nbi * nbi + nci * nci).r;
             v[4] += Dbb1.r;
             v[5] += Dcc1.r;
-            v[6] += (-xbc[i] + conj(xcb[i])).r;
-            ch.obs[O_SITE0 + i] += (nb[i] + ncv[i]).r;
+            v[6] += (-xbci + conj(xcbi)).r;
+            ch.obs[O_SITE0 + i] += (nbi + nci).r;
         }
     }
     for (int e = c.tid(); e < ns * 3; e += c.nthreads()) {
@@ -704,13 +758,13 @@ DEVFN void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
     }
     for (int k = 0; k < NV; ++k) v[k] = wave_sum(c, v[k]);
     if (c.lane() == 0)
-        for (int k = 0; k < NV; ++k) L.red[c.wave() * NV + k] = v[k];
+        for (int k = 0; k < NV; ++k) red[c.wave() * NV + k] = v[k];
     c.sync();
     if (c.tid() == 0) {
         double s[NV];
         for (int k = 0; k < NV; ++k) {
             s[k] = 0.0;
-            for (int w = 0; w < c.nwaves(); ++w) s[k] += L.red[w * NV + k];
+            for (int w = 0; w < c.nwaves(); ++w) s[k] += red[w * NV + k];
         }
         double* o = ch.obs;
         const double dens = (s[0] + s[1]) / nsd;
@@ -751,9 +805,13 @@ DEVFN void measure_pairing(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
     c.sync();
 }
 
-DEV void measure(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const cd* G) {
-    if (m.kind == 0) measure_nc(c, m, ch, L, G);
-    else measure_pairing(c, m, ch, L, G);
+DEVFN void measure(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const cd* G) {
+    if (m.kind == 0) measure_nc_t(c, m, ch, L, G);
+    else measure_pairing_t(c, m, ch, L, G);
+}
+DEVFN void measure_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const cd* G) {
+    if (m.kind == 0) measure_nc_t(c, m, ch, L, as_lds(G));
+    else measure_pairing_t(c, m, ch, L, as_lds(G));
 }
 
 // ====================================================================== global path
@@ -892,7 +950,7 @@ DEVFN void apply_slices_right(const Ctx& c, const ModelDev& m, const ChainDev& c
         for (int nf = 1; nf <= 2; ++nf) {
             if (!m.active[nf - 1]) continue;
             hs_factors(c, m, L, phi_t, nf);
-            scale_mat(c, M, m.n, L.f, nullptr);
+            scale_mat_lds(c, M, m.n, L.f, nullptr);
         }
         lds_gemm(c, M, m.ez[0], m.n, true);
     }
@@ -908,7 +966,7 @@ DEVFN void apply_slices_left(const Ctx& c, const ModelDev& m, const ChainDev& ch
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
             hs_factors(c, m, L, phi_t, nf);
-            scale_mat(c, M, m.n, nullptr, L.f);
+            scale_mat_lds(c, M, m.n, nullptr, L.f);
         }
     }
 }
@@ -916,31 +974,31 @@ DEVFN void apply_slices_left(const Ctx& c, const ModelDev& m, const ChainDev& ch
 // Wrap_R at nt with G in LDS (L.mat): UR <- B(nt, nt-Nwrap] UR, stab_UR, stab_green.
 DEVFN void wrap_r_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
     const int n = m.n, st = nt / m.nwrap;
-    mat_copy(c, ch.g, L.mat, n, false);                 // wrapped G, for the drift check
-    mat_copy(c, L.mat, ch.ur, n, false);
+    global_from_lds(c, ch.g, L.mat, n);                 // wrapped G, for the drift check
+    lds_from_global(c, L.mat, ch.ur, n);
     apply_slices_right(c, m, ch, L, L.mat, nt - m.nwrap + 1, nt);
     scale_mat_cols_real(c, L.mat, n, ch.dr);
     stab_ur_from(c, m, ch, L, L.mat);
     const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, list_mat(ch.ullist, st, n),
                                  list_vec(ch.dllist, st, n), list_mat(ch.vllist, st, n), L.mat, ch.wa);
     store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
-    mat_copy(c, L.mat, ch.wa, n, false);
+    lds_from_global(c, L.mat, ch.wa, n);
     store_right(c, m, ch, st);
 }
 
 // Wrap_L at nt with G in LDS: UL <- UL B(nt+Nwrap, nt], stab_UL, stab_green.
 DEVFN void wrap_l_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
     const int n = m.n, st = nt / m.nwrap;
-    mat_copy(c, ch.g, L.mat, n, false);
-    mat_copy(c, L.mat, ch.ul, n, false);
+    global_from_lds(c, ch.g, L.mat, n);
+    lds_from_global(c, L.mat, ch.ul, n);
     apply_slices_left(c, m, ch, L, L.mat, nt + 1, nt + m.nwrap);
-    mat_copy(c, ch.ul, L.mat, n, false);
+    global_from_lds(c, ch.ul, L.mat, n);
     load_ud(c, L.mat, ch.ul, ch.dl, n, true);
     stab_ul_from(c, m, ch, L, L.mat);
     const double ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
                                  list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, L.mat, ch.wa);
     store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
-    mat_copy(c, L.mat, ch.wa, n, false);
+    lds_from_global(c, L.mat, ch.wa, n);
     store_left(c, m, ch, st);
 }
 
@@ -951,7 +1009,7 @@ DEV void prepare_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
     store_right(c, m, ch, 0);
     store_left(c, m, ch, m.nst);
     for (int st = 1; st <= m.nst; ++st) {
-        mat_copy(c, L.mat, ch.ur, n, false);
+        lds_from_global(c, L.mat, ch.ur, n);
         apply_slices_right(c, m, ch, L, L.mat, (st - 1) * m.nwrap + 1, st * m.nwrap);
         scale_mat_cols_real(c, L.mat, n, ch.dr);
         stab_ur_from(c, m, ch, L, L.mat);
@@ -975,7 +1033,7 @@ DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
             tm.stop(c, ch, S_T_WRAP);
         }
         if (meas) {
-            measure(c, m, ch, L, G);
+            measure_lds(c, m, ch, L, G);
             tm.stop(c, ch, S_T_MEAS);
         }
         // propT_L: G <- e^{+dtau K} (G e^{-dtau K}); UL's product is applied at the next wrap
@@ -986,10 +1044,10 @@ DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
-            site_loop(c, m, ch, L, G, t, nf, false, seed);
+            site_loop_lds(c, m, ch, L, G, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
-            scale_mat(c, G, n, L.finv, L.f);         // G <- e^{-V} G e^{V}
+            scale_mat_lds(c, G, n, L.finv, L.f);         // G <- e^{-V} G e^{V}
             tm.stop(c, ch, S_T_HS);
         }
     }
@@ -1010,9 +1068,9 @@ DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
         for (int nf = 1; nf <= 2; ++nf) {
             if (!m.active[nf - 1]) continue;
             hs_factors(c, m, L, phi_t, nf);
-            scale_mat(c, G, n, L.f, L.finv);         // G <- e^{V} G e^{-V} (old fields)
+            scale_mat_lds(c, G, n, L.f, L.finv);         // G <- e^{V} G e^{-V} (old fields)
             tm.stop(c, ch, S_T_HS);
-            site_loop(c, m, ch, L, G, t, nf, true, seed);
+            site_loop_lds(c, m, ch, L, G, t, nf, true, seed);
             tm.stop(c, ch, S_T_SITE);
         }
         // propT_R: G <- (e^{-dtau K} G) e^{+dtau K}; UR's product is applied at the wrap
@@ -1024,7 +1082,7 @@ DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
             tm.stop(c, ch, S_T_WRAP);
         }
         if (meas) {
-            measure(c, m, ch, L, G);
+            measure_lds(c, m, ch, L, G);
             tm.stop(c, ch, S_T_MEAS);
         }
     }
@@ -1049,7 +1107,7 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a) {
         const ModelDev& m = a.models[ch.model];
         const Lds L = carve(c.smem, m.n, c.nwaves(), kLds);
         int seed = *ch.seed;
-        if (kLds) mat_copy(c, L.mat, ch.g, m.n, false);
+        if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
         for (int s = 0; s < a.nsweep; ++s) {
             if (kLds) {
                 sweep_left_lds(c, m, ch, L, seed, a.measure != 0);
@@ -1059,7 +1117,7 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a) {
                 sweep_right_global(c, m, ch, L, seed, a.measure != 0);
             }
         }
-        if (kLds) mat_copy(c, ch.g, L.mat, m.n, false);
+        if (kLds) global_from_lds(c, ch.g, L.mat, m.n);
         if (c.tid() == 0) *ch.seed = seed;
         c.sync();
     }
