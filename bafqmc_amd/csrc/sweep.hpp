@@ -122,8 +122,8 @@ struct Lds {
     double* dtmp;      // n
     cd* f;             // n
     cd* finv;          // n
-    cd* col;           // 2 x n
-    cd* row;           // 2 x n
+    cd* col;           // 2 slots x 2 sectors x n
+    cd* row;           // 2 slots x 2 sectors x n
     double* red;       // nwaves x 64
     cd* site;          // 6 x ns (pairing measurement)
     Proposal* prop;    // ns
@@ -140,7 +140,7 @@ HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat) {
     b += lds_align(16 * sizeof(double));              // bcast
     b += lds_align((size_t)n * sizeof(double));       // dtmp
     b += 2 * lds_align((size_t)n * sizeof(cd));       // f, finv
-    b += 2 * lds_align(2 * (size_t)n * sizeof(cd));   // col, row
+    b += 2 * lds_align(4 * (size_t)n * sizeof(cd));   // col, row: [slot][sector][n]
     b += lds_align((size_t)nwaves * 64 * sizeof(double));
     b += lds_align(6 * (size_t)n * sizeof(cd));
     b += lds_align((size_t)n * sizeof(Proposal));
@@ -163,8 +163,8 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat) {
     L.dtmp = (double*)take((size_t)n * sizeof(double));
     L.f = (cd*)take((size_t)n * sizeof(cd));
     L.finv = (cd*)take((size_t)n * sizeof(cd));
-    L.col = (cd*)take(2 * (size_t)n * sizeof(cd));
-    L.row = (cd*)take(2 * (size_t)n * sizeof(cd));
+    L.col = (cd*)take(4 * (size_t)n * sizeof(cd));
+    L.row = (cd*)take(4 * (size_t)n * sizeof(cd));
     L.red = (double*)take((size_t)nwaves * 64 * sizeof(double));
     L.site = (cd*)take(6 * (size_t)n * sizeof(cd));
     L.prop = (Proposal*)take((size_t)n * sizeof(Proposal));
@@ -402,6 +402,189 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
     site_loop_t(c, m, ch, L, as_lds(G), t, nf, ascending, seed);
 }
 
+// ---------------------------------------------------------------- register-resident site phase
+// All local updates of one time slice (both HS channels) with G held in
+// registers.  Thread (cc, r0) owns column cc, rows r0, r0 + rstep, ... of G
+// (n <= threads).  Per site, the owners publish the site's column(s) and
+// row(s) into a double-buffered LDS slot; after one barrier every thread
+// reads G_SS from it, takes the (identical) Metropolis decision and applies
+// the rank-k update to its own entries, then publishes the next site.
+// sweep right (right = true): channels U1, U2, each with the similarity
+// pre-scale e^{V} G e^{-V} and ascending sites (localU.f90:97-113);
+// sweep left: channels U2, U1, descending sites, then e^{-V} G e^{V} (:79-95).
+// rows of G held per thread: ceil(n / floor(threads / n)); instantiated for 4, 12 and 20.
+HOSTDEV int reg_rows_needed(int n, int threads) { return (n + threads / n - 1) / (threads / n); }
+
+template <int kRegRows>
+DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
+                            int& seed) {
+    const int n = m.n, ns = m.ns;
+    const bool pairing = m.kind == 1;
+    const int tid = c.tid(), nth = c.nthreads();
+    const int rstep = nth / n;
+    const int cc = tid % n, r0 = tid / n;
+    const bool owner = r0 < rstep;
+    auto M = as_lds(L.mat);
+    auto f = as_lds(L.f);
+    auto finv = as_lds(L.finv);
+    auto props = as_lds(L.prop);
+    // col buffers: [2 slots][2 sectors][n], row buffers likewise
+    auto colb = as_lds(L.col);
+    auto rowb = as_lds(L.row);
+    cd g[kRegRows];
+#pragma unroll
+    for (int k = 0; k < kRegRows; ++k) {
+        const int r = r0 + k * rstep;
+        g[k] = (owner && r < n) ? ldc(M, r * n + cc) : cd{0.0, 0.0};
+    }
+    double* phi_t = ch.phi + (size_t)t * ns * 2;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int nf = right ? 1 + pass : 2 - pass;
+        if (!m.active[nf - 1]) continue;
+        if (right) {
+            hs_factors(c, m, L, phi_t, nf);     // old fields
+            const cd fc = ldc(finv, cc);
+#pragma unroll
+            for (int k = 0; k < kRegRows; ++k) {
+                const int r = r0 + k * rstep;
+                if (owner && r < n) g[k] = (ldc(f, r) * g[k]) * fc;
+            }
+        }
+        make_proposals(c, m, L, phi_t, nf, right, seed);
+        seed = lcg_jump(seed, 2 * ns);
+        // publish the first site
+        auto publish = [&](int slot, int i) {
+            const int s0 = i, s1 = ns + i;
+            auto col0 = colb + (slot * 2) * n, col1 = colb + (slot * 2 + 1) * n;
+            auto row0 = rowb + (slot * 2) * n, row1 = rowb + (slot * 2 + 1) * n;
+            if (!owner) return;
+            if (cc == s0) {
+#pragma unroll
+                for (int k = 0; k < kRegRows; ++k) {
+                    const int r = r0 + k * rstep;
+                    if (r < n) stc(col0, r, g[k]);
+                }
+            }
+            if (pairing && cc == s1) {
+#pragma unroll
+                for (int k = 0; k < kRegRows; ++k) {
+                    const int r = r0 + k * rstep;
+                    if (r < n) stc(col1, r, g[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kRegRows; ++k) {
+                const int r = r0 + k * rstep;
+                if (r == s0) stc(row0, cc, g[k]);
+                if (pairing && r == s1) stc(row1, cc, g[k]);
+            }
+        };
+        publish(0, right ? 0 : ns - 1);
+        c.sync();
+        int nacc = 0;
+        for (int q = 0; q < ns; ++q) {
+            const int i = right ? q : ns - 1 - q;
+            const int slot = q & 1;
+            const int s0 = i, s1 = ns + i;
+            auto col0 = colb + (slot * 2) * n, col1 = colb + (slot * 2 + 1) * n;
+            auto row0 = rowb + (slot * 2) * n, row1 = rowb + (slot * 2 + 1) * n;
+            auto pp = props + i;
+            const double ratio0 = pp->ratio0, u = pp->u;
+            const cd d0 = cd{pp->d0.r, pp->d0.i}, d1 = cd{pp->d1.r, pp->d1.i};
+            cd m00, m01 = cd{0.0, 0.0}, m10 = cd{0.0, 0.0}, m11 = cd{0.0, 0.0};
+            bool acc;
+            if (!pairing) {
+                const cd g00 = ldc(col0, s0);
+                const cd den = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - g00);
+                const cd rdet = cinv(den);
+                const cd z = (cd{ratio0, 0.0} * rdet) * conj(rdet);
+                acc = cabs(z) > u;
+                m00 = rdet * d0;
+            } else {
+                const cd g00 = ldc(col0, s0), g01 = ldc(row0, s1);
+                const cd g10 = ldc(col0, s1), g11 = ldc(col1, s1);
+                const cd p00 = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - g00);
+                const cd p01 = -(d0 * g01);
+                const cd p10 = -(d1 * g10);
+                const cd p11 = cd{1.0, 0.0} + d1 * (cd{1.0, 0.0} - g11);
+                const cd det = p00 * p11 - p01 * p10;
+                acc = fabs(ratio0 / cabs(det)) > u;
+                const cd idet = cinv(det);
+                m00 = (p11 * idet) * d0;
+                m01 = (-(p01 * idet)) * d1;
+                m10 = (-(p10 * idet)) * d0;
+                m11 = (p00 * idet) * d1;
+            }
+            if (acc) {
+                ++nacc;
+                if (tid == 0) phi_t[i * 2 + nf - 1] = pp->phi_new;
+                if (owner) {
+                    // G -= G[:,S] M (I_S - G[S,:]) on this thread's entries
+                    cd e0 = -ldc(row0, cc);
+                    if (cc == s0) e0 = e0 + cd{1.0, 0.0};
+                    cd w0 = m00 * e0, w1 = cd{0.0, 0.0};
+                    if (pairing) {
+                        cd e1 = -ldc(row1, cc);
+                        if (cc == s1) e1 = e1 + cd{1.0, 0.0};
+                        w0 = w0 + m01 * e1;
+                        w1 = m10 * e0 + m11 * e1;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kRegRows; ++k) {
+                        const int r = r0 + k * rstep;
+                        if (r < n) {
+                            cd x = g[k] - ldc(col0, r) * w0;
+                            if (pairing) x = x - ldc(col1, r) * w1;
+                            g[k] = x;
+                        }
+                    }
+                }
+            }
+            if (q + 1 < ns) publish(slot ^ 1, right ? q + 1 : ns - 2 - q);
+            c.sync();
+        }
+        if (tid == 0) {
+            ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
+            ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
+        }
+        if (!right) {
+            hs_factors(c, m, L, phi_t, nf);     // new fields
+            const cd fc = ldc(f, cc);
+#pragma unroll
+            for (int k = 0; k < kRegRows; ++k) {
+                const int r = r0 + k * rstep;
+                if (owner && r < n) g[k] = (ldc(finv, r) * g[k]) * fc;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kRegRows; ++k) {
+        const int r = r0 + k * rstep;
+        if (owner && r < n) stc(M, r * n + cc, g[k]);
+    }
+    c.sync();
+}
+
+DEVFN void slice_sites_r4(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
+                          int& seed) {
+    slice_sites_regs_t<4>(c, m, ch, L, t, right, seed);
+}
+DEVFN void slice_sites_r12(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
+                           int& seed) {
+    slice_sites_regs_t<12>(c, m, ch, L, t, right, seed);
+}
+DEVFN void slice_sites_r20(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
+                           int& seed) {
+    slice_sites_regs_t<20>(c, m, ch, L, t, right, seed);
+}
+DEV void slice_sites_regs(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
+                          int& seed) {
+    const int rows = reg_rows_needed(m.n, c.nthreads());
+    if (rows <= 4) slice_sites_r4(c, m, ch, L, t, right, seed);
+    else if (rows <= 12) slice_sites_r12(c, m, ch, L, t, right, seed);
+    else slice_sites_r20(c, m, ch, L, t, right, seed);
+}
+
 // ---------------------------------------------------------------- MFMA GEMM in LDS
 // M (LDS, n x n complex) <- E M (left) or M E (right) with E real, read in
 // MFMA operand order from global memory (coalesced 512 B per k-step).  Each
@@ -418,25 +601,42 @@ DEVFN void lds_gemm(const Ctx& c, cd* M_, const double* Ez, int n, bool left) {
         const int t = w + q * nw;
         if (t < ntt) {
             const int ti = t / nt, tj = t - ti * nt;
+            // propagator fragments are fetched kChunk k-steps at a time so the
+            // global-memory latency is paid once per chunk, not per MFMA
+            constexpr int kChunk = 8;
             if (left) {
                 const int col = 16 * tj + li;
                 auto ez = as_global(Ez + (size_t)ti * nk * 64 + lane);
-                for (int kk = 0; kk < nk; ++kk) {
-                    const int k = 4 * kk + lk;
-                    const double a = ez[(size_t)kk * 64];
-                    const cd b = (k < n && col < n) ? ldc(M, k * n + col) : cd{0.0, 0.0};
-                    ar[q] = c.mfma(a, b.r, ar[q]);
-                    ai[q] = c.mfma(a, b.i, ai[q]);
+                for (int k0 = 0; k0 < nk; k0 += kChunk) {
+                    double eb[kChunk];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) eb[u] = (k0 + u < nk) ? ez[(size_t)(k0 + u) * 64] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) {
+                        if (k0 + u < nk) {
+                            const int k = 4 * (k0 + u) + lk;
+                            const cd b = (k < n && col < n) ? ldc(M, k * n + col) : cd{0.0, 0.0};
+                            ar[q] = c.mfma(eb[u], b.r, ar[q]);
+                            ai[q] = c.mfma(eb[u], b.i, ai[q]);
+                        }
+                    }
                 }
             } else {
                 const int row = 16 * ti + li;
                 auto ez = as_global(Ez + (size_t)tj * nk * 64 + lane);
-                for (int kk = 0; kk < nk; ++kk) {
-                    const int k = 4 * kk + lk;
-                    const cd a = (row < n && k < n) ? ldc(M, row * n + k) : cd{0.0, 0.0};
-                    const double b = ez[(size_t)kk * 64];
-                    ar[q] = c.mfma(a.r, b, ar[q]);
-                    ai[q] = c.mfma(a.i, b, ai[q]);
+                for (int k0 = 0; k0 < nk; k0 += kChunk) {
+                    double eb[kChunk];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) eb[u] = (k0 + u < nk) ? ez[(size_t)(k0 + u) * 64] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) {
+                        if (k0 + u < nk) {
+                            const int k = 4 * (k0 + u) + lk;
+                            const cd a = (row < n && k < n) ? ldc(M, row * n + k) : cd{0.0, 0.0};
+                            ar[q] = c.mfma(a.r, eb[u], ar[q]);
+                            ai[q] = c.mfma(a.i, eb[u], ai[q]);
+                        }
+                    }
                 }
             }
         }
@@ -1040,16 +1240,8 @@ DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         lds_gemm(c, G, m.ez[2], n, false);
         lds_gemm(c, G, m.ez[1], n, true);
         tm.stop(c, ch, S_T_KIN);
-        const int t = nt - 1;
-        const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
-        for (int nf = 2; nf >= 1; --nf) {
-            if (!m.active[nf - 1]) continue;
-            site_loop_lds(c, m, ch, L, G, t, nf, false, seed);
-            tm.stop(c, ch, S_T_SITE);
-            hs_factors(c, m, L, phi_t, nf);
-            scale_mat_lds(c, G, n, L.finv, L.f);         // G <- e^{-V} G e^{V}
-            tm.stop(c, ch, S_T_HS);
-        }
+        slice_sites_regs(c, m, ch, L, nt - 1, false, seed);
+        tm.stop(c, ch, S_T_SITE);
     }
     wrap_l_lds(c, m, ch, L, 0);
     tm.stop(c, ch, S_T_WRAP);
@@ -1063,16 +1255,8 @@ DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
     live_identity(c, n, ch.ur, ch.dr, ch.vr);     // Wrap_R(0): R list slot 0 is the identity
     tm.start(c);
     for (int nt = 1; nt <= m.ltrot; ++nt) {
-        const int t = nt - 1;
-        const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
-        for (int nf = 1; nf <= 2; ++nf) {
-            if (!m.active[nf - 1]) continue;
-            hs_factors(c, m, L, phi_t, nf);
-            scale_mat_lds(c, G, n, L.f, L.finv);         // G <- e^{V} G e^{-V} (old fields)
-            tm.stop(c, ch, S_T_HS);
-            site_loop_lds(c, m, ch, L, G, t, nf, true, seed);
-            tm.stop(c, ch, S_T_SITE);
-        }
+        slice_sites_regs(c, m, ch, L, nt - 1, true, seed);
+        tm.stop(c, ch, S_T_SITE);
         // propT_R: G <- (e^{-dtau K} G) e^{+dtau K}; UR's product is applied at the wrap
         lds_gemm(c, G, m.ez[0], n, true);
         lds_gemm(c, G, m.ez[3], n, false);
