@@ -131,6 +131,13 @@ struct Lds {
 };
 
 HOSTDEV size_t lds_align(size_t b) { return (b + 15) & ~(size_t)15; }
+// rows per LDS column vector: n, or the padded row range of the register-resident
+// site phase (threads / n rows per step, at most 20 steps).
+HOSTDEV int col_rows(int n, int nwaves) {
+    const int threads = 64 * nwaves;
+    const int padded = n <= threads ? (threads / n) * 20 : n;
+    return padded > n ? padded : n;
+}
 
 HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat) {
     size_t b = 0;
@@ -140,7 +147,8 @@ HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat) {
     b += lds_align(16 * sizeof(double));              // bcast
     b += lds_align((size_t)n * sizeof(double));       // dtmp
     b += 2 * lds_align((size_t)n * sizeof(cd));       // f, finv
-    b += 2 * lds_align(4 * (size_t)n * sizeof(cd));   // col, row: [slot][sector][n]
+    b += lds_align(4 * (size_t)col_rows(n, nwaves) * sizeof(cd));   // col: [slot][sector][rows]
+    b += lds_align(4 * (size_t)n * sizeof(cd));                     // row: [slot][sector][n]
     b += lds_align((size_t)nwaves * 64 * sizeof(double));
     b += lds_align(6 * (size_t)n * sizeof(cd));
     b += lds_align((size_t)n * sizeof(Proposal));
@@ -163,7 +171,7 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat) {
     L.dtmp = (double*)take((size_t)n * sizeof(double));
     L.f = (cd*)take((size_t)n * sizeof(cd));
     L.finv = (cd*)take((size_t)n * sizeof(cd));
-    L.col = (cd*)take(4 * (size_t)n * sizeof(cd));
+    L.col = (cd*)take(4 * (size_t)col_rows(n, nwaves) * sizeof(cd));
     L.row = (cd*)take(4 * (size_t)n * sizeof(cd));
     L.red = (double*)take((size_t)nwaves * 64 * sizeof(double));
     L.site = (cd*)take(6 * (size_t)n * sizeof(cd));
@@ -424,13 +432,16 @@ DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch,
     const int rstep = nth / n;
     const int cc = tid % n, r0 = tid / n;
     const bool owner = r0 < rstep;
+    // padded row range [0, npad): rows >= n are zero in every LDS vector read
+    // below, so the per-row work needs no predicates.
+    const int npad = rstep * kRegRows;
     auto M = as_lds(L.mat);
     auto f = as_lds(L.f);
     auto finv = as_lds(L.finv);
     auto props = as_lds(L.prop);
-    // col buffers: [2 slots][2 sectors][n], row buffers likewise
-    auto colb = as_lds(L.col);
-    auto rowb = as_lds(L.row);
+    auto colb = as_lds(L.col);      // [slot][sector][npad]
+    auto rowb = as_lds(L.row);      // [slot][sector][n]
+    for (int e = tid; e < 4 * npad; e += nth) stc(colb, e, cd{0.0, 0.0});
     cd g[kRegRows];
 #pragma unroll
     for (int k = 0; k < kRegRows; ++k) {
@@ -438,6 +449,31 @@ DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch,
         g[k] = (owner && r < n) ? ldc(M, r * n + cc) : cd{0.0, 0.0};
     }
     double* phi_t = ch.phi + (size_t)t * ns * 2;
+    // publish the column(s) and row(s) of site i (after this thread's updates)
+    auto publish = [&](int slot, int i) {
+        if (!owner) return;
+        const int s0 = i, s1 = ns + i;
+        if (cc == s0 || (pairing && cc == s1)) {
+            auto col = colb + (slot * 2 + (cc == s0 ? 0 : 1)) * npad;
+#pragma unroll
+            for (int k = 0; k < kRegRows; ++k) stc(col, r0 + k * rstep, g[k]);
+        }
+        // row entries: this thread holds row s iff (s - r0) % rstep == 0
+        const int d0 = s0 - r0;
+        const int k0 = d0 / rstep;
+        cd v0 = g[0];
+#pragma unroll
+        for (int k = 1; k < kRegRows; ++k) v0 = (k == k0) ? g[k] : v0;
+        if (d0 >= 0 && d0 % rstep == 0) stc(rowb + (slot * 2) * n, cc, v0);
+        if (pairing) {
+            const int d1 = s1 - r0;
+            const int k1 = d1 / rstep;
+            cd v1 = g[0];
+#pragma unroll
+            for (int k = 1; k < kRegRows; ++k) v1 = (k == k1) ? g[k] : v1;
+            if (d1 >= 0 && d1 % rstep == 0) stc(rowb + (slot * 2 + 1) * n, cc, v1);
+        }
+    };
     for (int pass = 0; pass < 2; ++pass) {
         const int nf = right ? 1 + pass : 2 - pass;
         if (!m.active[nf - 1]) continue;
@@ -447,38 +483,11 @@ DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch,
 #pragma unroll
             for (int k = 0; k < kRegRows; ++k) {
                 const int r = r0 + k * rstep;
-                if (owner && r < n) g[k] = (ldc(f, r) * g[k]) * fc;
+                if (r < n) g[k] = (ldc(f, r) * g[k]) * fc;
             }
         }
         make_proposals(c, m, L, phi_t, nf, right, seed);
         seed = lcg_jump(seed, 2 * ns);
-        // publish the first site
-        auto publish = [&](int slot, int i) {
-            const int s0 = i, s1 = ns + i;
-            auto col0 = colb + (slot * 2) * n, col1 = colb + (slot * 2 + 1) * n;
-            auto row0 = rowb + (slot * 2) * n, row1 = rowb + (slot * 2 + 1) * n;
-            if (!owner) return;
-            if (cc == s0) {
-#pragma unroll
-                for (int k = 0; k < kRegRows; ++k) {
-                    const int r = r0 + k * rstep;
-                    if (r < n) stc(col0, r, g[k]);
-                }
-            }
-            if (pairing && cc == s1) {
-#pragma unroll
-                for (int k = 0; k < kRegRows; ++k) {
-                    const int r = r0 + k * rstep;
-                    if (r < n) stc(col1, r, g[k]);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kRegRows; ++k) {
-                const int r = r0 + k * rstep;
-                if (r == s0) stc(row0, cc, g[k]);
-                if (pairing && r == s1) stc(row1, cc, g[k]);
-            }
-        };
         publish(0, right ? 0 : ns - 1);
         c.sync();
         int nacc = 0;
@@ -486,7 +495,7 @@ DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch,
             const int i = right ? q : ns - 1 - q;
             const int slot = q & 1;
             const int s0 = i, s1 = ns + i;
-            auto col0 = colb + (slot * 2) * n, col1 = colb + (slot * 2 + 1) * n;
+            auto col0 = colb + (slot * 2) * npad, col1 = colb + (slot * 2 + 1) * npad;
             auto row0 = rowb + (slot * 2) * n, row1 = rowb + (slot * 2 + 1) * n;
             auto pp = props + i;
             const double ratio0 = pp->ratio0, u = pp->u;
@@ -515,29 +524,28 @@ DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch,
                 m10 = (-(p10 * idet)) * d0;
                 m11 = (p00 * idet) * d1;
             }
-            if (acc) {
-                ++nacc;
-                if (tid == 0) phi_t[i * 2 + nf - 1] = pp->phi_new;
-                if (owner) {
-                    // G -= G[:,S] M (I_S - G[S,:]) on this thread's entries
-                    cd e0 = -ldc(row0, cc);
-                    if (cc == s0) e0 = e0 + cd{1.0, 0.0};
-                    cd w0 = m00 * e0, w1 = cd{0.0, 0.0};
-                    if (pairing) {
-                        cd e1 = -ldc(row1, cc);
-                        if (cc == s1) e1 = e1 + cd{1.0, 0.0};
-                        w0 = w0 + m01 * e1;
-                        w1 = m10 * e0 + m11 * e1;
-                    }
+            if (acc) ++nacc;
+            if (acc && tid == 0) phi_t[i * 2 + nf - 1] = pp->phi_new;
+            if (acc && owner) {
+                // G -= G[:,S] M (I_S - G[S,:]) on this thread's entries
+                cd e0 = -ldc(row0, cc);
+                if (cc == s0) e0 = e0 + cd{1.0, 0.0};
+                cd w0 = m00 * e0, w1 = cd{0.0, 0.0};
+                if (pairing) {
+                    cd e1 = -ldc(row1, cc);
+                    if (cc == s1) e1 = e1 + cd{1.0, 0.0};
+                    w0 = w0 + m01 * e1;
+                    w1 = m10 * e0 + m11 * e1;
+                }
+                if (pairing) {
 #pragma unroll
                     for (int k = 0; k < kRegRows; ++k) {
                         const int r = r0 + k * rstep;
-                        if (r < n) {
-                            cd x = g[k] - ldc(col0, r) * w0;
-                            if (pairing) x = x - ldc(col1, r) * w1;
-                            g[k] = x;
-                        }
+                        g[k] = g[k] - ldc(col0, r) * w0 - ldc(col1, r) * w1;
                     }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kRegRows; ++k) g[k] = g[k] - ldc(col0, r0 + k * rstep) * w0;
                 }
             }
             if (q + 1 < ns) publish(slot ^ 1, right ? q + 1 : ns - 2 - q);
@@ -553,7 +561,7 @@ DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch,
 #pragma unroll
             for (int k = 0; k < kRegRows; ++k) {
                 const int r = r0 + k * rstep;
-                if (owner && r < n) g[k] = (ldc(finv, r) * g[k]) * fc;
+                if (r < n) g[k] = (ldc(finv, r) * g[k]) * fc;
             }
         }
     }
