@@ -131,14 +131,6 @@ struct Lds {
 };
 
 HOSTDEV size_t lds_align(size_t b) { return (b + 15) & ~(size_t)15; }
-// rows per LDS column vector: n, or the padded row range of the register-resident
-// site phase (threads / n rows per step, at most 20 steps).
-HOSTDEV int col_rows(int n, int nwaves) {
-    const int threads = 64 * nwaves;
-    const int padded = n <= threads ? (threads / n) * 20 : n;
-    return padded > n ? padded : n;
-}
-
 HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat) {
     size_t b = 0;
     b += lds_align((size_t)n * sizeof(cd));           // tau
@@ -147,8 +139,7 @@ HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat) {
     b += lds_align(16 * sizeof(double));              // bcast
     b += lds_align((size_t)n * sizeof(double));       // dtmp
     b += 2 * lds_align((size_t)n * sizeof(cd));       // f, finv
-    b += lds_align(4 * (size_t)col_rows(n, nwaves) * sizeof(cd));   // col: [slot][sector][rows]
-    b += lds_align(4 * (size_t)n * sizeof(cd));                     // row: [slot][sector][n]
+    b += 2 * lds_align(4 * (size_t)n * sizeof(cd));   // col, row: [slot][sector][n]
     b += lds_align((size_t)nwaves * 64 * sizeof(double));
     b += lds_align(6 * (size_t)n * sizeof(cd));
     b += lds_align((size_t)n * sizeof(Proposal));
@@ -171,7 +162,7 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat) {
     L.dtmp = (double*)take((size_t)n * sizeof(double));
     L.f = (cd*)take((size_t)n * sizeof(cd));
     L.finv = (cd*)take((size_t)n * sizeof(cd));
-    L.col = (cd*)take(4 * (size_t)col_rows(n, nwaves) * sizeof(cd));
+    L.col = (cd*)take(4 * (size_t)n * sizeof(cd));
     L.row = (cd*)take(4 * (size_t)n * sizeof(cd));
     L.red = (double*)take((size_t)nwaves * 64 * sizeof(double));
     L.site = (cd*)take(6 * (size_t)n * sizeof(cd));
@@ -410,124 +401,87 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
     site_loop_t(c, m, ch, L, as_lds(G), t, nf, ascending, seed);
 }
 
-// ---------------------------------------------------------------- register-resident site phase
-// All local updates of one time slice (both HS channels) with G held in
-// registers.  Thread (cc, r0) owns column cc, rows r0, r0 + rstep, ... of G
-// (n <= threads).  Per site, the owners publish the site's column(s) and
-// row(s) into a double-buffered LDS slot; after one barrier every thread
-// reads G_SS from it, takes the (identical) Metropolis decision and applies
-// the rank-k update to its own entries, then publishes the next site.
-// sweep right (right = true): channels U1, U2, each with the similarity
-// pre-scale e^{V} G e^{-V} and ascending sites (localU.f90:97-113);
-// sweep left: channels U2, U1, descending sites, then e^{-V} G e^{V} (:79-95).
-// rows of G held per thread: ceil(n / floor(threads / n)); instantiated for 4, 12 and 20.
-HOSTDEV int reg_rows_needed(int n, int threads) { return (n + threads / n - 1) / (threads / n); }
-
-template <int kRegRows>
-DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
-                            int& seed) {
+// ---------------------------------------------------------------- LDS-resident site loop, one barrier per site
+// G in LDS; thread (cc, r0) updates column cc, rows r0, r0 + rstep, ... (n <= threads).
+// The column(s) and row(s) of the NEXT site are published into the other slot
+// of a double-buffered LDS snapshot by the threads that produce those entries
+// (while updating, or by copying when the current proposal is rejected), so a
+// single barrier separates consecutive sites.
+DEVFN void site_loop_lds1(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t, int nf,
+                          bool ascending, int& seed) {
+    auto G = as_lds(G_);
     const int n = m.n, ns = m.ns;
     const bool pairing = m.kind == 1;
     const int tid = c.tid(), nth = c.nthreads();
+    double* phi_t = ch.phi + (size_t)t * ns * 2;
+    make_proposals(c, m, L, phi_t, nf, ascending, seed);
+    seed = lcg_jump(seed, 2 * ns);
     const int rstep = nth / n;
     const int cc = tid % n, r0 = tid / n;
     const bool owner = r0 < rstep;
-    // padded row range [0, npad): rows >= n are zero in every LDS vector read
-    // below, so the per-row work needs no predicates.
-    const int npad = rstep * kRegRows;
-    auto M = as_lds(L.mat);
-    auto f = as_lds(L.f);
-    auto finv = as_lds(L.finv);
-    auto props = as_lds(L.prop);
-    auto colb = as_lds(L.col);      // [slot][sector][npad]
+    auto colb = as_lds(L.col);      // [slot][sector][n]
     auto rowb = as_lds(L.row);      // [slot][sector][n]
-    for (int e = tid; e < 4 * npad; e += nth) stc(colb, e, cd{0.0, 0.0});
-    cd g[kRegRows];
-#pragma unroll
-    for (int k = 0; k < kRegRows; ++k) {
-        const int r = r0 + k * rstep;
-        g[k] = (owner && r < n) ? ldc(M, r * n + cc) : cd{0.0, 0.0};
-    }
-    double* phi_t = ch.phi + (size_t)t * ns * 2;
-    // publish the column(s) and row(s) of site i (after this thread's updates)
-    auto publish = [&](int slot, int i) {
+    auto props = as_lds(L.prop);
+    auto site_of = [&](int q) { return ascending ? q : ns - 1 - q; };
+    // copy the column(s)/row(s) of site i from G into slot
+    auto publish_from_g = [&](int slot, int i) {
         if (!owner) return;
         const int s0 = i, s1 = ns + i;
         if (cc == s0 || (pairing && cc == s1)) {
-            auto col = colb + (slot * 2 + (cc == s0 ? 0 : 1)) * npad;
-#pragma unroll
-            for (int k = 0; k < kRegRows; ++k) stc(col, r0 + k * rstep, g[k]);
+            auto col = colb + (slot * 2 + (cc == s0 ? 0 : 1)) * n;
+            for (int r = r0; r < n; r += rstep) stc(col, r, ldc(G, r * n + cc));
         }
-        // row entries: this thread holds row s iff (s - r0) % rstep == 0
         const int d0 = s0 - r0;
-        const int k0 = d0 / rstep;
-        cd v0 = g[0];
-#pragma unroll
-        for (int k = 1; k < kRegRows; ++k) v0 = (k == k0) ? g[k] : v0;
-        if (d0 >= 0 && d0 % rstep == 0) stc(rowb + (slot * 2) * n, cc, v0);
+        if (d0 >= 0 && d0 % rstep == 0) stc(rowb + (slot * 2) * n, cc, ldc(G, s0 * n + cc));
         if (pairing) {
             const int d1 = s1 - r0;
-            const int k1 = d1 / rstep;
-            cd v1 = g[0];
-#pragma unroll
-            for (int k = 1; k < kRegRows; ++k) v1 = (k == k1) ? g[k] : v1;
-            if (d1 >= 0 && d1 % rstep == 0) stc(rowb + (slot * 2 + 1) * n, cc, v1);
+            if (d1 >= 0 && d1 % rstep == 0) stc(rowb + (slot * 2 + 1) * n, cc, ldc(G, s1 * n + cc));
         }
     };
-    for (int pass = 0; pass < 2; ++pass) {
-        const int nf = right ? 1 + pass : 2 - pass;
-        if (!m.active[nf - 1]) continue;
-        if (right) {
-            hs_factors(c, m, L, phi_t, nf);     // old fields
-            const cd fc = ldc(finv, cc);
-#pragma unroll
-            for (int k = 0; k < kRegRows; ++k) {
-                const int r = r0 + k * rstep;
-                if (r < n) g[k] = (ldc(f, r) * g[k]) * fc;
-            }
+    publish_from_g(0, site_of(0));
+    c.sync();
+    int nacc = 0;
+    for (int q = 0; q < ns; ++q) {
+        const int i = site_of(q);
+        const int slot = q & 1;
+        const int s0 = i, s1 = ns + i;
+        auto col0 = colb + (slot * 2) * n, col1 = colb + (slot * 2 + 1) * n;
+        auto row0 = rowb + (slot * 2) * n, row1 = rowb + (slot * 2 + 1) * n;
+        auto pp = props + i;
+        const double ratio0 = pp->ratio0, u = pp->u;
+        const cd d0 = cd{pp->d0.r, pp->d0.i}, d1 = cd{pp->d1.r, pp->d1.i};
+        cd m00, m01 = cd{0.0, 0.0}, m10 = cd{0.0, 0.0}, m11 = cd{0.0, 0.0};
+        bool acc;
+        if (!pairing) {
+            const cd g00 = ldc(col0, s0);
+            const cd den = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - g00);
+            const cd rdet = cinv(den);
+            const cd z = (cd{ratio0, 0.0} * rdet) * conj(rdet);
+            acc = cabs(z) > u;
+            m00 = rdet * d0;
+        } else {
+            const cd g00 = ldc(col0, s0), g01 = ldc(row0, s1);
+            const cd g10 = ldc(col0, s1), g11 = ldc(col1, s1);
+            const cd p00 = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - g00);
+            const cd p01 = -(d0 * g01);
+            const cd p10 = -(d1 * g10);
+            const cd p11 = cd{1.0, 0.0} + d1 * (cd{1.0, 0.0} - g11);
+            const cd det = p00 * p11 - p01 * p10;
+            acc = fabs(ratio0 / cabs(det)) > u;
+            const cd idet = cinv(det);
+            m00 = (p11 * idet) * d0;
+            m01 = (-(p01 * idet)) * d1;
+            m10 = (-(p10 * idet)) * d0;
+            m11 = (p00 * idet) * d1;
         }
-        make_proposals(c, m, L, phi_t, nf, right, seed);
-        seed = lcg_jump(seed, 2 * ns);
-        publish(0, right ? 0 : ns - 1);
-        c.sync();
-        int nacc = 0;
-        for (int q = 0; q < ns; ++q) {
-            const int i = right ? q : ns - 1 - q;
-            const int slot = q & 1;
-            const int s0 = i, s1 = ns + i;
-            auto col0 = colb + (slot * 2) * npad, col1 = colb + (slot * 2 + 1) * npad;
-            auto row0 = rowb + (slot * 2) * n, row1 = rowb + (slot * 2 + 1) * n;
-            auto pp = props + i;
-            const double ratio0 = pp->ratio0, u = pp->u;
-            const cd d0 = cd{pp->d0.r, pp->d0.i}, d1 = cd{pp->d1.r, pp->d1.i};
-            cd m00, m01 = cd{0.0, 0.0}, m10 = cd{0.0, 0.0}, m11 = cd{0.0, 0.0};
-            bool acc;
-            if (!pairing) {
-                const cd g00 = ldc(col0, s0);
-                const cd den = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - g00);
-                const cd rdet = cinv(den);
-                const cd z = (cd{ratio0, 0.0} * rdet) * conj(rdet);
-                acc = cabs(z) > u;
-                m00 = rdet * d0;
-            } else {
-                const cd g00 = ldc(col0, s0), g01 = ldc(row0, s1);
-                const cd g10 = ldc(col0, s1), g11 = ldc(col1, s1);
-                const cd p00 = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - g00);
-                const cd p01 = -(d0 * g01);
-                const cd p10 = -(d1 * g10);
-                const cd p11 = cd{1.0, 0.0} + d1 * (cd{1.0, 0.0} - g11);
-                const cd det = p00 * p11 - p01 * p10;
-                acc = fabs(ratio0 / cabs(det)) > u;
-                const cd idet = cinv(det);
-                m00 = (p11 * idet) * d0;
-                m01 = (-(p01 * idet)) * d1;
-                m10 = (-(p10 * idet)) * d0;
-                m11 = (p00 * idet) * d1;
-            }
-            if (acc) ++nacc;
-            if (acc && tid == 0) phi_t[i * 2 + nf - 1] = pp->phi_new;
-            if (acc && owner) {
-                // G -= G[:,S] M (I_S - G[S,:]) on this thread's entries
+        const bool has_next = q + 1 < ns;
+        const int nx = has_next ? site_of(q + 1) : -1;
+        const int nx0 = has_next ? nx : -1, nx1 = has_next ? ns + nx : -1;
+        if (acc) {
+            ++nacc;
+            if (tid == 0) phi_t[i * 2 + nf - 1] = pp->phi_new;
+            if (owner) {
+                // G -= G[:,S] M (I_S - G[S,:]) on this thread's column, publishing the next site
                 cd e0 = -ldc(row0, cc);
                 if (cc == s0) e0 = e0 + cd{1.0, 0.0};
                 cd w0 = m00 * e0, w1 = cd{0.0, 0.0};
@@ -537,60 +491,28 @@ DEV void slice_sites_regs_t(const Ctx& c, const ModelDev& m, const ChainDev& ch,
                     w0 = w0 + m01 * e1;
                     w1 = m10 * e0 + m11 * e1;
                 }
-                if (pairing) {
-#pragma unroll
-                    for (int k = 0; k < kRegRows; ++k) {
-                        const int r = r0 + k * rstep;
-                        g[k] = g[k] - ldc(col0, r) * w0 - ldc(col1, r) * w1;
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < kRegRows; ++k) g[k] = g[k] - ldc(col0, r0 + k * rstep) * w0;
+                const int nslot = slot ^ 1;
+                const bool pub_col = cc == nx0 || (pairing && cc == nx1);
+                auto ncol = colb + (nslot * 2 + (cc == nx0 ? 0 : 1)) * n;
+                auto nrow0 = rowb + (nslot * 2) * n, nrow1 = rowb + (nslot * 2 + 1) * n;
+                for (int r = r0; r < n; r += rstep) {
+                    cd x = ldc(G, r * n + cc) - ldc(col0, r) * w0;
+                    if (pairing) x = x - ldc(col1, r) * w1;
+                    stc(G, r * n + cc, x);
+                    if (pub_col) stc(ncol, r, x);
+                    if (r == nx0) stc(nrow0, cc, x);
+                    if (r == nx1) stc(nrow1, cc, x);
                 }
             }
-            if (q + 1 < ns) publish(slot ^ 1, right ? q + 1 : ns - 2 - q);
-            c.sync();
+        } else if (has_next) {
+            publish_from_g(slot ^ 1, nx);
         }
-        if (tid == 0) {
-            ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
-            ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
-        }
-        if (!right) {
-            hs_factors(c, m, L, phi_t, nf);     // new fields
-            const cd fc = ldc(f, cc);
-#pragma unroll
-            for (int k = 0; k < kRegRows; ++k) {
-                const int r = r0 + k * rstep;
-                if (r < n) g[k] = (ldc(finv, r) * g[k]) * fc;
-            }
-        }
+        c.sync();
     }
-#pragma unroll
-    for (int k = 0; k < kRegRows; ++k) {
-        const int r = r0 + k * rstep;
-        if (owner && r < n) stc(M, r * n + cc, g[k]);
+    if (tid == 0) {
+        ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
+        ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
     }
-    c.sync();
-}
-
-DEVFN void slice_sites_r4(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
-                          int& seed) {
-    slice_sites_regs_t<4>(c, m, ch, L, t, right, seed);
-}
-DEVFN void slice_sites_r12(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
-                           int& seed) {
-    slice_sites_regs_t<12>(c, m, ch, L, t, right, seed);
-}
-DEVFN void slice_sites_r20(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
-                           int& seed) {
-    slice_sites_regs_t<20>(c, m, ch, L, t, right, seed);
-}
-DEV void slice_sites_regs(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int t, bool right,
-                          int& seed) {
-    const int rows = reg_rows_needed(m.n, c.nthreads());
-    if (rows <= 4) slice_sites_r4(c, m, ch, L, t, right, seed);
-    else if (rows <= 12) slice_sites_r12(c, m, ch, L, t, right, seed);
-    else slice_sites_r20(c, m, ch, L, t, right, seed);
 }
 
 // ---------------------------------------------------------------- MFMA GEMM in LDS
@@ -1248,8 +1170,16 @@ DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         lds_gemm(c, G, m.ez[2], n, false);
         lds_gemm(c, G, m.ez[1], n, true);
         tm.stop(c, ch, S_T_KIN);
-        slice_sites_regs(c, m, ch, L, nt - 1, false, seed);
-        tm.stop(c, ch, S_T_SITE);
+        const int t = nt - 1;
+        const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
+        for (int nf = 2; nf >= 1; --nf) {
+            if (!m.active[nf - 1]) continue;
+            site_loop_lds1(c, m, ch, L, G, t, nf, false, seed);
+            tm.stop(c, ch, S_T_SITE);
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat_lds(c, G, n, L.finv, L.f);         // G <- e^{-V} G e^{V}
+            tm.stop(c, ch, S_T_HS);
+        }
     }
     wrap_l_lds(c, m, ch, L, 0);
     tm.stop(c, ch, S_T_WRAP);
@@ -1263,8 +1193,16 @@ DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
     live_identity(c, n, ch.ur, ch.dr, ch.vr);     // Wrap_R(0): R list slot 0 is the identity
     tm.start(c);
     for (int nt = 1; nt <= m.ltrot; ++nt) {
-        slice_sites_regs(c, m, ch, L, nt - 1, true, seed);
-        tm.stop(c, ch, S_T_SITE);
+        const int t = nt - 1;
+        const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
+        for (int nf = 1; nf <= 2; ++nf) {
+            if (!m.active[nf - 1]) continue;
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat_lds(c, G, n, L.f, L.finv);         // G <- e^{V} G e^{-V} (old fields)
+            tm.stop(c, ch, S_T_HS);
+            site_loop_lds1(c, m, ch, L, G, t, nf, true, seed);
+            tm.stop(c, ch, S_T_SITE);
+        }
         // propT_R: G <- (e^{-dtau K} G) e^{+dtau K}; UR's product is applied at the wrap
         lds_gemm(c, G, m.ez[0], n, true);
         lds_gemm(c, G, m.ez[3], n, false);
