@@ -401,120 +401,6 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
     site_loop_t(c, m, ch, L, as_lds(G), t, nf, ascending, seed);
 }
 
-// ---------------------------------------------------------------- LDS-resident site loop, one barrier per site
-// G in LDS; thread (cc, r0) updates column cc, rows r0, r0 + rstep, ... (n <= threads).
-// The column(s) and row(s) of the NEXT site are published into the other slot
-// of a double-buffered LDS snapshot by the threads that produce those entries
-// (while updating, or by copying when the current proposal is rejected), so a
-// single barrier separates consecutive sites.
-DEVFN void site_loop_lds1(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t, int nf,
-                          bool ascending, int& seed) {
-    auto G = as_lds(G_);
-    const int n = m.n, ns = m.ns;
-    const bool pairing = m.kind == 1;
-    const int tid = c.tid(), nth = c.nthreads();
-    double* phi_t = ch.phi + (size_t)t * ns * 2;
-    make_proposals(c, m, L, phi_t, nf, ascending, seed);
-    seed = lcg_jump(seed, 2 * ns);
-    const int rstep = nth / n;
-    const int cc = tid % n, r0 = tid / n;
-    const bool owner = r0 < rstep;
-    auto colb = as_lds(L.col);      // [slot][sector][n]
-    auto rowb = as_lds(L.row);      // [slot][sector][n]
-    auto props = as_lds(L.prop);
-    auto site_of = [&](int q) { return ascending ? q : ns - 1 - q; };
-    // copy the column(s)/row(s) of site i from G into slot
-    auto publish_from_g = [&](int slot, int i) {
-        if (!owner) return;
-        const int s0 = i, s1 = ns + i;
-        if (cc == s0 || (pairing && cc == s1)) {
-            auto col = colb + (slot * 2 + (cc == s0 ? 0 : 1)) * n;
-            for (int r = r0; r < n; r += rstep) stc(col, r, ldc(G, r * n + cc));
-        }
-        const int d0 = s0 - r0;
-        if (d0 >= 0 && d0 % rstep == 0) stc(rowb + (slot * 2) * n, cc, ldc(G, s0 * n + cc));
-        if (pairing) {
-            const int d1 = s1 - r0;
-            if (d1 >= 0 && d1 % rstep == 0) stc(rowb + (slot * 2 + 1) * n, cc, ldc(G, s1 * n + cc));
-        }
-    };
-    publish_from_g(0, site_of(0));
-    c.sync();
-    int nacc = 0;
-    for (int q = 0; q < ns; ++q) {
-        const int i = site_of(q);
-        const int slot = q & 1;
-        const int s0 = i, s1 = ns + i;
-        auto col0 = colb + (slot * 2) * n, col1 = colb + (slot * 2 + 1) * n;
-        auto row0 = rowb + (slot * 2) * n, row1 = rowb + (slot * 2 + 1) * n;
-        auto pp = props + i;
-        const double ratio0 = pp->ratio0, u = pp->u;
-        const cd d0 = cd{pp->d0.r, pp->d0.i}, d1 = cd{pp->d1.r, pp->d1.i};
-        cd m00, m01 = cd{0.0, 0.0}, m10 = cd{0.0, 0.0}, m11 = cd{0.0, 0.0};
-        bool acc;
-        if (!pairing) {
-            const cd g00 = ldc(col0, s0);
-            const cd den = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - g00);
-            const cd rdet = cinv(den);
-            const cd z = (cd{ratio0, 0.0} * rdet) * conj(rdet);
-            acc = cabs(z) > u;
-            m00 = rdet * d0;
-        } else {
-            const cd g00 = ldc(col0, s0), g01 = ldc(row0, s1);
-            const cd g10 = ldc(col0, s1), g11 = ldc(col1, s1);
-            const cd p00 = cd{1.0, 0.0} + d0 * (cd{1.0, 0.0} - g00);
-            const cd p01 = -(d0 * g01);
-            const cd p10 = -(d1 * g10);
-            const cd p11 = cd{1.0, 0.0} + d1 * (cd{1.0, 0.0} - g11);
-            const cd det = p00 * p11 - p01 * p10;
-            acc = fabs(ratio0 / cabs(det)) > u;
-            const cd idet = cinv(det);
-            m00 = (p11 * idet) * d0;
-            m01 = (-(p01 * idet)) * d1;
-            m10 = (-(p10 * idet)) * d0;
-            m11 = (p00 * idet) * d1;
-        }
-        const bool has_next = q + 1 < ns;
-        const int nx = has_next ? site_of(q + 1) : -1;
-        const int nx0 = has_next ? nx : -1, nx1 = has_next ? ns + nx : -1;
-        if (acc) {
-            ++nacc;
-            if (tid == 0) phi_t[i * 2 + nf - 1] = pp->phi_new;
-            if (owner) {
-                // G -= G[:,S] M (I_S - G[S,:]) on this thread's column, publishing the next site
-                cd e0 = -ldc(row0, cc);
-                if (cc == s0) e0 = e0 + cd{1.0, 0.0};
-                cd w0 = m00 * e0, w1 = cd{0.0, 0.0};
-                if (pairing) {
-                    cd e1 = -ldc(row1, cc);
-                    if (cc == s1) e1 = e1 + cd{1.0, 0.0};
-                    w0 = w0 + m01 * e1;
-                    w1 = m10 * e0 + m11 * e1;
-                }
-                const int nslot = slot ^ 1;
-                const bool pub_col = cc == nx0 || (pairing && cc == nx1);
-                auto ncol = colb + (nslot * 2 + (cc == nx0 ? 0 : 1)) * n;
-                auto nrow0 = rowb + (nslot * 2) * n, nrow1 = rowb + (nslot * 2 + 1) * n;
-                for (int r = r0; r < n; r += rstep) {
-                    cd x = ldc(G, r * n + cc) - ldc(col0, r) * w0;
-                    if (pairing) x = x - ldc(col1, r) * w1;
-                    stc(G, r * n + cc, x);
-                    if (pub_col) stc(ncol, r, x);
-                    if (r == nx0) stc(nrow0, cc, x);
-                    if (r == nx1) stc(nrow1, cc, x);
-                }
-            }
-        } else if (has_next) {
-            publish_from_g(slot ^ 1, nx);
-        }
-        c.sync();
-    }
-    if (tid == 0) {
-        ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
-        ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
-    }
-}
-
 // ---------------------------------------------------------------- MFMA GEMM in LDS
 // M (LDS, n x n complex) <- E M (left) or M E (right) with E real, read in
 // MFMA operand order from global memory (coalesced 512 B per k-step).  Each
@@ -1174,7 +1060,7 @@ DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
-            site_loop_lds1(c, m, ch, L, G, t, nf, false, seed);
+            site_loop_lds(c, m, ch, L, G, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
             scale_mat_lds(c, G, n, L.finv, L.f);         // G <- e^{-V} G e^{V}
@@ -1200,7 +1086,7 @@ DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
             hs_factors(c, m, L, phi_t, nf);
             scale_mat_lds(c, G, n, L.f, L.finv);         // G <- e^{V} G e^{-V} (old fields)
             tm.stop(c, ch, S_T_HS);
-            site_loop_lds1(c, m, ch, L, G, t, nf, true, seed);
+            site_loop_lds(c, m, ch, L, G, t, nf, true, seed);
             tm.stop(c, ch, S_T_SITE);
         }
         // propT_R: G <- (e^{-dtau K} G) e^{+dtau K}; UR's product is applied at the wrap
