@@ -1,0 +1,34 @@
+"""Command line: the drop-in for ``mpirun -np R bosonDQMC.out`` in a run directory.
+
+    python -m bafqmc_amd run [--solver number_conserving|pairing] [--chains R] [DIR]
+
+DIR (default: the current directory) holds paramC_sets.txt, confin.txt and
+seeds.txt; the per-bin observable files, info.txt and confout.txt are written
+there, as the reference executable does (src/number_conserving/run_paper.py:113,162
+launches it with cwd = the run directory).  R replaces the MPI rank count.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m bafqmc_amd")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("run", help="run the sweep in a reference run directory")
+    r.add_argument("directory", nargs="?", default=".")
+    r.add_argument("--solver", default=os.environ.get("BAFQMC_SOLVER", "number_conserving"),
+                   choices=["number_conserving", "pairing"])
+    r.add_argument("--chains", type=int, default=int(os.environ.get("BAFQMC_CHAINS", "1")),
+                   help="independent Markov chains (the reference's MPI ranks)")
+    r.add_argument("--no-poles", action="store_true", help="skip the per-bin pole diagnostics")
+    a = ap.parse_args(argv)
+    from .driver import run
+    run(a.directory, a.solver, a.chains, poles=False if a.no_poles else None)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -27,7 +27,8 @@ int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block,
 double rt_clock_hz();          // rate of Ctx::clock()
 int rt_sync();
 const char* rt_error();
-int rt_block_threads();        // threads per workgroup for the sweep kernels
+int rt_block_threads();        // threads per workgroup for the global-path sweep kernels
+int rt_lds_block_threads();    // threads per workgroup for the LDS-path sweep kernels
 int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
 int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
 int rt_debug_mfma(double* out);
@@ -268,9 +269,13 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
         // BAFQ_FORCE_GLOBAL=1 runs every chain on the global-memory path (tests of that path)
         const char* fg = std::getenv("BAFQ_FORCE_GLOBAL");
         const bool force_global = fg && fg[0] == '1';
-        const int path = (m.n <= kLdsMaxN && !force_global) ? P_LDS : P_GLOBAL;
+        // the LDS path needs every MFMA output tile of an N x N matrix covered by the waves
+        const int lds_waves = rt_lds_block_threads() / 64;
+        const int ntile = (m.n + 15) / 16;
+        const bool lds_ok = m.n <= kLdsMaxN && ntile * ntile <= lds_waves * kMaxTiles;
+        const int path = (lds_ok && !force_global) ? P_LDS : P_GLOBAL;
         pids[path].push_back(c);
-        const size_t lb = path == P_LDS ? lds_bytes(m.n, kLdsWaves, true) : lds_bytes(m.n, nw_glb, false);
+        const size_t lb = path == P_LDS ? lds_bytes(m.n, lds_waves, true) : lds_bytes(m.n, nw_glb, false);
         h->max_lds[path] = std::max(h->max_lds[path], lb);
     }
     rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * n_chains);
@@ -341,7 +346,7 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
             if (h->path_count[p] == 0) continue;
             a.ids = h->d_path_ids[p];
             a.count = h->path_count[p];
-            const int block = p == P_LDS ? kLdsWaves * 64 : rt_block_threads();
+            const int block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
             double ms = 0.0;
             if (rt_launch(k, (PathId)p, a, a.count, block, h->max_lds[p], &ms)) return fail(h, rt_error());
             total_ms += ms;

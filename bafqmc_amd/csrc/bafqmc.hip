@@ -106,6 +106,8 @@ double rt_clock_hz() {
     return 1000.0 * khz;
 }
 
+int rt_lds_block_threads() { return kLdsWaves * 64; }
+
 int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
     static hipEvent_t ev[2] = {nullptr, nullptr};
     if (!ev[0]) {

@@ -61,6 +61,15 @@ int rt_block_threads() {
 
 double rt_clock_hz() { return 1.0e9; }
 
+// The emulator may run the LDS path with fewer waves (small N only; the host
+// checks that the waves cover every MFMA tile).
+int rt_lds_block_threads() {
+    const char* s = std::getenv("BAFQ_LDS_BLOCK");
+    int b = s ? std::atoi(s) : 128;
+    if (b < 64 || b > 512 || (b % 64) != 0) b = 128;
+    return b;
+}
+
 int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
     if (ms) *ms = 0.0;
     if (k == K_WARMUP) {

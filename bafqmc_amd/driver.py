@@ -1,0 +1,192 @@
+"""Drop-in replacement of the reference executables' main program.
+
+``run(directory, solver, nchains)`` reads the reference's input files from a run
+directory and writes its output files there, like ``mpirun -np R bosonDQMC.out``
+does (src/number_conserving/src/main.f90:22-83; src/pairing/src/main.f90),
+with the R MPI ranks replaced by R chains of one batched GPU solver:
+
+* inputs: ``paramC_sets.txt``, ``confin.txt``, ``seeds.txt``
+  (src/number_conserving/src/calc_basic.f90:48-91, fields.f90:47-113);
+* per bin, one appended row per observable file, averaged over chains like the
+  per-bin ``MPI_REDUCE(SUM)/ISIZE`` (src/number_conserving/src/fourier_trans.f90:350-457,
+  file list :259-345; pairing src/pairing/src/fourier_trans.f90:275-406);
+* number-conserving pole diagnostics per chain and bin (pole_diagnostics.f90:91-187);
+* at the end ``info.txt`` and ``confout.txt`` (main.f90:59-76, fields.f90:236-278).
+
+The time-displaced path (``is_tau``) writes no estimators in the reference
+(src/number_conserving/src/obser_tau.f90:39-54) and is not run here.
+"""
+from __future__ import annotations
+
+import math
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from .model import ModelSpec, Params, read_params, read_fields, write_fields
+from .solver import BatchedSolver, NC_OUTPUTS, PAIRING_OUTPUTS
+
+TINY = np.finfo(float).tiny
+
+
+def _fmt(v: float) -> str:
+    return f" {v: .16E}"
+
+
+def _append(path: Path, text: str):
+    with open(path, "a") as f:
+        f.write(text)
+
+
+def write_bin(run_dir: Path, per_chain: list, pairing: bool):
+    """Chain-averaged row of every observable file (m_write_obs_equal)."""
+    names = PAIRING_OUTPUTS if pairing else NC_OUTPUTS
+    nc = len(per_chain)
+    for name in names:
+        vals = [b[name] for b in per_chain]
+        if name == "density_site_total":
+            v = np.mean(np.stack(vals), axis=0)
+            if pairing:
+                line = "".join(f" {x:24.16E}" for x in v) + "\n"
+            else:
+                line = "".join(f"{x:24.16E} " for x in v) + "\n"
+        elif isinstance(vals[0], complex):
+            z = sum(vals) / nc
+            line = _fmt(z.real) + _fmt(z.imag) + "\n"
+        else:
+            line = _fmt(sum(vals) / nc) + "\n"
+        _append(run_dir / name, line)
+
+
+def pole_diagnostics(g: np.ndarray, phi: np.ndarray):
+    """calc_local_diagnostics (pole_diagnostics.f90:91-132): poles z = 1/eig(G), |.|, SVD."""
+    ev = np.linalg.eigvals(g)
+    sv = np.linalg.svd(g, compute_uv=False)
+    z = np.where(np.abs(ev) <= TINY, 1.0 / TINY + 0j, 1.0 / np.where(ev == 0, 1.0, ev))
+    pole_distance = max(float(np.min(np.abs(z))), TINY)
+    log_weight = -0.5 * float(np.sum(phi * phi)) + 2.0 * float(np.sum(np.log(np.maximum(sv, TINY))))
+    return z, (pole_distance, -math.log10(pole_distance), float(np.max(np.abs(ev))), float(np.max(sv)),
+               log_weight)
+
+
+def write_poles(run_dir: Path, zs: list, scalars: list):
+    """append_pole_z / append_scalar_column (pole_diagnostics.f90:134-156)."""
+    lines = []
+    for z in zs:
+        lines.append("".join(f" {x.real:24.16E} {x.imag:24.16E}" for x in z) + "\n")
+    _append(run_dir / "pole_z", "".join(lines))
+    for col, name in enumerate(("pole_distance", "pole_x", "green_spectral_radius", "green_smax", "log_weight")):
+        _append(run_dir / name, "".join(f"{s[col]:24.16E}\n" for s in scalars))
+
+
+def info_header(p: Params, nchains: int, spectrum) -> str:
+    """write_info (calc_basic.f90:138-174) plus the single-particle spectrum (model.f90:27-29)."""
+    rows = [
+        " =========================",
+        " DQMC for pairing Bose Hubbard on triangular lattice" if p.pairing
+        else " DQMC for boson Hubbard on triangular lattice",
+        f" Linear lengh Lx                                : {p.nlx}",
+        f" Linear lengh Ly                                : {p.nly}",
+        f" Hopping t                                      : {1.0}",
+        f" Hubbard U1 (relative-density, repulsive)       : {p.u1}",
+        f" Hubbard U2 (total-density, attractive)         : {p.u2}",
+    ]
+    if p.pairing:
+        rows.append(f" pairing Delta                                  : {p.rdelta}")
+    rows += [
+        f" chemical potential                             : {p.mu}",
+        f" Local update auxiliary field magnitude Shift   : {p.shift_loc}",
+    ]
+    if p.is_warm:
+        rows += [f" # Warm                                         : {p.nwarm}",
+                 f" Thermalize U1 relative-field magnitude          : {p.shift_warm1}",
+                 f" Thermalize U2 total-field magnitude             : {p.shift_warm2}"]
+    rows += [
+        f" Choosing initial distribution type             : {p.ini_type}",
+        f" Magnitude of Gaussian distribution             : {p.ini_ampl}",
+        f" Beta                                           : {p.beta}",
+        f" Trotter number                                 : {p.ltrot}",
+        f" =>Dtau                                         : {p.dtau}",
+        f" N_Ortho                                        : {p.nwrap}",
+        f" # Bins                                         : {p.nbin}",
+        f" # Nsweep in one bin                            : {p.nsweep}",
+        f" # Cores                                        : {nchains}",
+        f" highest energy level of single particle Ham     : {float(np.max(spectrum))}",
+        f" lowest  energy level of single particle Ham     : {float(np.min(spectrum))}",
+        f" bandwidth            of single particle Ham     : {float(np.max(spectrum) - np.min(spectrum))}",
+    ]
+    return "\n".join(rows) + "\n"
+
+
+def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, poles: bool | None = None,
+        log=sys.stdout):
+    """One reference run in ``run_dir``; returns a dict of run statistics."""
+    t0 = time.perf_counter()
+    run_dir = Path(run_dir)
+    pairing = solver in ("pairing", "pa", "nambu")
+    p = read_params(run_dir / "paramC_sets.txt", pairing)
+    if poles is None:
+        poles = not pairing          # the pairing executable writes no pole files
+    phis, seeds = read_fields(run_dir, p, nchains)
+    spec = ModelSpec(p)
+    (run_dir / "info.txt").write_text(info_header(p, nchains, spec.spectrum))
+    s = BatchedSolver([spec], [0] * nchains, lib=lib)
+    try:
+        s.set_fields(list(phis), seeds)
+        therm = None
+        if p.is_warm and p.nwarm > 0:
+            # Local_sweep_therm per warm-up sweep (local_sweep.f90:64-78)
+            s.warmup(p.nwarm)
+            therm = s.read_bin()
+        else:
+            _append(run_dir / "info.txt", " Skipping Bosonic warm-up\n")
+        s.prepare()
+        s.read_bin()           # clear prepare's counters
+        acc = np.zeros(2)
+        xmax, xmean, nwrap_total = 0.0, 0.0, 0.0
+        for b in range(p.nbin):
+            s.sweep(p.nsweep, measure=True)
+            per = s.read_bin()
+            write_bin(run_dir, per, pairing)
+            for d in per:
+                st = d["_stats"]
+                if st["prop_u1"] > 0:
+                    acc[0] += st["acc_u1"] / st["prop_u1"]
+                if st["prop_u2"] > 0:
+                    acc[1] += st["acc_u2"] / st["prop_u2"]
+                xmax = max(xmax, st["xmax"])
+                xmean += st["xmean_sum"]
+                nwrap_total += st["nwrap"]
+            if poles:
+                fields, _ = s.get_fields()
+                zs, scal = [], []
+                for c in range(nchains):
+                    z, sc = pole_diagnostics(s.green(c), fields[c])
+                    zs.append(z)
+                    scal.append(sc)
+                write_poles(run_dir, zs, scal)
+        fields, seeds_out = s.get_fields()
+    finally:
+        s.close()
+    write_fields(run_dir, fields, seeds_out)
+    wall = time.perf_counter() - t0
+    tail = []
+    if therm is not None:
+        for k, name in ((0, "U1"), (1, "U2")):
+            prop = sum(d["_stats"][f"therm_prop_u{k + 1}"] for d in therm)
+            if prop > 0:
+                accept = sum(d["_stats"][f"therm_acc_u{k + 1}"] for d in therm) / prop
+                tail.append(f" Thermalize {name} Accept Ratio                     : {accept}")
+    denom = max(nchains * p.nbin, 1)
+    if abs(p.u1) > 1e-10:
+        tail.append(f" Accept_U1_shift                                : {acc[0] / denom}")
+    if abs(p.u2) > 1e-10:
+        tail.append(f" Accept_U2_shift                                : {acc[1] / denom}")
+    tail += [f" Max diff Matrix                                : {xmax}",
+             f" Mean diff Matrix                               : {xmean / max(nwrap_total, 1.0)}",
+             f" Tot CPU time                                   : {wall}"]
+    _append(run_dir / "info.txt", "\n".join(tail) + "\n")
+    print(f"bafqmc_amd: {nchains} chains x {p.nbin} bins in {wall:.2f} s", file=log)
+    return {"wall_s": wall, "accept": acc / denom, "xmax": xmax}

@@ -22,6 +22,9 @@ def hip_lib():
 
 @pytest.fixture(scope="session")
 def emu_lib():
+    # small workgroups keep the thread-per-lane emulator fast (small N only)
+    os.environ.setdefault("BAFQ_BLOCK", "64")
+    os.environ.setdefault("BAFQ_LDS_BLOCK", "128")
     from bafqmc_amd.build import EMU_LIB, build_emu
     if not EMU_LIB.exists():
         build_emu()

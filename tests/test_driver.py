@@ -1,0 +1,96 @@
+"""The drop-in run-directory driver: reference input grammar in, reference output layout out.
+
+CPU tests run the kernels on the emulator back end on the smallest golden cases;
+the GPU test runs a two-chain case (seeds.txt field generation, chain averaging).
+"""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+from bafqmc_amd import driver
+from bafqmc_amd.model import Params, read_params
+from oracle import run_ref
+from tests.helpers import load_golden, golden_row, assert_obs_close
+
+
+def _write_case(tmp_path, name):
+    z, pd = load_golden(name)
+    p = dict(pd)
+    d = tmp_path / name
+    fields = list(z["fields_in"]) if "fields_in" in z.files else None
+    run_ref.write_inputs(d, p, [int(s) for s in z["seeds"]], fields)
+    return z, pd, d
+
+
+def _check_outputs(z, pd, d):
+    for key in z.files:
+        if not key.startswith("out_"):
+            continue
+        name = key[4:]
+        rows = run_ref.read_rows(d / name)
+        ref = z[key]
+        if name == "pole_z":
+            # eigenvalue order is not defined; compare the sorted moduli
+            got = rows.reshape(rows.shape[0], -1, 2)
+            want = ref.reshape(ref.shape[0], -1, 2)
+            gm = np.sort(np.hypot(got[..., 0], got[..., 1]), axis=1)
+            wm = np.sort(np.hypot(want[..., 0], want[..., 1]), axis=1)
+            np.testing.assert_allclose(gm, wm, rtol=1e-9, err_msg=name)
+            continue
+        assert rows.shape == ref.shape, (name, rows.shape, ref.shape)
+        for b in range(rows.shape[0]):
+            want = ref[b]
+            got = rows[b]
+            if want.size == 2 and name not in ("density_site_total",):
+                assert_obs_close(complex(got[0], got[1]), complex(want[0], want[1]), f"{name} row {b}")
+            else:
+                assert_obs_close(got, want, f"{name} row {b}", atol=1e-11)
+    toks = (d / "confout.txt").read_text().split()
+    assert int(toks[0]) == int(z["seeds_out"][0])
+    nvals = z["fields_out"][0].size
+    np.testing.assert_allclose(np.array([float(t) for t in toks[1:1 + nvals]]), z["fields_out"][0].ravel(),
+                               rtol=0, atol=1e-12)
+    info = (d / "info.txt").read_text()
+    assert "Tot CPU time" in info and "Max diff Matrix" in info
+
+
+def test_read_params_roundtrip(tmp_path):
+    for pairing in (False, True):
+        p = Params(pairing=pairing, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5 if pairing else 0.0, nlx=6, nly=6,
+                   ltrot=80, beta=8.0, nlx_therm=6, nly_therm=6, ltrot_therm=80, nwrap=10, nbin=3, nsweep=2,
+                   shift_loc=1.0, is_warm=True, nwarm=5)
+        f = tmp_path / f"p{int(pairing)}.txt"
+        f.write_text(p.to_text())
+        q = read_params(f, pairing)
+        assert q == p
+
+
+def test_read_params_validation(tmp_path):
+    p = Params(u1=-1.0)
+    f = tmp_path / "bad.txt"
+    f.write_text(p.to_text())
+    with pytest.raises(ValueError, match="U1 must be non-negative"):
+        read_params(f, False)
+    p = Params(ltrot=40, nwrap=7, nlx_therm=3, nly_therm=3, ltrot_therm=40)
+    f.write_text(p.to_text())
+    with pytest.raises(ValueError, match="multiple of Nwrap"):
+        read_params(f, False)
+
+
+@pytest.mark.parametrize("name,solver", [("nc_fixed_3x3_w2", "number_conserving")])
+def test_driver_matches_reference_outputs_emulator(tmp_path, emu_lib, name, solver):
+    z, pd, d = _write_case(tmp_path, name)
+    driver.run(d, solver, len(z["seeds"]), lib=emu_lib)
+    _check_outputs(z, pd, d)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,solver", [("nc_traj_3x3_c1", "number_conserving"),
+                                         ("pairing_traj_3x3", "pairing")])
+def test_driver_matches_reference_outputs_gpu(tmp_path, hip_lib, name, solver):
+    z, pd, d = _write_case(tmp_path, name)
+    driver.run(d, solver, len(z["seeds"]), lib=hip_lib)
+    _check_outputs(z, pd, d)

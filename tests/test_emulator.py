@@ -37,11 +37,17 @@ def test_emu_gemm_qrp(emu_lib):
         assert rel_err(Q @ R, a[:, perm]) < 1e-13
 
 
-@pytest.mark.parametrize("case", [0, 1, 2, 3])
-def test_emu_sweep_matches_oracle(emu_lib, case):
+@pytest.mark.parametrize("case", [1, 3])
+def test_emu_lds_path_matches_oracle(emu_lib, case):
     check_sweep_vs_oracle(emu_lib, case)
 
 
-@pytest.mark.parametrize("name", ["nc_fixed_3x3_w1", "pairing_fixed_3x3_w2", "nc_traj_3x3_u2"])
+@pytest.mark.parametrize("case", [1, 3])
+def test_emu_global_path_matches_oracle(emu_lib, case, monkeypatch):
+    monkeypatch.setenv("BAFQ_FORCE_GLOBAL", "1")
+    check_sweep_vs_oracle(emu_lib, case)
+
+
+@pytest.mark.parametrize("name", ["pairing_fixed_3x3_w2"])
 def test_emu_matches_reference_fortran(emu_lib, name):
     check_golden(emu_lib, name)
