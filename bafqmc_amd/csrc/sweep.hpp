@@ -40,9 +40,11 @@ enum StatIndex {
     S_PROP_U1 = 0, S_ACC_U1, S_PROP_U2, S_ACC_U2, S_XMAX, S_XMEAN_SUM, S_NWRAP, S_LOGDET,
     S_THERM_PROP_U1, S_THERM_ACC_U1, S_THERM_PROP_U2, S_THERM_ACC_U2,
     // device-clock ticks spent per phase (thread 0 of the chain's workgroup)
-    S_T_SITE, S_T_KIN, S_T_MEAS, S_T_WRAP, S_T_HS, S_LEN = 20
+    S_T_SITE, S_T_KIN, S_T_MEAS, S_T_WRAP, S_T_HS,
+    // parts of S_T_WRAP: B-products of the U factor, the two pivoted QRs, the rest of stab_green
+    S_T_WPROD, S_T_WQR, S_T_WGREEN, S_LEN = 20
 };
-constexpr int kNumTimers = 5;
+constexpr int kNumTimers = 8;
 
 constexpr int kLdsWaves = 8;      // LDS path: 512 threads
 constexpr int kLdsMaxN = 96;      // LDS path: N <= 96 (G in LDS, <= 36 MFMA tiles)
@@ -990,13 +992,18 @@ DEVFN void apply_slices_left(const Ctx& c, const ModelDev& m, const ChainDev& ch
 // Wrap_R at nt with G in LDS (L.mat): UR <- B(nt, nt-Nwrap] UR, stab_UR, stab_green.
 DEVFN void wrap_r_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
     const int n = m.n, st = nt / m.nwrap;
+    Timer tm;
+    tm.start(c);
     global_from_lds(c, ch.g, L.mat, n);                 // wrapped G, for the drift check
     lds_from_global(c, L.mat, ch.ur, n);
     apply_slices_right(c, m, ch, L, L.mat, nt - m.nwrap + 1, nt);
     scale_mat_cols_real(c, L.mat, n, ch.dr);
+    tm.stop(c, ch, S_T_WPROD);
     stab_ur_from(c, m, ch, L, L.mat);
+    tm.stop(c, ch, S_T_WQR);
     const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, list_mat(ch.ullist, st, n),
                                  list_vec(ch.dllist, st, n), list_mat(ch.vllist, st, n), L.mat, ch.wa);
+    tm.stop(c, ch, S_T_WGREEN);
     store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
     lds_from_global(c, L.mat, ch.wa, n);
     store_right(c, m, ch, st);
@@ -1005,14 +1012,19 @@ DEVFN void wrap_r_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const
 // Wrap_L at nt with G in LDS: UL <- UL B(nt+Nwrap, nt], stab_UL, stab_green.
 DEVFN void wrap_l_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
     const int n = m.n, st = nt / m.nwrap;
+    Timer tm;
+    tm.start(c);
     global_from_lds(c, ch.g, L.mat, n);
     lds_from_global(c, L.mat, ch.ul, n);
     apply_slices_left(c, m, ch, L, L.mat, nt + 1, nt + m.nwrap);
     global_from_lds(c, ch.ul, L.mat, n);
     load_ud(c, L.mat, ch.ul, ch.dl, n, true);
+    tm.stop(c, ch, S_T_WPROD);
     stab_ul_from(c, m, ch, L, L.mat);
+    tm.stop(c, ch, S_T_WQR);
     const double ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
                                  list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, L.mat, ch.wa);
+    tm.stop(c, ch, S_T_WGREEN);
     store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
     lds_from_global(c, L.mat, ch.wa, n);
     store_left(c, m, ch, st);

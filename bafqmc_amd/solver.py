@@ -28,7 +28,8 @@ O_NOBS = 32
 O_SITE0 = 40
 STATS = {"prop_u1": 0, "acc_u1": 1, "prop_u2": 2, "acc_u2": 3, "xmax": 4, "xmean_sum": 5,
          "nwrap": 6, "logdet": 7, "therm_prop_u1": 8, "therm_acc_u1": 9, "therm_prop_u2": 10,
-         "therm_acc_u2": 11, "t_site": 12, "t_kinetic": 13, "t_measure": 14, "t_wrap": 15, "t_hs": 16}
+         "therm_acc_u2": 11, "t_site": 12, "t_kinetic": 13, "t_measure": 14, "t_wrap": 15, "t_hs": 16,
+         "t_wrap_products": 17, "t_wrap_stab_u": 18, "t_wrap_green": 19}
 
 NC_OUTPUTS = ["density_up", "density_do", "density_total", "density_site_total", "kinetic", "doubleOcc",
               "squareOcc", "num_up", "num_do", "numsquare_up", "numsquare_do", "onsite_n2_up",

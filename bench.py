@@ -133,14 +133,14 @@ def main():
         s.read_bin()
     sf = np.zeros((nchains, args.steps))
     kernel_ms = []
-    phase = np.zeros(5)
+    phase = np.zeros(8)
     t0 = time.perf_counter()
     for k in range(args.steps):
         s.sweep(1)
         kernel_ms.append(s.last_kernel_ms())
         obs, st = s.read_bin_raw()
         sf[:, k] = obs[:, 20] / obs[:, 32]
-        phase += st[:, 12:17].sum(axis=0)
+        phase += st[:, 12:20].sum(axis=0)
     wall = time.perf_counter() - t0
     ms_per_step = 1000.0 * wall / args.steps
     tau, window, _ = tau_int(sf)
@@ -167,7 +167,8 @@ def main():
                    "nwrap": p.nwrap, "sweeps_per_s": sweeps_per_s, "tau_int_sf_k": tau,
                    "tau_window": window, "kernel": "k_sweep_lds" if p.n <= 96 else "k_sweep",
                    "phase_ms_per_chain_sweep": {k: 1e3 * v / (nchains * args.steps) for k, v in
-                                                zip(("site", "kinetic", "measure", "wrap", "hs"), phase)}},
+                                                zip(("site", "kinetic", "measure", "wrap", "hs", "wrap_products",
+                                                     "wrap_stab_u", "wrap_green"), phase)}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel_ms": avg_ms, "flops_per_launch": f_chain * nchains},
