@@ -36,6 +36,9 @@ HOSTDEV cd cdiv(cd a, cd b) {
     return cd{(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
 }
 HOSTDEV cd cinv(cd b) { return cdiv(cd{1.0, 0.0}, b); }
+// Component-wise select.  A conditional expression of struct type can leave the
+// operands in private memory (a select between their addresses); this cannot.
+HOSTDEV cd csel(bool p, cd a, cd b) { return cd{p ? a.r : b.r, p ? a.i : b.i}; }
 // exp of a complex number, Fortran semantics exp(re) * (cos(im), sin(im)).
 HOSTDEV cd cexp(cd z) {
     const double e = exp(z.r);
@@ -125,30 +128,53 @@ DEV cd load_op(const cd* M, bool conj_t, int row, int col, int n) {
     return conj_t ? conj(M[col * n + row]) : M[row * n + col];
 }
 
-DEVFN void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool opB, int n) {
-    const int nt = (n + 15) >> 4;
+// Complex n x n product on FP64 MFMA 16x16x4: C[r][j] = sum_k a(r, k) b(k, j).
+// Each wavefront owns 16x16 output tiles; operand fragments for kChunk k-steps
+// are fetched before they are consumed so their latencies overlap.  The
+// loaders la(r, k), lb(k, j) apply any transposition, conjugation, masking or
+// permutation; st(r, j, v) stores one result.
+template <class LA, class LB, class ST>
+DEV void mfma_gemm(const Ctx& c, int n, LA la, LB lb, ST st) {
+    constexpr int kChunk = 4;
+    const int nt = (n + 15) >> 4, nk = (n + 3) >> 2;
     const int lane = c.lane();
     const int li = lane & 15, lk = lane >> 4;
     for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
         const int ti = t / nt, tj = t - ti * nt;
         const int ar = 16 * ti + li, bc = 16 * tj + li;
         d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < n; k0 += 4) {
-            const int k = k0 + lk;
-            cd a = cd{0.0, 0.0}, b = cd{0.0, 0.0};
-            if (ar < n && k < n) a = load_op(A, opA, ar, k, n);
-            if (bc < n && k < n) b = load_op(B, opB, k, bc, n);
-            accr = c.mfma(a.r, b.r, accr);
-            accr = c.mfma(-a.i, b.i, accr);
-            acci = c.mfma(a.r, b.i, acci);
-            acci = c.mfma(a.i, b.r, acci);
+        for (int k0 = 0; k0 < nk; k0 += kChunk) {
+            cd a[kChunk], b[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int k = 4 * (k0 + u) + lk;
+                a[u] = cd{0.0, 0.0};
+                b[u] = cd{0.0, 0.0};
+                if (k < n && ar < n) a[u] = la(ar, k);
+                if (k < n && bc < n) b[u] = lb(k, bc);
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                if (k0 + u < nk) {
+                    accr = c.mfma(a[u].r, b[u].r, accr);
+                    accr = c.mfma(-a[u].i, b[u].i, accr);
+                    acci = c.mfma(a[u].r, b[u].i, acci);
+                    acci = c.mfma(a[u].i, b[u].r, acci);
+                }
+            }
         }
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
-            if (row < n && col < n) C[row * n + col] = cd{accr[r], acci[r]};
+            if (row < n && col < n) st(row, col, cd{accr[r], acci[r]});
         }
     }
     c.sync();
+}
+
+DEVFN void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool opB, int n) {
+    mfma_gemm(
+        c, n, [&](int r, int k) { return load_op(A, opA, r, k, n); },
+        [&](int k, int j) { return load_op(B, opB, k, j, n); }, [&](int r, int j, cd v) { C[r * n + j] = v; });
 }
 
 // Real-by-complex: side 0: C = E * X ; side 1: C = X * E.  E real row-major.
@@ -404,34 +430,323 @@ DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n) {
     }
 }
 
-// Y = V * X[perm, :]   (ZLAPMR forward + ZTRMM 'L','U','N').
+// Y = V * X[perm, :]   (ZLAPMR forward + ZTRMM 'L','U','N'); V = upper triangle of A.
 DEVFN void trmm_perm_rows(const Ctx& c, cd* Y, const cd* A, const cd* X, const int* perm, int n) {
-    const int nn = n * n;
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
-        const int r = e / n, j = e - r * n;
-        double sr = 0.0, si = 0.0;
-        for (int k = r; k < n; ++k) {
-            const cd p2 = A[r * n + k] * X[perm[k] * n + j];
-            sr += p2.r; si += p2.i;
-        }
-        Y[e] = cd{sr, si};
-    }
-    c.sync();
+    mfma_gemm(
+        c, n, [&](int r, int k) { return csel(k >= r, A[r * n + k], cd{0.0, 0.0}); },
+        [&](int k, int j) { return X[perm[k] * n + j]; }, [&](int r, int j, cd v) { Y[r * n + j] = v; });
 }
 
 // Y = X[:, perm] * V^H  (ZLAPMT forward + ZTRMM 'R','U','C').
 DEVFN void perm_cols_times_vh(const Ctx& c, cd* Y, const cd* X, const cd* A, const int* perm, int n) {
-    const int nn = n * n;
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
-        const int r = e / n, j = e - r * n;
-        double sr = 0.0, si = 0.0;
-        for (int k = j; k < n; ++k) {
-            const cd p2 = X[r * n + perm[k]] * conj(A[j * n + k]);
-            sr += p2.r; si += p2.i;
+    mfma_gemm(
+        c, n, [&](int r, int k) { return X[r * n + perm[k]]; },
+        [&](int k, int j) { return csel(k >= j, conj(A[j * n + k]), cd{0.0, 0.0}); },
+        [&](int r, int j, cd v) { Y[r * n + j] = v; });
+}
+
+// ------------------------------------ column-parallel QR family (LDS, n <= 96)
+// One aligned quad of lanes carries one column of length n <= 96 in registers:
+// row r lives in lane r % 4 of the quad, register slot r / 4, and inner
+// products over a column are reduced with two DPP quad permutations.  Each
+// column goes through the whole chain of reflectors (or substitution steps)
+// without a workgroup barrier or a memory round trip per step.  R is the
+// number of register slots (4 R >= n); the factor matrix A is in LDS.  Loops
+// over slots are branch-free: out-of-range rows are clamped for the load and
+// masked to zero.
+constexpr int kColGroup = 4;
+constexpr int kColMaxN = 96;
+
+// Compiler-only fence: LDS values loaded before it are loaded again after it
+// instead of being kept live in registers.
+DEV void reload_fence() {
+#ifndef BAFQ_EMU
+    asm volatile("" ::: "memory");
+#endif
+}
+
+DEV bool col_path(const Ctx& c, int n, bool a_lds) {
+    return a_lds && n <= kColMaxN && c.nthreads() >= kColGroup * n;
+}
+
+// Entry (r, k) of reflector k (unit at r == k, zero above the diagonal and
+// beyond row n - 1).
+DEV cd refl_entry(LDS_AS const cd* A, int n, int r, int k) {
+    const int rc = r < n ? r : n - 1;
+    const cd v = ldc(A, rc * n + k);
+    const bool below = r > k && r < n;
+    return cd{below ? v.r : (r == k ? 1.0 : 0.0), below ? v.i : 0.0};
+}
+
+// x <- (I - t v v^H) x, v = reflector k stored below the diagonal of A.
+template <int R>
+DEV void refl_apply_col(const Ctx& c, cd (&x)[R], LDS_AS const cd* A, int n, int k, cd t, int gl) {
+    // slots whose four rows all lie above row k or beyond row n - 1 are skipped
+    // (k and n are uniform, so these are scalar branches)
+    // The reflector is read twice from LDS rather than held in registers.
+    double wr = 0.0, wi = 0.0;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (kColGroup * i + kColGroup - 1 >= k && kColGroup * i < n) {
+            const cd v = refl_entry(A, n, gl + kColGroup * i, k);
+            wr += v.r * x[i].r + v.i * x[i].i;
+            wi += v.r * x[i].i - v.i * x[i].r;
         }
-        Y[e] = cd{sr, si};
+    }
+    const cd s = t * cd{c.quad_sum(wr), c.quad_sum(wi)};
+    reload_fence();
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        if (kColGroup * i + kColGroup - 1 >= k && kColGroup * i < n)
+            x[i] = x[i] - refl_entry(A, n, gl + kColGroup * i, k) * s;
+}
+
+// Householder QR with column pivoting (ZGEQP3 semantics, exact column norms)
+// with the same outputs as qrp().  Quad q owns original column q for the whole
+// factorization; step k picks the largest remaining norm (ties to the lower
+// original index), the owner forms the reflector and publishes column k of the
+// result (V = D^{-1} R above the diagonal, reflector below), the other quads
+// apply it to their register columns: two barriers per step.
+template <int R>
+DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, double* D_) {
+    LDS_AS cd* tau = as_lds(qs.tau);
+    LDS_AS int* perm = as_lds(qs.perm);
+    LDS_AS double* nrm = as_lds(qs.nrm);
+    LDS_AS double* D = as_lds(D_);
+    const int col = c.tid() / kColGroup, gl = c.tid() % kColGroup;
+    const int colc = col < n ? col : n - 1;
+    bool active = col < n;
+    cd x[R];
+    double a2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = gl + kColGroup * i;
+        const cd v = ldc(A, (r < n ? r : n - 1) * n + colc);
+        const bool in = active && r < n;
+        x[i] = cd{in ? v.r : 0.0, in ? v.i : 0.0};
+        a2 += abs2(x[i]);
+    }
+    a2 = c.quad_sum(a2);
+    if (active && gl == 0) nrm[col] = a2;
+    c.sync();
+#pragma unroll 1
+    for (int k = 0; k < n; ++k) {
+        // ---- pivot (every wavefront computes the same answer)
+        double bv = -1.0;
+        int bi = n;
+        for (int j = c.lane(); j < n; j += 64) {
+            const double v = nrm[j];
+            if (v > bv) {
+                bv = v;
+                bi = j;
+            }
+        }
+        c.wave_argmax(bv, bi);
+        const bool piv = col == bi;
+        // ---- reflector of the pivot column (ZLARFG conventions), owner wavefront only
+        if (c.wave() == bi * kColGroup / 64) {
+        double ar = 0.0, ai = 0.0, x2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = gl + kColGroup * i;
+            const bool isk = piv && r == k, below = piv && r > k;
+            ar += isk ? x[i].r : 0.0;
+            ai += isk ? x[i].i : 0.0;
+            x2 += below ? abs2(x[i]) : 0.0;
+        }
+        ar = c.quad_sum(ar);
+        ai = c.quad_sum(ai);
+        x2 = c.quad_sum(x2);
+        if (piv) {
+            double beta;
+            cd t, scale;
+            if (x2 == 0.0 && ai == 0.0) {
+                beta = ar;
+                t = cd{0.0, 0.0};
+                scale = cd{0.0, 0.0};
+            } else {
+                const double nr = sqrt(ar * ar + ai * ai + x2);
+                beta = ar >= 0.0 ? -nr : nr;
+                t = cd{(beta - ar) / beta, -ai / beta};
+                scale = cinv(cd{ar - beta, ai});
+            }
+            // column k of the result: R above the diagonal (normalised at the
+            // end), beta on it, the reflector below
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = gl + kColGroup * i;
+                const cd v = csel(r < k, x[i], csel(r == k, cd{beta, 0.0}, x[i] * scale));
+                if (r < n) stc(A, r * n + k, v);
+            }
+            if (gl == 0) {
+                stc(tau, k, t);
+                perm[k] = col;
+                D[k] = fabs(beta);
+                nrm[col] = -1.0;
+            }
+            active = false;
+        }
+        }
+        c.sync();
+        // ---- H(k)^H = I - conj(tau) v v^H on the remaining columns, new norms
+        if (k + 1 < n) {
+            if (c.wave_any(active)) {
+                const cd tk = ldc(tau, k);
+                refl_apply_col<R>(c, x, A, n, k, csel(active, conj(tk), cd{0.0, 0.0}), gl);
+                a2 = 0.0;
+#pragma unroll
+                for (int i = 0; i < R; ++i)
+                    if (kColGroup * i + kColGroup - 1 > k) a2 += (gl + kColGroup * i > k) ? abs2(x[i]) : 0.0;
+                a2 = c.quad_sum(a2);
+                if (active && gl == 0) nrm[col] = a2;
+            }
+            c.sync();
+        }
     }
     c.sync();
+    // V = D^{-1} R
+    for (int e = c.tid(); e < n * n; e += c.nthreads()) {
+        const int r = e / n, j = e - r * n;
+        if (j >= r) {
+            const double d = D[r];
+            const cd a = ldc(A, e);
+            stc(A, e, cd{a.r / d, a.i / d});
+        }
+    }
+    c.sync();
+}
+
+// Q = H(0) ... H(n-1) (ZUNGQR), column j = H(0) ... H(n-1) e_j; writes Q, or
+// Q^H when conj_t.
+template <int R>
+DEV void form_q_cols_t(const Ctx& c, cd* Q, LDS_AS const cd* A, const cd* tau_, int n, bool conj_t) {
+    LDS_AS const cd* tau = as_lds(tau_);
+    GLB_AS cd* Qg = as_global(Q);
+    const int gl = c.tid() % kColGroup, ngrp = c.nthreads() / kColGroup;
+    for (int jb = 0; jb < n; jb += ngrp) {
+        const int j = jb + c.tid() / kColGroup;
+        cd x[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) x[i] = cd{gl + kColGroup * i == j ? 1.0 : 0.0, 0.0};
+        // H(k) e_j = e_j for k > j: start at the wavefront's largest column
+        const int jmax = jb + (c.wave() * 64 + 63) / kColGroup;
+        const int jmin = jb + c.wave() * 64 / kColGroup;
+#pragma unroll 1
+        for (int k = (jmax < n - 1 ? jmax : n - 1); k >= 0 && jmin < n; --k)
+            refl_apply_col<R>(c, x, A, n, k, ldc(tau, k), gl);
+        if (j < n) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = gl + kColGroup * i;
+                if (r < n) {
+                    if (conj_t) stc(Qg, j * n + r, conj(x[i]));
+                    else stc(Qg, r * n + j, x[i]);
+                }
+            }
+        }
+    }
+    c.sync();
+}
+
+// Y[perm[r], j] = (V^{-1} D^{-1} Q^H X)[r, j] with X = Xs (conj_t false) or
+// Xs^H (conj_t true): ZUNMQR('L','C'), the 1/D row scaling, ZTRSM('L','U','N','N')
+// and ZLAPMR backward of stabilization.f90:159-168, fused per column.
+template <int R>
+DEV void solve_cols_t(const Ctx& c, cd* Y, const cd* Xs, bool conj_t, LDS_AS const cd* A, const cd* tau_,
+                      const double* D_, const int* perm_, int n) {
+    LDS_AS const cd* tau = as_lds(tau_);
+    LDS_AS const double* D = as_lds(D_);
+    LDS_AS const int* perm = as_lds(perm_);
+    GLB_AS const cd* Xg = as_global(Xs);
+    GLB_AS cd* Yg = as_global(Y);
+    const int gl = c.tid() % kColGroup, ngrp = c.nthreads() / kColGroup;
+    for (int jb = 0; jb < n; jb += ngrp) {
+        const int j = jb + c.tid() / kColGroup;
+        const int jc = j < n ? j : n - 1;
+        cd x[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = gl + kColGroup * i, rc = r < n ? r : n - 1;
+            const cd v = csel(conj_t, conj(ldc(Xg, jc * n + rc)), ldc(Xg, rc * n + jc));
+            x[i] = csel(r < n, v, cd{0.0, 0.0});
+        }
+        const bool idle = jb + c.wave() * 64 / kColGroup >= n;   // whole wavefront past column n - 1
+#pragma unroll 1
+        for (int k = 0; k < n && !idle; ++k) refl_apply_col<R>(c, x, A, n, k, conj(ldc(tau, k)), gl);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = gl + kColGroup * i;
+            const double dr = D[r < n ? r : n - 1];
+            x[i] = cd{x[i].r / dr, x[i].i / dr};
+        }
+        // back substitution with V = upper triangle of A, column (axpy) form:
+        // row r = 4 i + g is final once rows > r are done; its owner lane divides
+        // by V_rr and the quad then removes V[q][r] y_r from the rows q < r.
+#pragma unroll
+        for (int i = R - 1; i >= 0; --i) {
+#pragma unroll 1
+            for (int g = kColGroup - 1; g >= 0; --g) {
+                const int r = kColGroup * i + g;
+                if (r >= n || idle) continue;
+                const bool mine = gl == g;
+                const cd y0 = cdiv(x[i], ldc(A, r * n + r));
+                const cd y = csel(mine, y0, cd{0.0, 0.0});
+                x[i] = csel(mine, y0, x[i]);
+                const cd yb = cd{c.quad_sum(y.r), c.quad_sum(y.i)};
+#pragma unroll
+                for (int q = 0; q <= i; ++q) {
+                    const int rq = gl + kColGroup * q;
+                    const cd a = ldc(A, (rq < n ? rq : n - 1) * n + r);
+                    const bool up = rq < r;
+                    x[q] = x[q] - cd{up ? a.r : 0.0, up ? a.i : 0.0} * yb;
+                }
+            }
+        }
+        if (j < n) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = gl + kColGroup * i;
+                if (r < n) stc(Yg, perm[r] * n + j, x[i]);
+            }
+        }
+    }
+    c.sync();
+}
+
+// One noinline instance per register-slot count R (4 R >= n), so that each
+// gets its own register allocation.
+template <int R>
+DEVFN void qrp_cols_r(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
+    qrp_cols_t<R>(c, as_lds(A), n, s, D);
+}
+template <int R>
+DEVFN void form_q_cols_r(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n, bool conj_t) {
+    form_q_cols_t<R>(c, Q, as_lds(A), tau, n, conj_t);
+}
+template <int R>
+DEVFN void solve_cols_r(const Ctx& c, cd* Y, const cd* Xs, bool conj_t, const cd* A, const cd* tau,
+                        const double* D, const int* perm, int n) {
+    solve_cols_t<R>(c, Y, Xs, conj_t, as_lds(A), tau, D, perm, n);
+}
+
+#define BAFQ_COL_DISPATCH(n, F, ...)                      \
+    do {                                                  \
+        const int rs_ = ((n) + 3) / 4;                    \
+        if (rs_ <= 6) F<6>(__VA_ARGS__);                  \
+        else if (rs_ <= 12) F<12>(__VA_ARGS__);           \
+        else if (rs_ <= 18) F<18>(__VA_ARGS__);           \
+        else F<24>(__VA_ARGS__);                          \
+    } while (0)
+
+DEV void qrp_cols(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
+    BAFQ_COL_DISPATCH(n, qrp_cols_r, c, A, n, s, D);
+}
+DEV void form_q_cols(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n, bool conj_t) {
+    BAFQ_COL_DISPATCH(n, form_q_cols_r, c, Q, A, tau, n, conj_t);
+}
+DEV void solve_cols(const Ctx& c, cd* Y, const cd* Xs, bool conj_t, const cd* A, const cd* tau, const double* D,
+                    const int* perm, int n) {
+    BAFQ_COL_DISPATCH(n, solve_cols_r, c, Y, Xs, conj_t, A, tau, D, perm, n);
 }
 
 }  // namespace bafq

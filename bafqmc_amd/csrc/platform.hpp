@@ -68,6 +68,36 @@ struct HipCtx {
     DEV int atomic_add(int* p, int v) const { return atomicAdd(p, v); }
     // constant-rate device clock (hipDeviceAttributeWallClockRate kHz)
     DEV uint64_t clock() const { return wall_clock64(); }
+
+    // DPP lane permutations (no LDS round trip): quad_perm [1,0,3,2] = 0xB1,
+    // quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141, row_mirror = 0x140.
+    template <int C, class T>
+    DEV static T dpp(T v) { return __builtin_amdgcn_update_dpp(T(0), v, C, 0xF, 0xF, false); }
+    // Sum over the aligned quad of lanes; all four lanes get the same bits.
+    DEV double quad_sum(double v) const {
+        v += dpp<0xB1>(v);
+        return v + dpp<0x4E>(v);
+    }
+    DEV bool wave_any(bool p) const { return __any(p); }
+    // Wavefront argmax of (v, i), ties to the smaller i; the result is uniform.
+    DEV void wave_argmax(double& v, int& i) const {
+        argmax_step(v, i, dpp<0xB1>(v), dpp<0xB1>(i));
+        argmax_step(v, i, dpp<0x4E>(v), dpp<0x4E>(i));
+        argmax_step(v, i, dpp<0x141>(v), dpp<0x141>(i));
+        argmax_step(v, i, dpp<0x140>(v), dpp<0x140>(i));
+        double bv = __builtin_amdgcn_readlane(v, 0);
+        int bi = __builtin_amdgcn_readlane(i, 0);
+        for (int r = 16; r < 64; r += 16)
+            argmax_step(bv, bi, __builtin_amdgcn_readlane(v, r), __builtin_amdgcn_readlane(i, r));
+        v = bv;
+        i = bi;
+    }
+    DEV static void argmax_step(double& v, int& i, double ov, int oi) {
+        if (ov > v || (ov == v && oi < i)) {
+            v = ov;
+            i = oi;
+        }
+    }
 };
 using Ctx = HipCtx;
 #else
@@ -127,6 +157,25 @@ struct EmuCtx {
     uint64_t clock() const {
         return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
                    std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
+    bool wave_any(bool p) const {
+        double v = p ? 1.0 : 0.0;
+        for (int m = 1; m < 64; m <<= 1) v += shfl_xor(v, m);
+        return v > 0.0;
+    }
+    double quad_sum(double v) const {
+        v += shfl_xor(v, 1);
+        return v + shfl_xor(v, 2);
+    }
+    void wave_argmax(double& v, int& i) const {
+        for (int m = 1; m < 64; m <<= 1) {
+            const double ov = shfl_xor(v, m);
+            const int oi = shfl_xor_i(i, m);
+            if (ov > v || (ov == v && oi < i)) {
+                v = ov;
+                i = oi;
+            }
+        }
     }
 };
 using Ctx = EmuCtx;

@@ -479,21 +479,30 @@ DEVFN void lds_gemm(const Ctx& c, cd* M_, const double* Ez, int n, bool left) {
 // stab_UR (src/number_conserving/src/stabilization.f90:69-93).
 DEVFN void stab_ur_from(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* A) {
     const int n = m.n;
-    qrp(c, A, n, L.qr, L.dtmp);
+    const bool cols = col_path(c, n, A == L.mat);
+    if (cols) qrp_cols(c, A, n, L.qr, L.dtmp);
+    else qrp(c, A, n, L.qr, L.dtmp);
     trmm_perm_rows(c, ch.w1, A, ch.vr, L.qr.perm, n);
     mat_copy(c, ch.vr, ch.w1, n, false);
-    form_q(c, ch.ur, A, L.qr.tau, n);
+    if (cols) form_q_cols(c, ch.ur, A, L.qr.tau, n, false);
+    else form_q(c, ch.ur, A, L.qr.tau, n);
     vec_copy(c, ch.dr, L.dtmp, n);
 }
 
 // stab_UL (stabilization.f90:95-119).
 DEVFN void stab_ul_from(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* A) {
     const int n = m.n;
-    qrp(c, A, n, L.qr, L.dtmp);
+    const bool cols = col_path(c, n, A == L.mat);
+    if (cols) qrp_cols(c, A, n, L.qr, L.dtmp);
+    else qrp(c, A, n, L.qr, L.dtmp);
     perm_cols_times_vh(c, ch.w1, ch.vl, A, L.qr.perm, n);
     mat_copy(c, ch.vl, ch.w1, n, false);
-    form_q(c, ch.w1, A, L.qr.tau, n);
-    mat_copy(c, ch.ul, ch.w1, n, true);
+    if (cols) {
+        form_q_cols(c, ch.ul, A, L.qr.tau, n, true);
+    } else {
+        form_q(c, ch.w1, A, L.qr.tau, n);
+        mat_copy(c, ch.ul, ch.w1, n, true);
+    }
     vec_copy(c, ch.dl, L.dtmp, n);
 }
 
@@ -529,21 +538,27 @@ DEVFN double stab_green(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         }
     }
     c.sync();
-    qrp(c, A, n, L.qr, L.dtmp);
-    // X = Q^H Rt, Rt = UR^H (lower branch) or UL (upper branch)
-    mat_copy(c, ch.w1, upper ? ul : ur, n, !upper);
-    apply_qh_left(c, ch.w1, A, L.qr.tau, n);
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
-        const int r = e / n;
-        ch.w1[e] = ch.w1[e] * cd{1.0 / L.dtmp[r], 0.0};
+    const bool cols = col_path(c, n, A == L.mat);
+    if (cols) qrp_cols(c, A, n, L.qr, L.dtmp);
+    else qrp(c, A, n, L.qr, L.dtmp);
+    // Y = P V^{-1} D^{-1} Q^H Rt, Rt = UR^H (lower branch) or UL (upper branch)
+    if (cols) {
+        solve_cols(c, ch.w2, upper ? ul : ur, !upper, A, L.qr.tau, L.dtmp, L.qr.perm, n);
+    } else {
+        mat_copy(c, ch.w1, upper ? ul : ur, n, !upper);
+        apply_qh_left(c, ch.w1, A, L.qr.tau, n);
+        for (int e = c.tid(); e < nn; e += c.nthreads()) {
+            const int r = e / n;
+            ch.w1[e] = ch.w1[e] * cd{1.0 / L.dtmp[r], 0.0};
+        }
+        c.sync();
+        trsm_upper_left(c, ch.w1, A, n);
+        for (int e = c.tid(); e < nn; e += c.nthreads()) {
+            const int r = e / n, k = e - r * n;
+            ch.w2[L.qr.perm[r] * n + k] = ch.w1[e];
+        }
+        c.sync();
     }
-    c.sync();
-    trsm_upper_left(c, ch.w1, A, n);
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
-        const int r = e / n, k = e - r * n;
-        ch.w2[L.qr.perm[r] * n + k] = ch.w1[e];
-    }
-    c.sync();
     // G' = Lf Y, Lf = UL^H (lower) or UR (upper); G = G' or G'^H
     if (!upper) {
         gemm_cc(c, Gout, ul, true, ch.w2, false, n);
