@@ -2,13 +2,12 @@
 
 The HIP library ``_lib/libbafqmc.so`` is the product.  The emulator build
 ``_lib/libbafqmc_emu.so`` runs the same kernels on CPU threads and is selected
-only explicitly (``load_library(emulator=True)`` or ``BAFQ_EMULATOR=1``), for
-the CPU test suite.  There is no other fallback: a missing library raises.
+only by an explicit ``load_library(emulator=True)`` from the CPU test suite.
+There is no fallback: a missing library raises.
 """
 from __future__ import annotations
 
 import ctypes as C
-import os
 from pathlib import Path
 
 import numpy as np
@@ -105,10 +104,8 @@ class NativeLibrary:
 _CACHE = {}
 
 
-def load_library(emulator: bool | None = None) -> NativeLibrary:
+def load_library(emulator: bool = False) -> NativeLibrary:
     """Load the HIP library (default) or, when asked explicitly, the CPU emulator."""
-    if emulator is None:
-        emulator = os.environ.get("BAFQ_EMULATOR", "0") == "1"
     path = EMU_LIB if emulator else HIP_LIB
     if path not in _CACHE:
         _CACHE[path] = NativeLibrary(path)

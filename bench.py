@@ -51,6 +51,15 @@ def flops_per_chain_sweep(n, ns, ltrot, nwrap, nch, c_meas=40.0):
             + 2 * ltrot * c_meas * ns ** 2)
 
 
+def measured_traffic(config, nchains):
+    """HBM bytes per launch of the sweep kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when it covers this workload."""
+    f = ROOT / "profiles" / "k_sweep_lds_pairing6x6_512chains_hbm.json"
+    if config != "c2" or nchains != 512 or not f.exists():
+        return None
+    return json.loads(f.read_text())["traffic_bytes_per_launch"]
+
+
 def make_params(d):
     from bafqmc_amd.model import Params
     p = Params(pairing=d["pairing"], u1=d["u1"], u2=d["u2"], mu=d["mu"], rdelta=d.get("rdelta", 0.0),
@@ -110,11 +119,22 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
               "(chains shard across GPUs by construction, but no multi-GPU path is built)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4}))
+        return
 
     from bafqmc_amd.model import ModelSpec, conf_set_batch
     from bafqmc_amd.solver import BatchedSolver
@@ -170,7 +190,7 @@ def main():
                                                 zip(("site", "kinetic", "measure", "wrap", "hs", "wrap_products",
                                                      "wrap_stab_u", "wrap_green"), phase)}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.config, nchains),
                      "kernel_ms": avg_ms, "flops_per_launch": f_chain * nchains},
     }
     if not args.no_cpu_baseline:

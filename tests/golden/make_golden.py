@@ -64,6 +64,13 @@ CASES = {
     "pairing_traj_6x6_c2": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=80,
                                        beta=8.0, nwrap=10, nbin=1, shift_loc=1.0, ini_type=2, ini_ampl=1.0),
                                 seeds=[13579113]),
+    # Gaussian warm-up sweeps before the local sweeps (Local_sweep_therm), both solvers
+    "nc_warm_3x3": dict(p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=3, nly=3, ltrot=20, beta=2.0,
+                               nwrap=5, nbin=2, shift_loc=1.5, ini_type=2, ini_ampl=1.0, nwarm=3),
+                        seeds=[86420, 13579]),
+    "pairing_warm_3x3": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=3, nly=3, ltrot=20,
+                                    beta=2.0, nwrap=10, nbin=2, shift_loc=1.0, ini_type=2, ini_ampl=1.0, nwarm=3),
+                             seeds=[424242]),
     # config-4 lattice: NC 9x9, beta=8, Ltrot=80 (one chain, one bin)
     "nc_traj_9x9_c4": dict(p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=80, beta=8.0,
                                   nwrap=10, nbin=1, shift_loc=1.5, ini_type=2, ini_ampl=1.0), seeds=[24681357]),

@@ -13,7 +13,7 @@ import pytest
 from bafqmc_amd import driver
 from bafqmc_amd.model import Params, read_params
 from oracle import run_ref
-from tests.helpers import load_golden, golden_row, assert_obs_close
+from tests.helpers import load_golden, golden_row, golden_info_value, assert_obs_close, obs_atol
 
 
 def _write_case(tmp_path, name):
@@ -45,9 +45,10 @@ def _check_outputs(z, pd, d):
             want = ref[b]
             got = rows[b]
             if want.size == 2 and name not in ("density_site_total",):
-                assert_obs_close(complex(got[0], got[1]), complex(want[0], want[1]), f"{name} row {b}")
+                assert_obs_close(complex(got[0], got[1]), complex(want[0], want[1]), f"{name} row {b}",
+                                 atol=obs_atol(name, pd["pairing"]))
             else:
-                assert_obs_close(got, want, f"{name} row {b}", atol=1e-11)
+                assert_obs_close(got, want, f"{name} row {b}", atol=max(1e-11, obs_atol(name, pd["pairing"])))
     toks = (d / "confout.txt").read_text().split()
     assert int(toks[0]) == int(z["seeds_out"][0])
     nvals = z["fields_out"][0].size
@@ -55,6 +56,14 @@ def _check_outputs(z, pd, d):
                                rtol=0, atol=1e-12)
     info = (d / "info.txt").read_text()
     assert "Tot CPU time" in info and "Max diff Matrix" in info
+    labels = ["Accept_U1_shift"] + (["Thermalize U1 Accept Ratio"] if pd.get("nwarm", 0) else [])
+    if abs(pd["u2"]) > 1e-10:
+        labels.append("Accept_U2_shift")
+        if pd.get("nwarm", 0):
+            labels.append("Thermalize U2 Accept Ratio")
+    for label in labels:
+        mine = golden_info_value({"info": info}, label)
+        assert abs(mine - golden_info_value(z, label)) < 1e-12, label
 
 
 def test_read_params_roundtrip(tmp_path):
@@ -89,7 +98,8 @@ def test_driver_matches_reference_outputs_emulator(tmp_path, emu_lib, name, solv
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,solver", [("nc_traj_3x3_c1", "number_conserving"),
-                                         ("pairing_traj_3x3", "pairing")])
+                                         ("pairing_traj_3x3", "pairing"),
+                                         ("nc_warm_3x3", "number_conserving")])
 def test_driver_matches_reference_outputs_gpu(tmp_path, hip_lib, name, solver):
     z, pd, d = _write_case(tmp_path, name)
     driver.run(d, solver, len(z["seeds"]), lib=hip_lib)

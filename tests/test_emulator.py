@@ -48,6 +48,6 @@ def test_emu_global_path_matches_oracle(emu_lib, case, monkeypatch):
     check_sweep_vs_oracle(emu_lib, case)
 
 
-@pytest.mark.parametrize("name", ["pairing_fixed_3x3_w2"])
+@pytest.mark.parametrize("name", ["pairing_fixed_3x3_w2", "pairing_warm_3x3"])
 def test_emu_matches_reference_fortran(emu_lib, name):
     check_golden(emu_lib, name)

@@ -12,8 +12,8 @@ import pytest
 from oracle import bafq_oracle as O
 from bafqmc_amd.model import ModelSpec, conf_set_batch
 from bafqmc_amd.solver import BatchedSolver
-from tests.helpers import (fixed_fields, load_golden, golden_row, oracle_params, solver_params, reduced,
-                           rel_err, assert_obs_close)
+from tests.helpers import (fixed_fields, load_golden, golden_row, golden_info_value, oracle_params, solver_params,
+                           reduced, rel_err, assert_obs_close, obs_atol)
 
 pytestmark = pytest.mark.gpu
 G_TOL = 1e-10
@@ -120,7 +120,7 @@ def check_sweep_vs_oracle(lib, case):
         s.sweep(1)
         mine = s.read_bin()[0]
         for k, v in ref.items():
-            assert_obs_close(mine[k], v, f"case {case} bin {b} {k}")
+            assert_obs_close(mine[k], v, f"case {case} bin {b} {k}", atol=obs_atol(k, d["pairing"]))
         go = reduced(ch.prop.g, ns) if d["pairing"] else ch.prop.g
         assert rel_err(s.green(0), go) < G_TOL
         f, sd = s.get_fields()
@@ -132,7 +132,8 @@ def check_sweep_vs_oracle(lib, case):
 
 
 GOLDEN = ["nc_fixed_3x3_w1", "nc_fixed_3x3_w6", "pairing_fixed_3x3_w2", "nc_traj_3x3_c1",
-          "nc_traj_3x3_u2", "pairing_traj_3x3", "pairing_traj_6x6_c2", "nc_traj_9x9_c4"]
+          "nc_traj_3x3_u2", "pairing_traj_3x3", "nc_warm_3x3", "pairing_warm_3x3", "pairing_traj_6x6_c2",
+          "nc_traj_9x9_c4"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -154,6 +155,15 @@ def check_golden(lib, name):
         phis = list(phis)
     s = BatchedSolver([ModelSpec(p)], [0] * len(seeds), lib=hip_lib)
     s.set_fields(phis, [int(x) for x in seeds_run])
+    if pd.get("nwarm", 0):
+        # Local_sweep_therm sweeps; acceptance as in the reference's info.txt
+        s.warmup(pd["nwarm"])
+        therm = s.read_bin()
+        for k, chan in ((1, "U1"), (2, "U2")):
+            want = golden_info_value(z, f"Thermalize {chan} Accept Ratio")
+            prop = sum(c["_stats"][f"therm_prop_u{k}"] for c in therm)
+            acc = sum(c["_stats"][f"therm_acc_u{k}"] for c in therm)
+            assert abs(acc / prop - want) < 1e-12, (name, chan, acc / prop, want)
     s.prepare()
     for b in range(pd["nbin"]):
         s.sweep(1)
@@ -165,7 +175,7 @@ def check_golden(lib, name):
             if k not in per[0]:
                 continue
             mine = sum(c[k] for c in per) / len(per)
-            assert_obs_close(mine, golden_row(z, k, b), f"{name} bin {b} {k}")
+            assert_obs_close(mine, golden_row(z, k, b), f"{name} bin {b} {k}", atol=obs_atol(k, pd["pairing"]))
     f, sd = s.get_fields()
     for c in range(len(seeds)):
         np.testing.assert_allclose(f[c], z["fields_out"][c], rtol=0, atol=1e-12)

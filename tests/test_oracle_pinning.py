@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from oracle import bafq_oracle as O
-from tests.helpers import fixed_fields, load_golden, golden_row, oracle_params, assert_obs_close
+from tests.helpers import fixed_fields, load_golden, golden_row, oracle_params, assert_obs_close, obs_atol
 
 ABS_TOL, REL_TOL = 2.0e-9, 2.0e-8
 
@@ -163,7 +163,7 @@ def test_pairing_ratio_bruteforce_and_block_structure():
 
 
 GOLDEN_CASES = ["nc_fixed_3x3_w1", "nc_fixed_3x3_w2", "nc_fixed_3x3_w6", "pairing_fixed_3x3_w2",
-                "nc_traj_3x3_c1", "nc_traj_3x3_u2", "pairing_traj_3x3"]
+                "nc_traj_3x3_c1", "nc_traj_3x3_u2", "pairing_traj_3x3", "nc_warm_3x3", "pairing_warm_3x3"]
 
 
 def oracle_run_golden(name):
@@ -181,6 +181,8 @@ def oracle_run_golden(name):
             phi = O.conf_set(r, pd.get("ini_type", 2), pd.get("ini_ampl", 1.0), (0.0, 0.0), ns, p.ltrot)
             seed = r.seed
         ch = O.Chain(p, phi, seed)
+        for _ in range(pd.get("nwarm", 0)):
+            ch.warmup(1.0)
         ch.pre()
         chains.append(ch)
     bins = []
@@ -198,7 +200,7 @@ def test_oracle_matches_reference_fortran(name):
             if "out_" + k not in z.files:
                 continue
             ref = golden_row(z, k, b)
-            assert_obs_close(v, ref, f"{k} bin {b}")
+            assert_obs_close(v, ref, f"{k} bin {b}", atol=obs_atol(k, pd["pairing"]))
     for c, ch in enumerate(chains):
         assert ch.rng.seed == int(z["seeds_out"][c])
         np.testing.assert_allclose(ch.phi, z["fields_out"][c], rtol=0, atol=1e-13)
