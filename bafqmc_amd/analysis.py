@@ -29,6 +29,26 @@ def seeds_for(seed: int, count: int):
     return out
 
 
+def chain_seeds(seed: int, count: int):
+    """`count` distinct positive seeds: the campaign writer's sequence while it
+    yields new values (its LCG modulo 10^8 with a multiplier divisible by 5
+    repeats after a few hundred terms), continued with the solver's own
+    generator (ranf, seed <- 48828125 seed mod 2^31) started from the last value."""
+    out, seen = [], set()
+    for v in seeds_for(seed, count):
+        if v in seen:
+            break
+        seen.add(v)
+        out.append(v)
+    x = out[-1] if out else (int(seed) % 2147483647) or 1
+    while len(out) < count:
+        x = (48828125 * x) % 2147483648
+        if x > 0 and x not in seen:
+            seen.add(x)
+            out.append(x)
+    return out
+
+
 def block_statistics(samples, block_size: int, skip: int = 0):
     x = np.asarray(samples, dtype=float)[skip:]
     nb = len(x) // block_size

@@ -29,9 +29,11 @@ int rt_sync();
 const char* rt_error();
 int rt_block_threads();        // threads per workgroup for the global-path sweep kernels
 int rt_lds_block_threads();    // threads per workgroup for the LDS-path sweep kernels
+size_t rt_lds_limit();         // LDS bytes available to one workgroup
 int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
 int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
 int rt_debug_mfma(double* out);
+int rt_debug_lanes(const double* in, const int32_t* idx, double* out);
 
 constexpr double kZero = 1.0e-10;   // src/number_conserving/src/calc_basic.f90:5
 
@@ -148,6 +150,21 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
         m.active[0] = std::fabs(d.u1) > kZero;
         m.active[1] = std::fabs(d.u2) > kZero;
         lattice_tables(d.nlx, d.nly, bonds[k], phases[k], m.has_k);
+        // delayed local updates on the LDS path: the widest buffers (<= 16 columns,
+        // a multiple of 4) that fit beside G in LDS.  BAFQ_DELAYED=0 selects
+        // per-site updates, BAFQ_DELAYED=4/8/12 caps the width (tests).
+        {
+            const char* ev = std::getenv("BAFQ_DELAYED");
+            const int cap = ev ? std::atoi(ev) : 16;
+            const int lds_threads = rt_lds_block_threads();
+            m.dk = 0;
+            if (m.n <= kLdsMaxN && 2 * m.n <= lds_threads)
+                for (int dk = (cap / 4) * 4 < 16 ? (cap / 4) * 4 : 16; dk >= 4; dk -= 4)
+                    if (lds_bytes(m.n, lds_threads / 64, true, dk) <= rt_lds_limit()) {
+                        m.dk = dk;
+                        break;
+                    }
+        }
     }
     // ---- layout of the device allocation
     Bump b;
@@ -272,10 +289,11 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
         // the LDS path needs every MFMA output tile of an N x N matrix covered by the waves
         const int lds_waves = rt_lds_block_threads() / 64;
         const int ntile = (m.n + 15) / 16;
-        const bool lds_ok = m.n <= kLdsMaxN && ntile * ntile <= lds_waves * kMaxTiles;
+        const bool lds_ok = m.n <= kLdsMaxN && ntile * ntile <= lds_waves * kMaxTiles &&
+                            lds_bytes(m.n, lds_waves, true, m.dk) <= rt_lds_limit();
         const int path = (lds_ok && !force_global) ? P_LDS : P_GLOBAL;
         pids[path].push_back(c);
-        const size_t lb = path == P_LDS ? lds_bytes(m.n, lds_waves, true) : lds_bytes(m.n, nw_glb, false);
+        const size_t lb = path == P_LDS ? lds_bytes(m.n, lds_waves, true, m.dk) : lds_bytes(m.n, nw_glb, false);
         h->max_lds[path] = std::max(h->max_lds[path], lb);
     }
     rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * n_chains);
@@ -421,5 +439,6 @@ int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
 }
 
 int bafq_debug_mfma_probe(double* out) { return bafq::rt_debug_mfma(out); }
+int bafq_debug_lane_ops(const double* in, const int32_t* idx, double* out) { return bafq::rt_debug_lanes(in, idx, out); }
 
 }  // extern "C"

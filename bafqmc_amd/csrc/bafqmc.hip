@@ -72,6 +72,36 @@ __global__ void __launch_bounds__(64) k_debug_mfma(double* out) {
     for (int r = 0; r < 4; ++r) out[(p * 64 + l) * 4 + r] = acc[r];
 }
 
+// Cross-lane operations of one wavefront: out[0:64] row16_sum, [64:128] quad_sum,
+// [128:192] lane_bcast(., 37), [192:256] / [256:320] wave_argmax value / index.
+__global__ void __launch_bounds__(64) k_debug_lanes(const double* in, const int* idx, double* out) {
+    Ctx c{nullptr};
+    const int l = threadIdx.x;
+    const double v = in[l];
+    out[l] = c.row16_sum(v);
+    out[64 + l] = c.quad_sum(v);
+    out[128 + l] = c.lane_bcast(v, 37);
+    double bv = v;
+    int bi = idx[l];
+    c.wave_argmax(bv, bi);
+    out[192 + l] = bv;
+    out[256 + l] = (double)bi;
+}
+
+int rt_debug_lanes(const double* in, const int32_t* idx, double* out) {
+    double* d = nullptr;
+    if (rt_malloc((void**)&d, sizeof(double) * (64 + 320) + sizeof(int) * 64)) return 1;
+    double* din = d + 320;
+    int* didx = (int*)(d + 384);
+    int rc = rt_h2d(din, in, sizeof(double) * 64) || rt_h2d(didx, idx, sizeof(int) * 64);
+    if (!rc) {
+        hipLaunchKernelGGL(k_debug_lanes, dim3(1), dim3(64), 0, 0, din, didx, d);
+        rc = check(hipGetLastError(), "k_debug_lanes") || rt_sync() || rt_d2h(out, d, sizeof(double) * 320);
+    }
+    rt_free(d);
+    return rc;
+}
+
 int rt_debug_mfma(double* out) {
     double* d = nullptr;
     if (rt_malloc((void**)&d, sizeof(double) * 64 * 64 * 4)) return 1;
@@ -107,6 +137,12 @@ double rt_clock_hz() {
 }
 
 int rt_lds_block_threads() { return kLdsWaves * 64; }
+
+size_t rt_lds_limit() {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess || v <= 0) return 65536;
+    return (size_t)v;
+}
 
 int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
     static hipEvent_t ev[2] = {nullptr, nullptr};

@@ -70,6 +70,8 @@ int rt_lds_block_threads() {
     return b;
 }
 
+size_t rt_lds_limit() { return 160 * 1024; }
+
 int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
     if (ms) *ms = 0.0;
     if (k == K_WARMUP) {
@@ -93,6 +95,22 @@ int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block,
             if (path == P_LDS) body_sweep<true>(c, a);
             else body_sweep<false>(c, a);
         }
+    });
+    return 0;
+}
+
+int rt_debug_lanes(const double* in, const int32_t* idx, double* out) {
+    run_groups(1, 64, 0, [&](EmuCtx& c) {
+        const int l = c.lane();
+        const double v = in[l];
+        out[l] = c.row16_sum(v);
+        out[64 + l] = c.quad_sum(v);
+        out[128 + l] = c.lane_bcast(v, 37);
+        double bv = v;
+        int bi = idx[l];
+        c.wave_argmax(bv, bi);
+        out[192 + l] = bv;
+        out[256 + l] = (double)bi;
     });
     return 0;
 }

@@ -71,13 +71,31 @@ struct HipCtx {
 
     // DPP lane permutations (no LDS round trip): quad_perm [1,0,3,2] = 0xB1,
     // quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141, row_mirror = 0x140.
+    // Every source lane of these permutations exists; callers keep all lanes of
+    // the wavefront active (bound_ctrl would read 0 from an inactive lane).
     template <int C, class T>
-    DEV static T dpp(T v) { return __builtin_amdgcn_update_dpp(T(0), v, C, 0xF, 0xF, false); }
+    DEV static T dpp(T v) { return __builtin_amdgcn_update_dpp(T(0), v, C, 0xF, 0xF, true); }
     // Sum over the aligned quad of lanes; all four lanes get the same bits.
     DEV double quad_sum(double v) const {
         v += dpp<0xB1>(v);
         return v + dpp<0x4E>(v);
     }
+    // Sum over the aligned row of 16 lanes; all sixteen lanes get the same bits.
+    DEV double row16_sum(double v) const {
+        v += dpp<0xB1>(v);
+        v += dpp<0x4E>(v);
+        v += dpp<0x141>(v);
+        return v + dpp<0x140>(v);
+    }
+    // Value of lane l (uniform) in every lane.  v_readlane_b32 moves 32 bits:
+    // a double goes through as its two halves.
+    DEV static double readlane_d(double v, int l) {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+        return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    }
+    DEV double lane_bcast(double v, int l) const { return readlane_d(v, l); }
     DEV bool wave_any(bool p) const { return __any(p); }
     // Wavefront argmax of (v, i), ties to the smaller i; the result is uniform.
     DEV void wave_argmax(double& v, int& i) const {
@@ -85,10 +103,10 @@ struct HipCtx {
         argmax_step(v, i, dpp<0x4E>(v), dpp<0x4E>(i));
         argmax_step(v, i, dpp<0x141>(v), dpp<0x141>(i));
         argmax_step(v, i, dpp<0x140>(v), dpp<0x140>(i));
-        double bv = __builtin_amdgcn_readlane(v, 0);
+        double bv = readlane_d(v, 0);
         int bi = __builtin_amdgcn_readlane(i, 0);
         for (int r = 16; r < 64; r += 16)
-            argmax_step(bv, bi, __builtin_amdgcn_readlane(v, r), __builtin_amdgcn_readlane(i, r));
+            argmax_step(bv, bi, readlane_d(v, r), __builtin_amdgcn_readlane(i, r));
         v = bv;
         i = bi;
     }
@@ -157,6 +175,18 @@ struct EmuCtx {
     uint64_t clock() const {
         return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
                    std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
+    double row16_sum(double v) const {
+        for (int m = 1; m < 16; m <<= 1) v += shfl_xor(v, m);
+        return v;
+    }
+    double lane_bcast(double v, int l) const {
+        double* b = wb(2);
+        b[lane()] = v;
+        wsync();
+        const double r = b[l];
+        wsync();
+        return r;
     }
     bool wave_any(bool p) const {
         double v = p ? 1.0 : 0.0;

@@ -53,6 +53,7 @@ constexpr int kMaxTiles = 5;      // ceil(36 / 8) output tiles per wavefront
 struct ModelDev {
     int kind;            // 0 number conserving, 1 Nambu pairing
     int n, ns, nlx, nly, ltrot, nwrap, nst, has_k;
+    int dk;              // LDS path: columns of the delayed-update buffers (0: per-site updates)
     int active[2];
     double u1, u2, mu, rdelta, shift_loc;
     cd alpha[2];
@@ -130,10 +131,12 @@ struct Lds {
     cd* site;          // 6 x ns (pairing measurement)
     Proposal* prop;    // ns
     cd* mat;           // n x n (LDS path only)
+    cd* du;            // dk x n   delayed-update factors, U transposed: G -= du^T dw (LDS path, dk > 0)
+    cd* dw;            // dk x n
 };
 
 HOSTDEV size_t lds_align(size_t b) { return (b + 15) & ~(size_t)15; }
-HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat) {
+HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat, int dk = 0) {
     size_t b = 0;
     b += lds_align((size_t)n * sizeof(cd));           // tau
     b += lds_align((size_t)n * sizeof(int) + 16);     // perm
@@ -146,10 +149,11 @@ HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat) {
     b += lds_align(6 * (size_t)n * sizeof(cd));
     b += lds_align((size_t)n * sizeof(Proposal));
     if (with_mat) b += lds_align((size_t)n * n * sizeof(cd));
+    b += 2 * lds_align((size_t)n * dk * sizeof(cd));
     return b + 64;
 }
 
-DEV Lds carve(char* base, int n, int nwaves, bool with_mat) {
+DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0) {
     Lds L;
     char* p = base;
     auto take = [&](size_t bytes) {
@@ -170,6 +174,8 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat) {
     L.site = (cd*)take(6 * (size_t)n * sizeof(cd));
     L.prop = (Proposal*)take((size_t)n * sizeof(Proposal));
     L.mat = with_mat ? (cd*)take((size_t)n * n * sizeof(cd)) : nullptr;
+    L.du = dk > 0 ? (cd*)take((size_t)n * dk * sizeof(cd)) : nullptr;
+    L.dw = dk > 0 ? (cd*)take((size_t)n * dk * sizeof(cd)) : nullptr;
     return L;
 }
 
@@ -401,6 +407,193 @@ DEVFN void site_loop(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
 DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t, int nf,
                          bool ascending, int& seed) {
     site_loop_t(c, m, ch, L, as_lds(G), t, nf, ascending, seed);
+}
+
+// ---------------------------------------------------------------- delayed local updates
+// LDS path.  The accepted corrections of consecutive sites are collected as
+// G = G0 - U W (U: n x cols, W: cols x n, cols <= dk) and applied to G0 with one
+// MFMA product when the buffers are full and at the end of the slice.  Site i
+// needs only G[S,S] = G0[S,S] - U[S,:] W[:,S] for its decision, which every
+// wavefront evaluates itself (lanes over the k x k entries and the inner index,
+// a DPP row sum), so a rejected proposal costs no barrier; an accepted one forms
+// its own factors from the current columns G[:,S] and rows G[S,:] and costs one.
+// Both factors are stored with the long index contiguous (Ut[j][r] = U[r][j],
+// W[j][c]) so that consecutive lanes touch consecutive LDS addresses.
+
+// G0 -= U[:, :cols] W[:cols, :] on FP64 MFMA 16x16x4 (each output tile owned by one wavefront).
+DEVFN void delayed_flush(const Ctx& c, cd* G_, const cd* Ut_, const cd* W_, int n, int cols) {
+    auto G = as_lds(G_);
+    auto Ut = as_lds(Ut_);
+    auto W = as_lds(W_);
+    const int nt = (n + 15) >> 4;
+    const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
+    for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
+        const int ti = t / nt, tj = t - ti * nt;
+        const int ar = 16 * ti + li, bc = 16 * tj + li;
+        cd a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kk = 4 * q + lk;
+            a[q] = cd{0.0, 0.0};
+            b[q] = cd{0.0, 0.0};
+            if (ar < n && kk < cols) a[q] = ldc(Ut, kk * n + ar);
+            if (bc < n && kk < cols) b[q] = ldc(W, kk * n + bc);
+        }
+        d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (4 * q < cols) {
+                accr = c.mfma(a[q].r, b[q].r, accr);
+                accr = c.mfma(-a[q].i, b[q].i, accr);
+                acci = c.mfma(a[q].r, b[q].i, acci);
+                acci = c.mfma(a[q].i, b[q].r, acci);
+            }
+        }
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
+            if (row < n && col < n) {
+                const cd g = ldc(G, row * n + col);
+                stc(G, row * n + col, cd{g.r - accr[r], g.i - acci[r]});
+            }
+        }
+    }
+    c.sync();
+}
+
+// Same decisions, RNG use and result as site_loop_t (LocalU_metro + LocalU_prop_L/R).
+// Requires dk <= 16, n <= 128 and 2 n <= threads.
+DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t,
+                             int nf, bool ascending, int& seed) {
+    const int n = m.n, ns = m.ns, dk = m.dk;
+    const bool pairing = m.kind == 1;
+    const int ksec = pairing ? 2 : 1;
+    const int tid = c.tid(), lane = c.lane();
+    const int wbase = c.nthreads() >= 256 ? 128 : c.nthreads() / 2;
+    auto G = as_lds(G_);
+    auto Ut = as_lds(L.du);
+    auto W = as_lds(L.dw);
+    double* phi_t = ch.phi + (size_t)t * ns * 2;
+    make_proposals(c, m, L, phi_t, nf, ascending, seed);
+    seed = lcg_jump(seed, 2 * ns);
+    auto props = as_lds(L.prop);
+    // lane roles in the correction U[S,:] W[:,S]: entry (ea, eb), inner index ej
+    const int ea = lane >> 5, eb = (lane >> 4) & 1, ej = lane & 15;
+    int nacc = 0, cols = 0;
+    for (int q = 0; q < ns; ++q) {
+        const int i = ascending ? q : ns - 1 - q;
+        const int s0 = i, s1 = ns + i;
+        const int sa = (pairing && ea) ? s1 : s0, sb = (pairing && eb) ? s1 : s0;
+        // everything the decision reads is loaded up front
+        double cr = 0.0, ci = 0.0;
+        if (ej < cols && (pairing || lane < 16)) {
+            const cd u = ldc(Ut, ej * n + sa);
+            const cd w = ldc(W, ej * n + sb);
+            cr = u.r * w.r - u.i * w.i;
+            ci = u.r * w.i + u.i * w.r;
+        }
+        const cd gss = ldc(G, sa * n + sb);
+        Proposal P;
+        {
+            auto pp = props + i;
+            P.phi_new = pp->phi_new;
+            P.ratio0 = pp->ratio0;
+            P.u = pp->u;
+            P.d0 = cd{pp->d0.r, pp->d0.i};
+            P.d1 = cd{pp->d1.r, pp->d1.i};
+        }
+        // lanes 16 e .. 16 e + 15 hold entry e = (ea, eb) of the current G[S,S]
+        const cd ge = cd{gss.r - c.row16_sum(cr), gss.i - c.row16_sum(ci)};
+        const cd g00 = cd{c.lane_bcast(ge.r, 0), c.lane_bcast(ge.i, 0)};
+        bool acc;
+        cd p00, p01 = cd{0.0, 0.0}, p10 = cd{0.0, 0.0}, p11 = cd{1.0, 0.0}, det;
+        if (!pairing) {
+            det = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
+            p00 = det;
+            const double a2 = abs2(det);
+            // |ratio0 / |den|^2| > u
+            acc = fabs(P.ratio0 / a2) > P.u;
+        } else {
+            const cd g01 = cd{c.lane_bcast(ge.r, 16), c.lane_bcast(ge.i, 16)};
+            const cd g10 = cd{c.lane_bcast(ge.r, 32), c.lane_bcast(ge.i, 32)};
+            const cd g11 = cd{c.lane_bcast(ge.r, 48), c.lane_bcast(ge.i, 48)};
+            p00 = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
+            p01 = -(P.d0 * g01);
+            p10 = -(P.d1 * g10);
+            p11 = cd{1.0, 0.0} + P.d1 * (cd{1.0, 0.0} - g11);
+            det = p00 * p11 - p01 * p10;
+            acc = fabs(P.ratio0 / sqrt(abs2(det))) > P.u;
+        }
+        if (!acc) continue;
+        ++nacc;
+        cd m00, m01 = cd{0.0, 0.0}, m10 = cd{0.0, 0.0}, m11 = cd{0.0, 0.0};
+        const cd idet = cinv(det);
+        if (!pairing) {
+            m00 = idet * P.d0;
+        } else {
+            m00 = (p11 * idet) * P.d0;
+            m01 = (-(p01 * idet)) * P.d1;
+            m10 = (-(p10 * idet)) * P.d0;
+            m11 = (p00 * idet) * P.d1;
+        }
+        if (tid == 0) (props + i)->pad = 1.0;                // accepted: phi is written after the loop
+        // New factors: threads [0, n) form row r of the new U columns, G[r,S] M with
+        // G[r,S] = G0[r,S] - U[r,:] W[:,S]; threads [wbase, wbase + n) column cb of
+        // the new W rows, I_S - G[S,cb] (wbase = 128: the two halves run on
+        // different SIMDs).  The inner index runs in chunks of four whose loads are
+        // issued together (rows beyond cols are read from inside the buffer and
+        // replaced by zero).
+        if (tid < n || (tid >= wbase && tid < wbase + n)) {
+            const bool urow = tid < n;
+            const int x = urow ? tid : tid - wbase;                // r or cb
+            const int xa = urow ? x * n + s0 : s0 * n + x;         // G0[r,s0] or G0[s0,cb]
+            const int xb = urow ? x * n + s1 : s1 * n + x;
+            cd a0 = ldc(G, xa), a1 = cd{0.0, 0.0};
+            if (pairing) a1 = ldc(G, xb);
+            // urow: v = U[r,j], p = W[j,s];  else: v = W[j,cb], p = U[s,j]
+            const auto vb = urow ? Ut : W;
+            const auto pb = urow ? W : Ut;
+            for (int j0 = 0; j0 < cols; j0 += 4) {
+                cd v[4], p0[4], p1[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + u < dk ? j0 + u : dk - 1;
+                    v[u] = ldc(vb, j * n + x);
+                    p0[u] = ldc(pb, j * n + s0);
+                    p1[u] = cd{0.0, 0.0};
+                    if (pairing) p1[u] = ldc(pb, j * n + s1);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const cd vv = csel(j0 + u < cols, v[u], cd{0.0, 0.0});
+                    a0 = a0 - vv * p0[u];
+                    if (pairing) a1 = a1 - vv * p1[u];
+                }
+            }
+            if (urow) {
+                stc(Ut, cols * n + x, a0 * m00 + a1 * m10);
+                if (pairing) stc(Ut, (cols + 1) * n + x, a0 * m01 + a1 * m11);
+            } else {
+                stc(W, cols * n + x, cd{(x == s0 ? 1.0 : 0.0) - a0.r, -a0.i});
+                if (pairing) stc(W, (cols + 1) * n + x, cd{(x == s1 ? 1.0 : 0.0) - a1.r, -a1.i});
+            }
+        }
+        cols += ksec;
+        c.sync();
+        if (cols + ksec > dk) {
+            delayed_flush(c, G_, L.du, L.dw, n, cols);
+            cols = 0;
+        }
+    }
+    if (cols > 0) delayed_flush(c, G_, L.du, L.dw, n, cols);
+    for (int i = tid; i < ns; i += c.nthreads()) {
+        auto pp = props + i;
+        if (pp->pad != 0.0) phi_t[i * 2 + nf - 1] = pp->phi_new;
+    }
+    if (tid == 0) {
+        ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
+        ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
+    }
+    c.sync();
 }
 
 // ---------------------------------------------------------------- MFMA GEMM in LDS
@@ -1087,7 +1280,8 @@ DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
-            site_loop_lds(c, m, ch, L, G, t, nf, false, seed);
+            if (m.dk > 0) site_loop_delayed(c, m, ch, L, G, t, nf, false, seed);
+            else site_loop_lds(c, m, ch, L, G, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
             scale_mat_lds(c, G, n, L.finv, L.f);         // G <- e^{-V} G e^{V}
@@ -1113,7 +1307,8 @@ DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
             hs_factors(c, m, L, phi_t, nf);
             scale_mat_lds(c, G, n, L.f, L.finv);         // G <- e^{V} G e^{-V} (old fields)
             tm.stop(c, ch, S_T_HS);
-            site_loop_lds(c, m, ch, L, G, t, nf, true, seed);
+            if (m.dk > 0) site_loop_delayed(c, m, ch, L, G, t, nf, true, seed);
+            else site_loop_lds(c, m, ch, L, G, t, nf, true, seed);
             tm.stop(c, ch, S_T_SITE);
         }
         // propT_R: G <- (e^{-dtau K} G) e^{+dtau K}; UR's product is applied at the wrap
@@ -1137,7 +1332,7 @@ DEV void body_prepare(const Ctx& c, const LaunchArgs& a) {
     for (int q = c.block(); q < a.count; q += c.nblocks()) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, kLds ? m.dk : 0);
         if (kLds) prepare_lds(c, m, ch, L);
         else prepare_global(c, m, ch, L);
     }
@@ -1148,7 +1343,7 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a) {
     for (int q = c.block(); q < a.count; q += c.nblocks()) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, kLds ? m.dk : 0);
         int seed = *ch.seed;
         if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
         for (int s = 0; s < a.nsweep; ++s) {

@@ -44,6 +44,7 @@ SYMBOLS = {
     "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),
     "bafq_debug_qrp": (C.c_int, [C.c_int, c_double_p, c_double_p, c_int32_p, c_double_p]),
     "bafq_debug_mfma_probe": (C.c_int, [c_double_p]),
+    "bafq_debug_lane_ops": (C.c_int, [c_double_p, c_int32_p, c_double_p]),
 }
 
 
@@ -87,6 +88,15 @@ class NativeLibrary:
         out = np.zeros((64, 64, 4))
         if self.lib.bafq_debug_mfma_probe(dptr(out)):
             raise RuntimeError("bafq_debug_mfma_probe failed")
+        return out
+
+    def debug_lane_ops(self, v, idx):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        assert v.shape == (64,) and idx.shape == (64,)
+        out = np.zeros(320)
+        if self.lib.bafq_debug_lane_ops(dptr(v), iptr(idx), dptr(out)):
+            raise RuntimeError("bafq_debug_lane_ops failed")
         return out
 
     def debug_qrp(self, a):

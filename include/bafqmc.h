@@ -100,6 +100,11 @@ int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
  * accumulator r of `lane` for A = one-hot at lane p, B = 1000 + lane
  * (64 x 64 x 4 doubles). */
 int bafq_debug_mfma_probe(double* out);
+/* Cross-lane operations of one 64-lane wavefront on in[64] / idx[64]:
+ * out[0:64] sums over aligned 16-lane rows, out[64:128] sums over aligned
+ * quads, out[128:192] lane 37's value in every lane, out[192:256] and
+ * out[256:320] the wavefront maximum of in and the lowest idx attaining it. */
+int bafq_debug_lane_ops(const double* in, const int32_t* idx, double* out);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from tests.helpers import rel_err
-from tests.test_gpu_parity import check_sweep_vs_oracle, check_golden, check_mfma_layout
+from tests.test_gpu_parity import check_sweep_vs_oracle, check_golden, check_mfma_layout, check_lane_ops
 
 
 def test_emu_mfma_layout(emu_lib):
@@ -51,3 +51,13 @@ def test_emu_global_path_matches_oracle(emu_lib, case, monkeypatch):
 @pytest.mark.parametrize("name", ["pairing_fixed_3x3_w2", "pairing_warm_3x3"])
 def test_emu_matches_reference_fortran(emu_lib, name):
     check_golden(emu_lib, name)
+
+
+@pytest.mark.parametrize("width", ["0", "4"])
+def test_emu_local_update_variants(emu_lib, width, monkeypatch):
+    monkeypatch.setenv("BAFQ_DELAYED", width)
+    check_sweep_vs_oracle(emu_lib, 3)
+
+
+def test_emu_lane_ops(emu_lib):
+    check_lane_ops(emu_lib)

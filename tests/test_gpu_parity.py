@@ -19,6 +19,27 @@ pytestmark = pytest.mark.gpu
 G_TOL = 1e-10
 
 
+def check_lane_ops(lib):
+    """DPP row/quad sums, 64-bit lane broadcast and wavefront argmax (values are
+    not integers, so a broadcast that truncated to 32-bit integers would fail)."""
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=64) * 10.0 ** rng.integers(-3, 4, size=64)
+    top = float(np.max(np.abs(v))) + 123.456789
+    v[[7, 50]] = top               # a tie for the maximum: the lowest index wins
+    idx = np.arange(64, dtype=np.int32)
+    out = lib.debug_lane_ops(v, idx)
+    rows = v.reshape(4, 16).sum(axis=1)
+    np.testing.assert_allclose(out[0:64], np.repeat(rows, 16), rtol=1e-14, atol=1e-14)
+    quads = v.reshape(16, 4).sum(axis=1)
+    np.testing.assert_allclose(out[64:128], np.repeat(quads, 4), rtol=1e-14, atol=1e-14)
+    assert np.all(out[128:192] == v[37])
+    assert np.all(out[192:256] == top) and np.all(out[256:320] == 7)
+
+
+def test_lane_ops(hip_lib):
+    check_lane_ops(hip_lib)
+
+
 def check_mfma_layout(lib):
     """FP64 MFMA 16x16x4: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, result
     D[i][j] in lane j + 16 (i % 4), accumulator register i // 4."""
@@ -194,6 +215,14 @@ def check_golden(lib, name):
 def test_global_memory_path_matches_reference(hip_lib, name, monkeypatch):
     """The global-memory path (used for N > 96) on the golden cases."""
     monkeypatch.setenv("BAFQ_FORCE_GLOBAL", "1")
+    check_golden(hip_lib, name)
+
+
+@pytest.mark.parametrize("width", ["0", "4"])
+@pytest.mark.parametrize("name", ["nc_traj_3x3_u2", "pairing_traj_3x3", "pairing_traj_6x6_c2"])
+def test_local_update_variants_match_reference(hip_lib, name, width, monkeypatch):
+    """Per-site rank updates (0) and delayed updates flushed every 4 columns (4)."""
+    monkeypatch.setenv("BAFQ_DELAYED", width)
     check_golden(hip_lib, name)
 
 
