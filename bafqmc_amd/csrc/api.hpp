@@ -31,7 +31,7 @@ int rt_block_threads();        // threads per workgroup for the global-path swee
 int rt_lds_block_threads();    // threads per workgroup for the LDS-path sweep kernels
 size_t rt_lds_limit();         // LDS bytes available to one workgroup
 int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
-int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
+int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int variant);
 int rt_debug_mfma(double* out);
 int rt_debug_lanes(const double* in, const int32_t* idx, double* out);
 
@@ -434,8 +434,8 @@ int bafq_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b,
     return bafq::rt_debug_gemm(n, a, op_a, b, op_b, c);
 }
 
-int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
-    return bafq::rt_debug_qrp(n, a, tau, perm, d);
+int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int variant) {
+    return bafq::rt_debug_qrp(n, a, tau, perm, d, variant);
 }
 
 int bafq_debug_mfma_probe(double* out) { return bafq::rt_debug_mfma(out); }

@@ -50,11 +50,20 @@ __global__ void __launch_bounds__(512) k_debug_gemm(cd* C, const cd* A, int oa, 
     gemm_cc(c, C, A, oa != 0, B, ob != 0, n);
 }
 
-__global__ void __launch_bounds__(512) k_debug_qrp(cd* A, int n, cd* tau, int* perm, double* d) {
+// variant 0: workgroup-cooperative QR of A in global memory; 1: column-parallel
+// QR of A copied into LDS (the LDS-path routine; needs 4 n <= threads).
+__global__ void __launch_bounds__(512) k_debug_qrp(cd* A, int n, cd* tau, int* perm, double* d, int variant) {
     extern __shared__ __align__(16) char smem[];
     Ctx c{smem};
-    const Lds L = carve(smem, n, c.nwaves(), false);
-    qrp(c, A, n, L.qr, L.dtmp);
+    const Lds L = carve(smem, n, c.nwaves(), variant == 1);
+    if (variant == 1) {
+        for (int e = c.tid(); e < n * n; e += c.nthreads()) L.mat[e] = A[e];
+        c.sync();
+        qrp_cols(c, L.mat, n, L.qr, L.dtmp);
+        for (int e = c.tid(); e < n * n; e += c.nthreads()) A[e] = L.mat[e];
+    } else {
+        qrp(c, A, n, L.qr, L.dtmp);
+    }
     for (int k = c.tid(); k < n; k += c.nthreads()) {
         tau[k] = L.qr.tau[k];
         perm[k] = L.qr.perm[k];
@@ -189,8 +198,9 @@ int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, d
     return rc;
 }
 
-int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
+int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int variant) {
     if (n < 1 || n > 512) { g_err = "n out of range"; return 1; }
+    if (variant == 1 && (n > kColMaxN || 4 * n > rt_lds_block_threads())) { g_err = "n too large for the column QR"; return 1; }
     const size_t bytes = sizeof(cd) * (size_t)n * n;
     cd *da = nullptr, *dt = nullptr;
     int* dp = nullptr;
@@ -199,10 +209,18 @@ int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
         rt_malloc((void**)&dp, sizeof(int) * n) || rt_malloc((void**)&dd, sizeof(double) * n)) return 1;
     int rc = rt_h2d(da, a, bytes);
     if (!rc) {
-        const int block = rt_block_threads();
-        hipLaunchKernelGGL(k_debug_qrp, dim3(1), dim3(block), lds_bytes(n, block / 64, false), 0, da, n, dt, dp, dd);
-        rc = check(hipGetLastError(), "k_debug_qrp") || rt_sync() || rt_d2h(a, da, bytes) ||
-             rt_d2h(tau, dt, sizeof(cd) * n) || rt_d2h(perm, dp, sizeof(int) * n) || rt_d2h(d, dd, sizeof(double) * n);
+        const int block = variant == 1 ? rt_lds_block_threads() : rt_block_threads();
+        const size_t lb = lds_bytes(n, block / 64, variant == 1);
+        if (lb > rt_lds_limit()) g_err = "matrix does not fit in LDS";
+        rc = lb > rt_lds_limit() || (lb > 65536 && check(hipFuncSetAttribute((const void*)k_debug_qrp,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb),
+                                  "hipFuncSetAttribute"));
+        if (!rc) {
+            hipLaunchKernelGGL(k_debug_qrp, dim3(1), dim3(block), lb, 0, da, n, dt, dp, dd, variant);
+            rc = check(hipGetLastError(), "k_debug_qrp") || rt_sync() || rt_d2h(a, da, bytes) ||
+                 rt_d2h(tau, dt, sizeof(cd) * n) || rt_d2h(perm, dp, sizeof(int) * n) ||
+                 rt_d2h(d, dd, sizeof(double) * n);
+        }
     }
     rt_free(da); rt_free(dt); rt_free(dp); rt_free(dd);
     return rc;

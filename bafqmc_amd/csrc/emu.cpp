@@ -1,43 +1,164 @@
-// CPU emulator back end (-DBAFQ_EMU): runs the same workgroup kernels with one
-// OS thread per GPU thread, for testing and bounds-checking without a GPU.
-// Device memory is host memory; workgroups of a launch run one after another.
+// CPU emulator back end (-DBAFQ_EMU): runs the same workgroup kernels on the
+// host, for testing and bounds-checking without a GPU.  Device memory is host
+// memory.  The threads of a workgroup are fibers on one OS thread with their own
+// stacks; a barrier hands control to the next fiber in thread order, so a
+// workgroup barrier completes when the last thread arrives and a wavefront
+// barrier when the wavefront's last lane does (every kernel here keeps barriers
+// workgroup-uniform and shuffles wavefront-uniform).  Independent workgroups of
+// a launch run in parallel on a few OS threads.  x86-64 only.
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
 #include <cstdlib>
+#include <functional>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "api.hpp"
 
+// Save the callee-saved registers, MXCSR and the x87 control word on the
+// current stack, store the stack pointer in *from and resume the context whose
+// stack pointer is `to`.
+extern "C" void bafq_emu_switch(void** from, void* to);
+asm(R"(
+    .text
+    .globl bafq_emu_switch
+    .type bafq_emu_switch,@function
+bafq_emu_switch:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    subq $8, %rsp
+    stmxcsr (%rsp)
+    fnstcw 4(%rsp)
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    ldmxcsr (%rsp)
+    fldcw 4(%rsp)
+    addq $8, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+    .size bafq_emu_switch, .-bafq_emu_switch
+)");
+
 namespace bafq {
 
 static std::string g_err;
 
-template <class F>
-static void run_groups(int grid, int block, size_t smem, F&& body) {
-    const int nw = block / 64;
-    for (int g = 0; g < grid; ++g) {
-        std::vector<char> lds(smem + 64, 0);
-        std::barrier<> bar(block);
-        std::vector<std::barrier<>*> wb;
-        for (int w = 0; w < nw; ++w) wb.push_back(new std::barrier<>(64));
-        EmuGroup grp;
-        grp.nthreads = block;
-        grp.bar = &bar;
-        grp.wbar = wb;
-        grp.wbuf.assign((size_t)nw * 3 * 64, 0.0);
-        grp.smem = lds.data();
-        grp.block = g;
-        grp.nblocks = grid;
-        std::vector<std::thread> th;
-        th.reserve(block);
-        for (int t = 0; t < block; ++t)
-            th.emplace_back([&, t]() {
-                EmuCtx c{&grp, t, grp.smem};
-                body(c);
-            });
-        for (auto& x : th) x.join();
-        for (auto* p : wb) delete p;
+namespace {
+
+constexpr size_t kFiberStack = 256 * 1024;
+
+struct Sched {
+    std::vector<void*> sp;                 // saved stack pointer per fiber
+    std::vector<std::unique_ptr<char[]>>* stacks;
+    void* main_sp = nullptr;
+    int cur = 0;
+    int nthreads = 0;
+    const std::function<void(EmuCtx&)>* body = nullptr;
+    EmuGroup* grp = nullptr;
+};
+
+thread_local Sched* tl_sched = nullptr;
+
+void switch_to(Sched* s, int from, int to) {
+    s->cur = to;
+    bafq_emu_switch(&s->sp[from], s->sp[to]);
+}
+
+void group_barrier(EmuGroup* g, int t) {
+    Sched* s = (Sched*)g->sched;
+    switch_to(s, t, t + 1 < s->nthreads ? t + 1 : 0);
+}
+
+void wave_barrier(EmuGroup* g, int t) {
+    Sched* s = (Sched*)g->sched;
+    switch_to(s, t, (t & 63) != 63 ? t + 1 : t - 63);
+}
+
+[[noreturn]] void fiber_main() {
+    Sched* s = tl_sched;
+    const int t = s->cur;
+    EmuCtx c{s->grp, t, s->grp->smem};
+    (*s->body)(c);
+    // finished: the next thread resumes; the last one returns to the launcher
+    if (t + 1 < s->nthreads) {
+        s->cur = t + 1;
+        bafq_emu_switch(&s->sp[t], s->sp[t + 1]);
+    } else {
+        bafq_emu_switch(&s->sp[t], s->main_sp);
     }
+    std::abort();
+}
+
+// A fresh context that starts in fiber_main on the given stack.
+void* make_context(char* stack, size_t size) {
+    uintptr_t top = ((uintptr_t)(stack + size)) & ~(uintptr_t)15;
+    uint64_t* p = (uint64_t*)top;
+    *--p = 0;                                  // fake return address of fiber_main
+    *--p = (uint64_t)(uintptr_t)&fiber_main;   // consumed by `ret`
+    for (int i = 0; i < 6; ++i) *--p = 0;      // rbp rbx r12 r13 r14 r15
+    --p;
+    uint32_t* ctl = (uint32_t*)p;
+    ctl[0] = 0x1F80;                           // default MXCSR
+    ctl[1] = 0x037F;                           // default x87 control word
+    return (void*)p;
+}
+
+void run_one_group(int g, int grid, int block, size_t smem, const std::function<void(EmuCtx&)>& body,
+                   std::vector<std::unique_ptr<char[]>>& stacks) {
+    const int nw = (block + 63) / 64;
+    std::vector<char> lds(smem + 64, 0);
+    EmuGroup grp;
+    grp.nthreads = block;
+    grp.wbuf.assign((size_t)nw * 3 * 64, 0.0);
+    grp.smem = lds.data();
+    grp.block = g;
+    grp.nblocks = grid;
+    grp.group_barrier = group_barrier;
+    grp.wave_barrier = wave_barrier;
+    Sched s;
+    s.nthreads = block;
+    s.body = &body;
+    s.grp = &grp;
+    s.sp.resize(block);
+    while ((int)stacks.size() < block) stacks.emplace_back(new char[kFiberStack]);
+    for (int t = 0; t < block; ++t) s.sp[t] = make_context(stacks[t].get(), kFiberStack);
+    grp.sched = &s;
+    tl_sched = &s;
+    s.cur = 0;
+    bafq_emu_switch(&s.main_sp, s.sp[0]);
+    tl_sched = nullptr;
+}
+
+}  // namespace
+
+template <class F>
+static void run_groups(int grid, int block, size_t smem, F&& fn) {
+    const std::function<void(EmuCtx&)> body = fn;
+    const int nt = std::max(1, std::min<int>(grid, (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        std::vector<std::unique_ptr<char[]>> stacks;
+        for (int g = next++; g < grid; g = next++) run_one_group(g, grid, block, smem, body, stacks);
+    };
+    if (nt == 1) {
+        worker();
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i) th.emplace_back(worker);
+    for (auto& x : th) x.join();
 }
 
 int rt_malloc(void** p, size_t bytes) {
@@ -52,21 +173,22 @@ int rt_memset(void* dst, int v, size_t bytes) { std::memset(dst, v, bytes); retu
 int rt_sync() { return 0; }
 const char* rt_error() { return g_err.c_str(); }
 
+// Workgroup sizes default to the GPU's (256 threads on the global-memory
+// path, 512 on the LDS path); BAFQ_BLOCK / BAFQ_LDS_BLOCK select others in
+// tests (the host checks that the waves cover every MFMA tile).
 int rt_block_threads() {
     const char* s = std::getenv("BAFQ_BLOCK");
-    int b = s ? std::atoi(s) : 128;
-    if (b < 64 || b > 1024 || (b % 64) != 0) b = 128;
+    int b = s ? std::atoi(s) : 256;
+    if (b < 64 || b > 1024 || (b % 64) != 0) b = 256;
     return b;
 }
 
 double rt_clock_hz() { return 1.0e9; }
 
-// The emulator may run the LDS path with fewer waves (small N only; the host
-// checks that the waves cover every MFMA tile).
 int rt_lds_block_threads() {
     const char* s = std::getenv("BAFQ_LDS_BLOCK");
-    int b = s ? std::atoi(s) : 128;
-    if (b < 64 || b > 512 || (b % 64) != 0) b = 128;
+    int b = s ? std::atoi(s) : 512;
+    if (b < 64 || b > 512 || (b % 64) != 0) b = 512;
     return b;
 }
 
@@ -82,6 +204,10 @@ int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block,
                 grp.nthreads = block;
                 grp.block = g;
                 grp.nblocks = grid;
+                grp.smem = nullptr;
+                grp.group_barrier = nullptr;
+                grp.wave_barrier = nullptr;
+                grp.sched = nullptr;
                 EmuCtx c{&grp, t, nullptr};
                 body_warmup(c, a);
             }
@@ -132,11 +258,19 @@ int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, d
     return 0;
 }
 
-int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d) {
-    const int block = rt_block_threads();
-    run_groups(1, block, lds_bytes(n, block / 64, false), [&](EmuCtx& ctx) {
-        const Lds L = carve(ctx.smem, n, ctx.nwaves(), false);
-        qrp(ctx, (cd*)a, n, L.qr, L.dtmp);
+int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int variant) {
+    if (variant == 1 && (n > kColMaxN || 4 * n > rt_lds_block_threads())) { g_err = "n too large for the column QR"; return 1; }
+    const int block = variant == 1 ? rt_lds_block_threads() : rt_block_threads();
+    run_groups(1, block, lds_bytes(n, block / 64, variant == 1), [&](EmuCtx& ctx) {
+        const Lds L = carve(ctx.smem, n, ctx.nwaves(), variant == 1);
+        if (variant == 1) {
+            for (int e = ctx.tid(); e < n * n; e += ctx.nthreads()) L.mat[e] = ((cd*)a)[e];
+            ctx.sync();
+            qrp_cols(ctx, L.mat, n, L.qr, L.dtmp);
+            for (int e = ctx.tid(); e < n * n; e += ctx.nthreads()) ((cd*)a)[e] = L.mat[e];
+        } else {
+            qrp(ctx, (cd*)a, n, L.qr, L.dtmp);
+        }
         for (int k = ctx.tid(); k < n; k += ctx.nthreads()) {
             ((cd*)tau)[k] = L.qr.tau[k];
             perm[k] = L.qr.perm[k];

@@ -582,7 +582,6 @@ DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, doub
                 stc(tau, k, t);
                 perm[k] = col;
                 D[k] = fabs(beta);
-                nrm[col] = -1.0;
             }
             active = false;
         }
@@ -590,6 +589,9 @@ DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, doub
         c.sync();
         // ---- H(k)^H = I - conj(tau) v v^H on the remaining columns, new norms
         if (k + 1 < n) {
+            // the chosen column leaves the pivot search; every wavefront has read
+            // the norms of this step before the barrier above
+            if (piv && gl == 0) nrm[col] = -1.0;
             if (c.wave_any(active)) {
                 const cd tk = ldc(tau, k);
                 refl_apply_col<R>(c, x, A, n, k, csel(active, conj(tk), cd{0.0, 0.0}), gl);

@@ -6,8 +6,9 @@
 // operation.  Two back ends implement it:
 //
 //   * HipCtx  (default): CDNA4 / gfx950, compiled by hipcc.
-//   * EmuCtx  (-DBAFQ_EMU): a CPU emulator in which every GPU thread is an OS
-//     thread, barriers are real barriers and a wavefront's MFMA is evaluated
+//   * EmuCtx  (-DBAFQ_EMU): a CPU emulator in which every GPU thread of a
+//     workgroup is a fiber (user-level context) on one OS thread, barriers
+//     switch between fibers, and a wavefront's shuffles and MFMA are evaluated
 //     from the 64 lanes' operands.  It exists so the same kernels can be run
 //     and bounds-checked on a machine without a GPU.
 //
@@ -28,7 +29,6 @@
 typedef double d4 __attribute__((ext_vector_type(4)));
 #else
 #include <atomic>
-#include <barrier>
 #include <chrono>
 #include <cstring>
 #include <vector>
@@ -122,11 +122,13 @@ using Ctx = HipCtx;
 // ---------------------------------------------------------------- emulator
 struct EmuGroup {
     int nthreads;
-    std::barrier<>* bar;
-    std::vector<std::barrier<>*> wbar;   // one barrier per wavefront
     std::vector<double> wbuf;            // [wave][3][64] exchange buffers
     char* smem;
     int block, nblocks;
+    // barrier of the workgroup / of thread t's wavefront (emu.cpp)
+    void (*group_barrier)(EmuGroup*, int t);
+    void (*wave_barrier)(EmuGroup*, int t);
+    void* sched;
 };
 
 struct EmuCtx {
@@ -140,8 +142,8 @@ struct EmuCtx {
     int nwaves() const { return g->nthreads >> 6; }
     int block() const { return g->block; }
     int nblocks() const { return g->nblocks; }
-    void sync() const { g->bar->arrive_and_wait(); }
-    void wsync() const { g->wbar[wave()]->arrive_and_wait(); }
+    void sync() const { g->group_barrier(g, t); }
+    void wsync() const { g->wave_barrier(g, t); }
     double* wb(int slot) const { return &g->wbuf[(size_t)(wave() * 3 + slot) * 64]; }
     double shfl_xor(double v, int mask) const {
         double* b = wb(0);

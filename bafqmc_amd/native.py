@@ -42,7 +42,7 @@ SYMBOLS = {
     "bafq_get_green": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),
-    "bafq_debug_qrp": (C.c_int, [C.c_int, c_double_p, c_double_p, c_int32_p, c_double_p]),
+    "bafq_debug_qrp": (C.c_int, [C.c_int, c_double_p, c_double_p, c_int32_p, c_double_p, C.c_int]),
     "bafq_debug_mfma_probe": (C.c_int, [c_double_p]),
     "bafq_debug_lane_ops": (C.c_int, [c_double_p, c_int32_p, c_double_p]),
 }
@@ -99,13 +99,14 @@ class NativeLibrary:
             raise RuntimeError("bafq_debug_lane_ops failed")
         return out
 
-    def debug_qrp(self, a):
+    def debug_qrp(self, a, variant=0):
         n = a.shape[0]
         a = np.ascontiguousarray(a, dtype=np.complex128).copy()
         tau = np.zeros(n, np.complex128)
         perm = np.zeros(n, np.int32)
         d = np.zeros(n)
-        rc = self.lib.bafq_debug_qrp(n, dptr(a.view(np.float64)), dptr(tau.view(np.float64)), iptr(perm), dptr(d))
+        rc = self.lib.bafq_debug_qrp(n, dptr(a.view(np.float64)), dptr(tau.view(np.float64)), iptr(perm), dptr(d),
+                                     variant)
         if rc:
             raise RuntimeError("bafq_debug_qrp failed")
         return a, tau, perm, d

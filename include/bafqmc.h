@@ -93,9 +93,11 @@ int bafq_last_kernel_ms(const bafq_handle* h, double* ms);
 
 /* Self-test hooks with no counterpart in the reference: run the device
  * complex GEMM C = op(A) op(B) (op 0: none, 1: conjugate transpose) and the
- * column-pivoted QR on host matrices, for the numerics tests. */
+ * column-pivoted QR on host matrices, for the numerics tests.  QR variant 0 is
+ * the workgroup-cooperative routine on a matrix in device memory, variant 1
+ * the column-parallel routine on a matrix in LDS (n <= 96). */
 int bafq_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
-int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d);
+int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int variant);
 /* Lane layout of the FP64 MFMA 16x16x4 tile operation: out[p][lane][r] =
  * accumulator r of `lane` for A = one-hot at lane p, B = 1000 + lane
  * (64 x 64 x 4 doubles). */

@@ -22,9 +22,8 @@ def hip_lib():
 
 @pytest.fixture(scope="session")
 def emu_lib():
-    # small workgroups keep the thread-per-lane emulator fast (small N only)
-    os.environ.setdefault("BAFQ_BLOCK", "64")
-    os.environ.setdefault("BAFQ_LDS_BLOCK", "128")
+    """The same device code compiled for the CPU (fibers per workgroup), with
+    the GPU's workgroup sizes."""
     from bafqmc_amd.build import EMU_LIB, build_emu
     if not EMU_LIB.exists():
         build_emu()

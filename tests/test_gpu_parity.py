@@ -78,12 +78,11 @@ def test_device_gemm(hip_lib, n):
             assert rel_err(c, A @ B) < 1e-13
 
 
-@pytest.mark.parametrize("n", [9, 18, 72, 81, 144])
-def test_device_qrp(hip_lib, n):
+def check_qrp(lib, n, variant):
     rng = np.random.default_rng(100 + n)
     # graded columns, like the stabilisation matrices
     a = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) * np.logspace(0, 8, n)[None, :]
-    r, tau, perm, d = hip_lib.debug_qrp(a)
+    r, tau, perm, d = lib.debug_qrp(a, variant)
     R = np.triu(r) * d[:, None]
     Q = np.eye(n, dtype=complex)
     for k in range(n - 1, -1, -1):
@@ -97,8 +96,16 @@ def test_device_qrp(hip_lib, n):
     assert np.all(np.diff(d) <= 1e-12 * d[0])          # pivoting orders |R_kk|
     # same |R_kk| as LAPACK's pivoted QR
     import scipy.linalg as sla
-    _, rl, _ = sla.qr(a, pivoting=True)
+    _, rl, pl = sla.qr(a, pivoting=True)
     np.testing.assert_allclose(d, np.abs(np.diag(rl)), rtol=1e-9)
+    # pivots by exact column norms: the same order as LAPACK's on these (tie-free) inputs
+    np.testing.assert_array_equal(perm, pl)
+
+
+@pytest.mark.parametrize("n,variant", [(9, 0), (18, 0), (72, 0), (81, 0), (144, 0),
+                                       (9, 1), (18, 1), (72, 1), (81, 1), (88, 1)])
+def test_device_qrp(hip_lib, n, variant):
+    check_qrp(hip_lib, n, variant)
 
 
 CASES = [
@@ -227,6 +234,10 @@ def test_local_update_variants_match_reference(hip_lib, name, width, monkeypatch
 
 
 def test_uneven_batch_equals_single_chains(hip_lib):
+    check_uneven_batch(hip_lib)
+
+
+def check_uneven_batch(hip_lib):
     """Chains of different lattices/solvers in one launch evolve exactly as alone."""
     specs = [CASES[1], CASES[3], CASES[4], CASES[5]]
     models = [ModelSpec(solver_params(d)) for d in specs]
@@ -254,6 +265,10 @@ def test_uneven_batch_equals_single_chains(hip_lib):
 
 @pytest.mark.parametrize("pairing,delta", [(False, 0.0), (True, 0.2)])
 def test_gaussian_closed_form_gpu(hip_lib, pairing, delta):
+    check_gaussian(hip_lib, pairing, delta)
+
+
+def check_gaussian(hip_lib, pairing, delta):
     d = dict(pairing=pairing, u1=0.0, u2=0.0, mu=-5.0, rdelta=delta, nlx=3, nly=3, ltrot=100, beta=1.0,
              nwrap=10, shift_loc=1.0)
     s = BatchedSolver([ModelSpec(solver_params(d))], [0], lib=hip_lib)
