@@ -65,6 +65,7 @@ struct Sched {
     void* main_sp = nullptr;
     int cur = 0;
     int nthreads = 0;
+    bool reverse = false;                  // run threads in descending order between barriers
     const std::function<void(EmuCtx&)>* body = nullptr;
     EmuGroup* grp = nullptr;
 };
@@ -76,14 +77,20 @@ void switch_to(Sched* s, int from, int to) {
     bafq_emu_switch(&s->sp[from], s->sp[to]);
 }
 
+// Forward order: thread t hands over to t + 1; the last arrival restarts the
+// round at thread 0.  Reverse order (BAFQ_EMU_REVERSE=1): t hands over to t - 1
+// and rounds start at the last thread.  A result that depends on the order
+// reveals a read that no barrier separates from the write it needs.
 void group_barrier(EmuGroup* g, int t) {
     Sched* s = (Sched*)g->sched;
-    switch_to(s, t, t + 1 < s->nthreads ? t + 1 : 0);
+    if (!s->reverse) switch_to(s, t, t + 1 < s->nthreads ? t + 1 : 0);
+    else switch_to(s, t, t > 0 ? t - 1 : s->nthreads - 1);
 }
 
 void wave_barrier(EmuGroup* g, int t) {
     Sched* s = (Sched*)g->sched;
-    switch_to(s, t, (t & 63) != 63 ? t + 1 : t - 63);
+    if (!s->reverse) switch_to(s, t, (t & 63) != 63 ? t + 1 : t - 63);
+    else switch_to(s, t, (t & 63) != 0 ? t - 1 : t + 63);
 }
 
 [[noreturn]] void fiber_main() {
@@ -92,9 +99,10 @@ void wave_barrier(EmuGroup* g, int t) {
     EmuCtx c{s->grp, t, s->grp->smem};
     (*s->body)(c);
     // finished: the next thread resumes; the last one returns to the launcher
-    if (t + 1 < s->nthreads) {
-        s->cur = t + 1;
-        bafq_emu_switch(&s->sp[t], s->sp[t + 1]);
+    const int next = s->reverse ? t - 1 : t + 1;
+    if (next >= 0 && next < s->nthreads) {
+        s->cur = next;
+        bafq_emu_switch(&s->sp[t], s->sp[next]);
     } else {
         bafq_emu_switch(&s->sp[t], s->main_sp);
     }
@@ -135,9 +143,11 @@ void run_one_group(int g, int grid, int block, size_t smem, const std::function<
     while ((int)stacks.size() < block) stacks.emplace_back(new char[kFiberStack]);
     for (int t = 0; t < block; ++t) s.sp[t] = make_context(stacks[t].get(), kFiberStack);
     grp.sched = &s;
+    const char* rev = std::getenv("BAFQ_EMU_REVERSE");
+    s.reverse = rev && rev[0] == '1';
     tl_sched = &s;
-    s.cur = 0;
-    bafq_emu_switch(&s.main_sp, s.sp[0]);
+    s.cur = s.reverse ? block - 1 : 0;
+    bafq_emu_switch(&s.main_sp, s.sp[s.cur]);
     tl_sched = nullptr;
 }
 

@@ -12,7 +12,10 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from tests.helpers import rel_err
+from tests.helpers import rel_err, fixed_fields, solver_params
+from tests.test_gpu_parity import CASES
+from bafqmc_amd.model import ModelSpec
+from bafqmc_amd.solver import BatchedSolver
 from tests.test_gpu_parity import (check_sweep_vs_oracle, check_golden, check_mfma_layout, check_lane_ops, check_qrp,
                                    check_uneven_batch, check_gaussian, GOLDEN)
 
@@ -72,3 +75,25 @@ def test_emu_gaussian_closed_form(emu_lib, pairing, delta):
 
 def test_emu_lane_ops(emu_lib):
     check_lane_ops(emu_lib)
+
+
+@pytest.mark.parametrize("case,force_global", [(3, "0"), (5, "0"), (4, "1")])
+def test_emu_thread_order_independence(emu_lib, case, force_global, monkeypatch):
+    """Race check: the same sweep with the threads of every workgroup run in
+    ascending and in descending order between barriers gives identical bits."""
+    monkeypatch.setenv("BAFQ_FORCE_GLOBAL", force_global)
+    d = CASES[case]
+    ns = d["nlx"] * d["nly"]
+    phi = fixed_fields(d["ltrot"], ns)
+    out = []
+    for rev in ("0", "1"):
+        monkeypatch.setenv("BAFQ_EMU_REVERSE", rev)
+        s = BatchedSolver([ModelSpec(solver_params(d))], [0], lib=emu_lib)
+        s.set_fields([phi], [1357911])
+        s.prepare()
+        s.sweep(1)
+        obs, st = s.read_bin_raw()
+        out.append((s.green(0), obs, s.get_fields()[0][0]))
+        s.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
