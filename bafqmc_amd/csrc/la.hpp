@@ -501,75 +501,88 @@ DEV void refl_apply_col(const Ctx& c, cd (&x)[R], LDS_AS const cd* A, int n, int
             x[i] = x[i] - refl_entry(A, n, gl + kColGroup * i, k) * s;
 }
 
+// ZLARFG scalars of the quad's register column as pivot of step k: alpha = x_k,
+// beta = -sign(Re alpha) |x_{k..n-1}|, tau = (beta - alpha) / beta,
+// scale = 1 / (alpha - beta); returns |x_{k..n-1}|^2 (the pivoting norm).
+template <int R>
+DEV double col_candidate(const Ctx& c, const cd (&x)[R], int n, int k, int gl, double& beta, cd& t, cd& scale) {
+    double ar = 0.0, ai = 0.0, x2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (kColGroup * i + kColGroup - 1 >= k && kColGroup * i < n) {
+            const int r = gl + kColGroup * i;
+            ar += r == k ? x[i].r : 0.0;
+            ai += r == k ? x[i].i : 0.0;
+            x2 += r > k ? abs2(x[i]) : 0.0;
+        }
+    }
+    ar = c.quad_sum(ar);
+    ai = c.quad_sum(ai);
+    x2 = c.quad_sum(x2);
+    if (x2 == 0.0 && ai == 0.0) {
+        beta = ar;
+        t = cd{0.0, 0.0};
+        scale = cd{0.0, 0.0};
+    } else {
+        const double nr = sqrt(ar * ar + ai * ai + x2);
+        beta = ar >= 0.0 ? -nr : nr;
+        t = cd{(beta - ar) / beta, -ai / beta};
+        scale = cinv(cd{ar - beta, ai});
+    }
+    return ar * ar + ai * ai + x2;
+}
+
+// Largest pivoting norm among the wavefront's active columns (ties to the
+// lower column), published in rec[2 w], rec[2 w + 1] for the next pivot search.
+DEV void publish_best(const Ctx& c, LDS_AS double* rec, bool active, double a2, int col, int n) {
+    double bv = active ? a2 : -1.0;
+    int bi = active ? col : n;
+    c.wave_argmax(bv, bi);
+    if (c.lane() == 0) {
+        rec[2 * c.wave()] = bv;
+        rec[2 * c.wave() + 1] = (double)bi;
+    }
+}
+
 // Householder QR with column pivoting (ZGEQP3 semantics, exact column norms)
 // with the same outputs as qrp().  Quad q owns original column q for the whole
-// factorization; step k picks the largest remaining norm (ties to the lower
-// original index), the owner forms the reflector and publishes column k of the
-// result (V = D^{-1} R above the diagonal, reflector below), the other quads
-// apply it to their register columns: two barriers per step.
+// factorization and keeps, besides the column, the ZLARFG scalars it would use
+// as the next pivot.  Step k: every wavefront picks the pivot from the
+// per-wavefront best norms of step k - 1 (ties to the lower original column);
+// the owner writes column k of the result (R above the diagonal, beta, the
+// reflector below) -- barrier -- the other quads apply H(k)^H to their columns,
+// compute their next norms and scalars, and each wavefront publishes its best
+// norm -- barrier.
 template <int R>
 DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, double* D_) {
     LDS_AS cd* tau = as_lds(qs.tau);
     LDS_AS int* perm = as_lds(qs.perm);
-    LDS_AS double* nrm = as_lds(qs.nrm);
+    LDS_AS double* rec = as_lds(qs.bcast);      // 2 doubles per wavefront (<= 8)
     LDS_AS double* D = as_lds(D_);
     const int col = c.tid() / kColGroup, gl = c.tid() % kColGroup;
     const int colc = col < n ? col : n - 1;
+    const int nw = c.nwaves();
     bool active = col < n;
     cd x[R];
-    double a2 = 0.0;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         const int r = gl + kColGroup * i;
         const cd v = ldc(A, (r < n ? r : n - 1) * n + colc);
         const bool in = active && r < n;
         x[i] = cd{in ? v.r : 0.0, in ? v.i : 0.0};
-        a2 += abs2(x[i]);
     }
-    a2 = c.quad_sum(a2);
-    if (active && gl == 0) nrm[col] = a2;
-    c.sync();
+    double beta;
+    cd t, scale;
+    double a2 = col_candidate<R>(c, x, n, 0, gl, beta, t, scale);
+    publish_best(c, rec, active, a2, col, n);
+    c.sync();                                    // columns in registers: A may be overwritten
 #pragma unroll 1
     for (int k = 0; k < n; ++k) {
-        // ---- pivot (every wavefront computes the same answer)
-        double bv = -1.0;
-        int bi = n;
-        for (int j = c.lane(); j < n; j += 64) {
-            const double v = nrm[j];
-            if (v > bv) {
-                bv = v;
-                bi = j;
-            }
-        }
+        // ---- pivot from the per-wavefront records
+        double bv = c.lane() < nw ? rec[2 * c.lane()] : -1.0;
+        int bi = c.lane() < nw ? (int)rec[2 * c.lane() + 1] : n;
         c.wave_argmax(bv, bi);
-        const bool piv = col == bi;
-        // ---- reflector of the pivot column (ZLARFG conventions), owner wavefront only
-        if (c.wave() == bi * kColGroup / 64) {
-        double ar = 0.0, ai = 0.0, x2 = 0.0;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            const int r = gl + kColGroup * i;
-            const bool isk = piv && r == k, below = piv && r > k;
-            ar += isk ? x[i].r : 0.0;
-            ai += isk ? x[i].i : 0.0;
-            x2 += below ? abs2(x[i]) : 0.0;
-        }
-        ar = c.quad_sum(ar);
-        ai = c.quad_sum(ai);
-        x2 = c.quad_sum(x2);
-        if (piv) {
-            double beta;
-            cd t, scale;
-            if (x2 == 0.0 && ai == 0.0) {
-                beta = ar;
-                t = cd{0.0, 0.0};
-                scale = cd{0.0, 0.0};
-            } else {
-                const double nr = sqrt(ar * ar + ai * ai + x2);
-                beta = ar >= 0.0 ? -nr : nr;
-                t = cd{(beta - ar) / beta, -ai / beta};
-                scale = cinv(cd{ar - beta, ai});
-            }
+        if (col == bi) {
             // column k of the result: R above the diagonal (normalised at the
             // end), beta on it, the reflector below
 #pragma unroll
@@ -585,23 +598,15 @@ DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, doub
             }
             active = false;
         }
-        }
         c.sync();
-        // ---- H(k)^H = I - conj(tau) v v^H on the remaining columns, new norms
+        // ---- H(k)^H = I - conj(tau) v v^H on the remaining columns, next scalars
         if (k + 1 < n) {
-            // the chosen column leaves the pivot search; every wavefront has read
-            // the norms of this step before the barrier above
-            if (piv && gl == 0) nrm[col] = -1.0;
             if (c.wave_any(active)) {
                 const cd tk = ldc(tau, k);
                 refl_apply_col<R>(c, x, A, n, k, csel(active, conj(tk), cd{0.0, 0.0}), gl);
-                a2 = 0.0;
-#pragma unroll
-                for (int i = 0; i < R; ++i)
-                    if (kColGroup * i + kColGroup - 1 > k) a2 += (gl + kColGroup * i > k) ? abs2(x[i]) : 0.0;
-                a2 = c.quad_sum(a2);
-                if (active && gl == 0) nrm[col] = a2;
+                a2 = col_candidate<R>(c, x, n, k + 1, gl, beta, t, scale);
             }
+            publish_best(c, rec, active, a2, col, n);
             c.sync();
         }
     }
