@@ -392,6 +392,14 @@ int bafq_sweep(bafq_handle* h, int n_sweeps, int measure) {
     return launch_all(h, bafq::K_SWEEP, n_sweeps, measure);
 }
 
+int bafq_get_stats(bafq_handle* h, double* stats) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (rt_sync()) return fail(h, rt_error());
+    if (rt_d2h(stats, h->d_stats, sizeof(double) * S_LEN * h->chains.size())) return fail(h, rt_error());
+    return 0;
+}
+
 int bafq_read_bin(bafq_handle* h, double* obs, double* stats) {
     using namespace bafq;
     if (!h) return 1;

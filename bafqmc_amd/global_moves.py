@@ -1,0 +1,89 @@
+"""Global field moves accepted by exact Metropolis-Hastings.
+
+The weight of a field configuration phi (time slices x sites x 2 channels) is
+the product of the Gaussian HS measure and the bosonic determinant factor:
+
+* number conserving (two flavours, the second the complex conjugate of the
+  first): W = exp(-sum phi^2 / 2) |det G|^2, G = (1 - B(beta, 0))^-1 -- the
+  weight whose logarithm src/number_conserving/src/pole_diagnostics.f90:130-131
+  reports as log_weight;
+* Nambu pairing: W = exp(-sum phi^2 / 2) exp(-alpha_2 sum phi_2) |det G_2Ns|,
+  the 4Ns Nambu form's det(1 - B_4Ns)^(-1/2) = |det(1 - B_2Ns)|^-1 of the
+  reduced form, with the normal-ordering constant of the U2 channel
+  (src/pairing/src/operator_Hubbard.f90:100-137).
+
+Single-site moves of this weight reproduce the local ratios of LocalU_metro
+exactly (tests/test_global_moves.py).  The reference declares a global update
+(src/number_conserving/src/global_update.f90) but never calls it; the moves
+here are new, and the local sweep stays the sampler that they interleave with.
+
+A move: propose phi' for every chain, rebuild G(phi') from scratch (the
+stabilised B-product of bafq_prepare gives log|det G'|), accept with
+min(1, W'/W q(phi|phi') / q(phi'|phi)), and restore phi for rejected chains.
+The acceptance uniforms and the proposals come from a host generator, so the
+chains' own LCG streams are not touched.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+LOGDET = 7          # stats slot of log|det G| (csrc/sweep.hpp S_LOGDET)
+
+
+def log_weight(params, phi, logdet):
+    """log W of one chain's fields given log|det G| (see the module docstring)."""
+    lw = -0.5 * float(np.sum(phi * phi))
+    if params.pairing:
+        dtau = params.beta / params.ltrot
+        a2 = math.sqrt(-2.0 * params.u2 * dtau) if params.u2 < -1e-10 else 0.0
+        return lw - a2 * float(np.sum(phi[:, :, 1])) + logdet
+    return lw + 2.0 * logdet
+
+
+def log_weights(solver, fields=None):
+    """log W of every chain's current fields (G must be current: after
+    prepare or a sweep)."""
+    if fields is None:
+        fields, _ = solver.get_fields()
+    st = solver.stats()
+    return np.array([log_weight(solver.params(c), fields[c], st[c, LOGDET]) for c in range(solver.nchains)])
+
+
+def metropolis_global(solver, proposals, rng, log_q_ratio=None):
+    """One exact global Metropolis-Hastings step for every chain.
+
+    proposals: per-chain proposed fields phi'; log_q_ratio: per-chain
+    log q(phi|phi') - log q(phi'|phi) (None for a symmetric proposal).
+    Returns the boolean acceptance per chain.  Leaves G current."""
+    fields, seeds = solver.get_fields()
+    lw0 = log_weights(solver, fields)
+    solver.set_fields(proposals, seeds)
+    solver.prepare()
+    lw1 = log_weights(solver, proposals)
+    log_a = lw1 - lw0
+    if log_q_ratio is not None:
+        log_a = log_a + np.asarray(log_q_ratio)
+    u = rng.random(solver.nchains)
+    accept = np.log(np.maximum(u, 1e-300)) < log_a
+    if not np.all(accept):
+        final = [proposals[c] if accept[c] else fields[c] for c in range(solver.nchains)]
+        solver.set_fields(final, seeds)
+        solver.prepare()
+    return accept
+
+
+def uniform_shift_proposals(solver, rng, width):
+    """phi' = phi + delta on every slice and site, one delta ~ U(-width/2, width/2)
+    per chain and active channel (symmetric)."""
+    fields, _ = solver.get_fields()
+    out = []
+    for c, f in enumerate(fields):
+        p = solver.params(c)
+        g = f.copy()
+        for nf, active in ((0, abs(p.u1) > 1e-10), (1, abs(p.u2) > 1e-10)):
+            if active:
+                g[:, :, nf] += width * (rng.random() - 0.5)
+        out.append(g)
+    return out

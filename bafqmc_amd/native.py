@@ -39,6 +39,7 @@ SYMBOLS = {
     "bafq_prepare": (C.c_int, [C.c_void_p]),
     "bafq_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "bafq_read_bin": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    "bafq_get_stats": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_get_green": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),

@@ -143,6 +143,12 @@ class BatchedSolver:
         self._check(self.lib.bafq_get_green(self.h, int(chain), dptr(g.view(np.float64))), "bafq_get_green")
         return g
 
+    def stats(self):
+        """Per-chain counters without resetting them (slot 7: log|det G|)."""
+        st = np.zeros((self.nchains, self.stats_stride))
+        self._check(self.lib.bafq_get_stats(self.h, dptr(st)), "bafq_get_stats")
+        return st
+
     def read_bin_raw(self):
         obs = np.zeros((self.nchains, self.obs_stride))
         st = np.zeros((self.nchains, self.stats_stride))

@@ -82,6 +82,13 @@ int bafq_sweep(bafq_handle* h, int n_sweeps, int measure);
  * stats: n_chains * stats_stride. */
 int bafq_read_bin(bafq_handle* h, double* obs, double* stats);
 
+/* The per-chain counters of bafq_read_bin without reading or resetting
+ * anything; slot 7 holds log|det G| of the chain's current fields (from its
+ * last stabilised Green's function), the quantity the exact acceptance of a
+ * global move needs (the weight of src/number_conserving/src/pole_diagnostics.f90:130-131).
+ * No counterpart in the reference's executable, which runs no global moves. */
+int bafq_get_stats(bafq_handle* h, double* stats);
+
 /* Current equal-time Green's function of one chain, N x N complex
  * (Prop%Gr, src/number_conserving/src/process_matrix.f90:8-10). */
 int bafq_get_green(bafq_handle* h, int chain, double* g);
