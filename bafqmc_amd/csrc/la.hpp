@@ -142,7 +142,9 @@ DEV void mfma_gemm(const Ctx& c, int n, LA la, LB lb, ST st) {
     for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
         const int ti = t / nt, tj = t - ti * nt;
         const int ar = 16 * ti + li, bc = 16 * tj + li;
-        d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+        // three real products per complex one (Gauss): Re = sum ar br - sum ai bi,
+        // Im = sum (ar + ai)(br + bi) - sum ar br - sum ai bi
+        d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ss = rr;
         for (int k0 = 0; k0 < nk; k0 += kChunk) {
             cd a[kChunk], b[kChunk];
 #pragma unroll
@@ -156,16 +158,15 @@ DEV void mfma_gemm(const Ctx& c, int n, LA la, LB lb, ST st) {
 #pragma unroll
             for (int u = 0; u < kChunk; ++u) {
                 if (k0 + u < nk) {
-                    accr = c.mfma(a[u].r, b[u].r, accr);
-                    accr = c.mfma(-a[u].i, b[u].i, accr);
-                    acci = c.mfma(a[u].r, b[u].i, acci);
-                    acci = c.mfma(a[u].i, b[u].r, acci);
+                    rr = c.mfma(a[u].r, b[u].r, rr);
+                    ii = c.mfma(a[u].i, b[u].i, ii);
+                    ss = c.mfma(a[u].r + a[u].i, b[u].r + b[u].i, ss);
                 }
             }
         }
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
-            if (row < n && col < n) st(row, col, cd{accr[r], acci[r]});
+            if (row < n && col < n) st(row, col, cd{rr[r] - ii[r], ss[r] - rr[r] - ii[r]});
         }
     }
     c.sync();

@@ -430,7 +430,8 @@ DEVFN void delayed_flush(const Ctx& c, cd* G_, const cd* Ut_, const cd* W_, int 
     for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
         const int ti = t / nt, tj = t - ti * nt;
         const int ar = 16 * ti + li, bc = 16 * tj + li;
-        cd a[4], b[4];
+        // operands and the tile of G0 are loaded before the products
+        cd a[4], b[4], g[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int kk = 4 * q + lk;
@@ -439,22 +440,28 @@ DEVFN void delayed_flush(const Ctx& c, cd* G_, const cd* Ut_, const cd* W_, int 
             if (ar < n && kk < cols) a[q] = ldc(Ut, kk * n + ar);
             if (bc < n && kk < cols) b[q] = ldc(W, kk * n + bc);
         }
-        d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
+            g[r] = cd{0.0, 0.0};
+            if (row < n && col < n) g[r] = ldc(G, row * n + col);
+        }
+        // three real products per complex one (Gauss): Re = ar br - ai bi,
+        // Im = (ar + ai)(br + bi) - ar br - ai bi; three independent chains
+        d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ss = rr;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (4 * q < cols) {
-                accr = c.mfma(a[q].r, b[q].r, accr);
-                accr = c.mfma(-a[q].i, b[q].i, accr);
-                acci = c.mfma(a[q].r, b[q].i, acci);
-                acci = c.mfma(a[q].i, b[q].r, acci);
+                rr = c.mfma(a[q].r, b[q].r, rr);
+                ii = c.mfma(a[q].i, b[q].i, ii);
+                ss = c.mfma(a[q].r + a[q].i, b[q].r + b[q].i, ss);
             }
         }
+#pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
-            if (row < n && col < n) {
-                const cd g = ldc(G, row * n + col);
-                stc(G, row * n + col, cd{g.r - accr[r], g.i - acci[r]});
-            }
+            if (row < n && col < n)
+                stc(G, row * n + col, cd{g[r].r - (rr[r] - ii[r]), g[r].i - (ss[r] - rr[r] - ii[r])});
         }
     }
     c.sync();
