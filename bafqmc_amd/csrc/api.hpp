@@ -11,6 +11,7 @@
 
 #include "../../include/bafqmc.h"
 #include "sweep.hpp"
+#include "net.hpp"
 
 namespace bafq {
 
@@ -34,6 +35,15 @@ int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, d
 int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int variant);
 int rt_debug_mfma(double* out);
 int rt_debug_lanes(const double* in, const int32_t* idx, double* out);
+// periodic 3D convolution network (net.hpp); device pointers
+int rt_conv_fwd(const float* in, int cin, const float* W, const float* b, float* out, int cout, int batch,
+                NetShape s, int relu);
+int rt_conv_bwd_data(const float* gout, int cout, const float* W, float* gin, int cin, const float* act, int mask,
+                     int batch, NetShape s);
+int rt_conv_bwd_weight(const float* gout, int cout, const float* in, int cin, float* gW, float* gb, int batch,
+                       NetShape s);
+int rt_mse_grad(const float* y, const float* t, float* g, size_t n, float scale);
+int rt_adam(float* p, const float* g, float* m, float* v, float lr, int step, size_t n);
 
 constexpr double kZero = 1.0e-10;   // src/number_conserving/src/calc_basic.f90:5
 
@@ -448,5 +458,175 @@ int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int 
 
 int bafq_debug_mfma_probe(double* out) { return bafq::rt_debug_mfma(out); }
 int bafq_debug_lane_ops(const double* in, const int32_t* idx, double* out) { return bafq::rt_debug_lanes(in, idx, out); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ network
+struct bafq_net {
+    bafq_net_desc d;
+    int max_batch = 0;
+    size_t nparam = 0;
+    std::vector<size_t> woff, boff;      // parameter offsets per layer
+    std::vector<int> cin, cout;
+    float* dev = nullptr;                // one allocation
+    float *params = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+    std::vector<float*> act;             // act[0] input ... act[L] output
+    float *g0 = nullptr, *g1 = nullptr, *target = nullptr;
+    int step = 0;
+    std::string err;
+};
+
+static int net_fail(bafq_net* h, const std::string& msg) {
+    if (h) h->err = msg;
+    return 1;
+}
+
+extern "C" {
+
+int bafq_net_create(const bafq_net_desc* d, int max_batch, bafq_net** out) {
+    using namespace bafq;
+    *out = nullptr;
+    bafq_net* h = new bafq_net();
+    *out = h;
+    if (!d || d->lt < 1 || d->ly < 1 || d->lx < 1 || d->channels < 1 || d->hidden < 1 || d->layers < 2 ||
+        max_batch < 1)
+        return net_fail(h, "invalid network description");
+    h->d = *d;
+    h->max_batch = max_batch;
+    const int L = d->layers;
+    for (int l = 0; l < L; ++l) {
+        h->cin.push_back(l == 0 ? d->channels : d->hidden);
+        h->cout.push_back(l == L - 1 ? d->channels : d->hidden);
+        h->woff.push_back(h->nparam);
+        h->nparam += (size_t)h->cout[l] * h->cin[l] * 27;
+        h->boff.push_back(h->nparam);
+        h->nparam += (size_t)h->cout[l];
+    }
+    const size_t vol = (size_t)d->lt * d->ly * d->lx;
+    const int cmax = d->channels > d->hidden ? d->channels : d->hidden;
+    size_t total = 4 * h->nparam + 2 * (size_t)max_batch * cmax * vol + (size_t)max_batch * d->channels * vol;
+    for (int l = 0; l <= L; ++l) total += (size_t)max_batch * (l == 0 ? d->channels : h->cout[l - 1]) * vol;
+    if (rt_malloc((void**)&h->dev, total * sizeof(float))) return net_fail(h, rt_error());
+    if (rt_memset(h->dev, 0, total * sizeof(float))) return net_fail(h, rt_error());
+    float* p = h->dev;
+    auto take = [&](size_t n) { float* q = p; p += n; return q; };
+    h->params = take(h->nparam);
+    h->grad = take(h->nparam);
+    h->m = take(h->nparam);
+    h->v = take(h->nparam);
+    for (int l = 0; l <= L; ++l) h->act.push_back(take((size_t)max_batch * (l == 0 ? d->channels : h->cout[l - 1]) * vol));
+    h->g0 = take((size_t)max_batch * cmax * vol);
+    h->g1 = take((size_t)max_batch * cmax * vol);
+    h->target = take((size_t)max_batch * d->channels * vol);
+    return 0;
+}
+
+void bafq_net_destroy(bafq_net* h) {
+    if (!h) return;
+    if (h->dev) bafq::rt_free(h->dev);
+    delete h;
+}
+
+const char* bafq_net_last_error(const bafq_net* h) { return h ? h->err.c_str() : "null handle"; }
+
+int64_t bafq_net_param_count(const bafq_net* h) { return h ? (int64_t)h->nparam : -1; }
+
+int bafq_net_set_params(bafq_net* h, const float* params) {
+    using namespace bafq;
+    if (!h) return 1;
+    h->step = 0;
+    if (rt_h2d(h->params, params, h->nparam * sizeof(float)) || rt_memset(h->m, 0, h->nparam * sizeof(float)) ||
+        rt_memset(h->v, 0, h->nparam * sizeof(float)) || rt_sync())
+        return net_fail(h, rt_error());
+    return 0;
+}
+
+int bafq_net_get_params(bafq_net* h, float* params) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (rt_sync() || rt_d2h(params, h->params, h->nparam * sizeof(float))) return net_fail(h, rt_error());
+    return 0;
+}
+
+static int net_forward_dev(bafq_net* h, int batch) {
+    using namespace bafq;
+    const NetShape s{h->d.lt, h->d.ly, h->d.lx};
+    const int L = h->d.layers;
+    for (int l = 0; l < L; ++l)
+        if (rt_conv_fwd(h->act[l], h->cin[l], h->params + h->woff[l], h->params + h->boff[l], h->act[l + 1],
+                        h->cout[l], batch, s, l < L - 1 ? 1 : 0))
+            return net_fail(h, rt_error());
+    return 0;
+}
+
+int bafq_net_forward(bafq_net* h, int batch, const float* x, float* y) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (batch < 1 || batch > h->max_batch) return net_fail(h, "batch out of range");
+    const size_t n = (size_t)batch * h->d.channels * h->d.lt * h->d.ly * h->d.lx;
+    if (rt_h2d(h->act[0], x, n * sizeof(float))) return net_fail(h, rt_error());
+    if (net_forward_dev(h, batch)) return 1;
+    if (rt_sync() || rt_d2h(y, h->act[h->d.layers], n * sizeof(float))) return net_fail(h, rt_error());
+    return 0;
+}
+
+// Loss, gradient in h->grad; returns the loss through *loss.
+static int net_gradient_dev(bafq_net* h, int batch, double* loss) {
+    using namespace bafq;
+    const NetShape s{h->d.lt, h->d.ly, h->d.lx};
+    const int L = h->d.layers;
+    const size_t n = (size_t)batch * h->d.channels * s.vol();
+    if (net_forward_dev(h, batch)) return 1;
+    float* gcur = h->g0;
+    float* gnext = h->g1;
+    if (rt_mse_grad(h->act[L], h->target, gcur, n, 2.0f / (float)n)) return net_fail(h, rt_error());
+    {
+        std::vector<float> g(n);
+        if (rt_sync() || rt_d2h(g.data(), gcur, n * sizeof(float))) return net_fail(h, rt_error());
+        double acc = 0.0;
+        for (size_t i = 0; i < n; ++i) acc += (double)g[i] * g[i];
+        *loss = acc * (double)n / 4.0;     // g = 2 (y - t) / n
+    }
+    for (int l = L - 1; l >= 0; --l) {
+        if (rt_conv_bwd_weight(gcur, h->cout[l], h->act[l], h->cin[l], h->grad + h->woff[l], h->grad + h->boff[l],
+                               batch, s))
+            return net_fail(h, rt_error());
+        if (l > 0) {
+            if (rt_conv_bwd_data(gcur, h->cout[l], h->params + h->woff[l], gnext, h->cin[l], h->act[l], 1, batch, s))
+                return net_fail(h, rt_error());
+            float* t = gcur; gcur = gnext; gnext = t;
+        }
+    }
+    return 0;
+}
+
+int bafq_net_gradient(bafq_net* h, int batch, const float* x, const float* target, double* loss, float* grad) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (batch < 1 || batch > h->max_batch) return net_fail(h, "batch out of range");
+    const size_t n = (size_t)batch * h->d.channels * h->d.lt * h->d.ly * h->d.lx;
+    if (rt_h2d(h->act[0], x, n * sizeof(float)) || rt_h2d(h->target, target, n * sizeof(float)))
+        return net_fail(h, rt_error());
+    if (net_gradient_dev(h, batch, loss)) return 1;
+    if (grad && (rt_sync() || rt_d2h(grad, h->grad, h->nparam * sizeof(float)))) return net_fail(h, rt_error());
+    return 0;
+}
+
+int bafq_net_train(bafq_net* h, int batch, const float* x, const float* target, int steps, float lr,
+                   double* loss) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (batch < 1 || batch > h->max_batch || steps < 1) return net_fail(h, "batch or steps out of range");
+    const size_t n = (size_t)batch * h->d.channels * h->d.lt * h->d.ly * h->d.lx;
+    if (rt_h2d(h->act[0], x, n * sizeof(float)) || rt_h2d(h->target, target, n * sizeof(float)))
+        return net_fail(h, rt_error());
+    for (int k = 0; k < steps; ++k) {
+        if (net_gradient_dev(h, batch, loss)) return 1;
+        ++h->step;
+        if (rt_adam(h->params, h->grad, h->m, h->v, lr, h->step, h->nparam)) return net_fail(h, rt_error());
+        // the input activation is overwritten by nothing: act[0] stays x
+    }
+    return rt_sync() ? net_fail(h, rt_error()) : 0;
+}
 
 }  // extern "C"

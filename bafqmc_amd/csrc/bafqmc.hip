@@ -111,6 +111,100 @@ int rt_debug_lanes(const double* in, const int32_t* idx, double* out) {
     return rc;
 }
 
+// ------------------------------------------------------------------ network
+__global__ void __launch_bounds__(256) k_conv_fwd(const float* in, int cin, const float* W, const float* b, float* out,
+                                                  int cout, int n, NetShape s, int relu) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const float y = conv_fwd_elem(in, cin, W, b, cout, s, e);
+        out[e] = relu && y < 0.0f ? 0.0f : y;
+    }
+}
+__global__ void __launch_bounds__(256) k_conv_bwd_data(const float* gout, int cout, const float* W, float* gin,
+                                                       int cin, const float* act, int mask, int n, NetShape s) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+        gin[e] = conv_bwd_data_elem(gout, cout, W, cin, act, mask != 0, s, e);
+}
+// one workgroup per weight; its threads split the batch x volume sum
+__global__ void __launch_bounds__(256) k_conv_bwd_weight(const float* gout, int cout, const float* in, int cin,
+                                                         float* gW, int batch, NetShape s) {
+    __shared__ float part[256];
+    const int e = blockIdx.x;
+    const int v = s.vol();
+    const int co = e / (cin * 27), r0 = e - co * cin * 27;
+    const int ci = r0 / 27, d = r0 - ci * 27;
+    const int dt = d / 9 - 1, dy = (d / 3) % 3 - 1, dx = d % 3 - 1;
+    float acc = 0.0f;
+    for (int q = threadIdx.x; q < batch * v; q += blockDim.x) {
+        const int bb = q / v, p = q - bb * v;
+        const int t = p / (s.ly * s.lx), r1 = p - t * s.ly * s.lx;
+        const int y = r1 / s.lx, x = r1 - y * s.lx;
+        const int src = ((wrap_idx(t + dt, s.lt) * s.ly) + wrap_idx(y + dy, s.ly)) * s.lx + wrap_idx(x + dx, s.lx);
+        acc += gout[((size_t)bb * cout + co) * v + p] * in[((size_t)bb * cin + ci) * v + src];
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gW[e] = part[0];
+}
+__global__ void __launch_bounds__(256) k_conv_bwd_bias(const float* gout, int cout, float* gb, int batch, NetShape s) {
+    __shared__ float part[256];
+    const int co = blockIdx.x, v = s.vol();
+    float acc = 0.0f;
+    for (int q = threadIdx.x; q < batch * v; q += blockDim.x) {
+        const int bb = q / v, p = q - bb * v;
+        acc += gout[((size_t)bb * cout + co) * v + p];
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gb[co] = part[0];
+}
+__global__ void __launch_bounds__(256) k_mse_grad(const float* y, const float* t, float* g, size_t n, float scale) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        g[e] = scale * (y[e] - t[e]);
+}
+__global__ void __launch_bounds__(256) k_adam(float* p, const float* g, float* m, float* v, float lr, int step,
+                                              size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        adam_elem(p, g, m, v, lr, step, (int)e);
+}
+
+static int grid_for(size_t n) { return (int)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535); }
+
+int rt_conv_fwd(const float* in, int cin, const float* W, const float* b, float* out, int cout, int batch,
+                NetShape s, int relu) {
+    const int n = batch * cout * s.vol();
+    hipLaunchKernelGGL(k_conv_fwd, dim3(grid_for(n)), dim3(256), 0, 0, in, cin, W, b, out, cout, n, s, relu);
+    return check(hipGetLastError(), "k_conv_fwd");
+}
+int rt_conv_bwd_data(const float* gout, int cout, const float* W, float* gin, int cin, const float* act, int mask,
+                     int batch, NetShape s) {
+    const int n = batch * cin * s.vol();
+    hipLaunchKernelGGL(k_conv_bwd_data, dim3(grid_for(n)), dim3(256), 0, 0, gout, cout, W, gin, cin, act, mask, n, s);
+    return check(hipGetLastError(), "k_conv_bwd_data");
+}
+int rt_conv_bwd_weight(const float* gout, int cout, const float* in, int cin, float* gW, float* gb, int batch,
+                       NetShape s) {
+    hipLaunchKernelGGL(k_conv_bwd_weight, dim3(cout * cin * 27), dim3(256), 0, 0, gout, cout, in, cin, gW, batch, s);
+    if (check(hipGetLastError(), "k_conv_bwd_weight")) return 1;
+    hipLaunchKernelGGL(k_conv_bwd_bias, dim3(cout), dim3(256), 0, 0, gout, cout, gb, batch, s);
+    return check(hipGetLastError(), "k_conv_bwd_bias");
+}
+int rt_mse_grad(const float* y, const float* t, float* g, size_t n, float scale) {
+    hipLaunchKernelGGL(k_mse_grad, dim3(grid_for(n)), dim3(256), 0, 0, y, t, g, n, scale);
+    return check(hipGetLastError(), "k_mse_grad");
+}
+int rt_adam(float* p, const float* g, float* m, float* v, float lr, int step, size_t n) {
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, 0, p, g, m, v, lr, step, n);
+    return check(hipGetLastError(), "k_adam");
+}
+
 int rt_debug_mfma(double* out) {
     double* d = nullptr;
     if (rt_malloc((void**)&d, sizeof(double) * 64 * 64 * 4)) return 1;

@@ -235,6 +235,37 @@ int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block,
     return 0;
 }
 
+// ------------------------------------------------------------------ network
+int rt_conv_fwd(const float* in, int cin, const float* W, const float* b, float* out, int cout, int batch,
+                NetShape s, int relu) {
+    const int n = batch * cout * s.vol();
+    for (int e = 0; e < n; ++e) {
+        const float y = conv_fwd_elem(in, cin, W, b, cout, s, e);
+        out[e] = relu && y < 0.0f ? 0.0f : y;
+    }
+    return 0;
+}
+int rt_conv_bwd_data(const float* gout, int cout, const float* W, float* gin, int cin, const float* act, int mask,
+                     int batch, NetShape s) {
+    const int n = batch * cin * s.vol();
+    for (int e = 0; e < n; ++e) gin[e] = conv_bwd_data_elem(gout, cout, W, cin, act, mask != 0, s, e);
+    return 0;
+}
+int rt_conv_bwd_weight(const float* gout, int cout, const float* in, int cin, float* gW, float* gb, int batch,
+                       NetShape s) {
+    for (int e = 0; e < cout * cin * 27; ++e) gW[e] = conv_bwd_weight_elem(gout, cout, in, cin, 0, batch, s, e);
+    for (int co = 0; co < cout; ++co) gb[co] = conv_bwd_bias_elem(gout, cout, batch, s, co);
+    return 0;
+}
+int rt_mse_grad(const float* y, const float* t, float* g, size_t n, float scale) {
+    for (size_t e = 0; e < n; ++e) g[e] = scale * (y[e] - t[e]);
+    return 0;
+}
+int rt_adam(float* p, const float* g, float* m, float* v, float lr, int step, size_t n) {
+    for (size_t e = 0; e < n; ++e) adam_elem(p, g, m, v, lr, step, (int)e);
+    return 0;
+}
+
 int rt_debug_lanes(const double* in, const int32_t* idx, double* out) {
     run_groups(1, 64, 0, [&](EmuCtx& c) {
         const int l = c.lane();

@@ -28,6 +28,13 @@ class ModelDesc(C.Structure):
     ]
 
 
+class NetDesc(C.Structure):
+    _fields_ = [("lt", C.c_int32), ("ly", C.c_int32), ("lx", C.c_int32),
+                ("channels", C.c_int32), ("hidden", C.c_int32), ("layers", C.c_int32)]
+
+
+c_float_p = C.POINTER(C.c_float)
+
 SYMBOLS = {
     "bafq_create": (C.c_int, [C.POINTER(ModelDesc), C.c_int, c_int32_p, C.c_int, C.POINTER(C.c_void_p)]),
     "bafq_destroy": (None, [C.c_void_p]),
@@ -40,6 +47,15 @@ SYMBOLS = {
     "bafq_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "bafq_read_bin": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     "bafq_get_stats": (C.c_int, [C.c_void_p, c_double_p]),
+    "bafq_net_create": (C.c_int, [C.POINTER(NetDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "bafq_net_destroy": (None, [C.c_void_p]),
+    "bafq_net_last_error": (C.c_char_p, [C.c_void_p]),
+    "bafq_net_param_count": (C.c_int64, [C.c_void_p]),
+    "bafq_net_set_params": (C.c_int, [C.c_void_p, c_float_p]),
+    "bafq_net_get_params": (C.c_int, [C.c_void_p, c_float_p]),
+    "bafq_net_forward": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p]),
+    "bafq_net_gradient": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, c_double_p, c_float_p]),
+    "bafq_net_train": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, C.c_int, C.c_float, c_double_p]),
     "bafq_get_green": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),

@@ -115,6 +115,37 @@ int bafq_debug_mfma_probe(double* out);
  * out[256:320] the wavefront maximum of in and the lowest idx attaining it. */
 int bafq_debug_lane_ops(const double* in, const int32_t* idx, double* out);
 
+/* ---------------------------------------------------------------------
+ * Learned global proposals: a periodic 3D convolutional network over HS
+ * fields, trained and evaluated on the GPU (no counterpart in the reference).
+ * Fields go in and out as float32 [batch][channels][lt][ly][lx]; parameters
+ * are the concatenation over layers of W[cout][cin][3][3][3] and b[cout];
+ * hidden layers use ReLU, the last layer is linear.  The loss is the mean
+ * squared error against a target of the output's shape.
+ * --------------------------------------------------------------------- */
+typedef struct bafq_net bafq_net;
+typedef struct bafq_net_desc {
+    int32_t lt, ly, lx;       /* lattice: time slices, rows, columns (periodic) */
+    int32_t channels;         /* field channels in and out */
+    int32_t hidden;           /* channels of the hidden layers */
+    int32_t layers;           /* convolution layers, >= 2 */
+} bafq_net_desc;
+
+int bafq_net_create(const bafq_net_desc* d, int max_batch, bafq_net** out);
+void bafq_net_destroy(bafq_net* h);
+const char* bafq_net_last_error(const bafq_net* h);
+int64_t bafq_net_param_count(const bafq_net* h);
+/* set (resets the Adam state) / get the parameters */
+int bafq_net_set_params(bafq_net* h, const float* params);
+int bafq_net_get_params(bafq_net* h, float* params);
+/* y = f(x) */
+int bafq_net_forward(bafq_net* h, int batch, const float* x, float* y);
+/* loss = |f(x) - target|^2 / size and its gradient (grad may be NULL) */
+int bafq_net_gradient(bafq_net* h, int batch, const float* x, const float* target, double* loss, float* grad);
+/* `steps` Adam steps (learning rate lr) on one batch; loss of the last step */
+int bafq_net_train(bafq_net* h, int batch, const float* x, const float* target, int steps, float lr,
+                   double* loss);
+
 #ifdef __cplusplus
 }
 #endif
