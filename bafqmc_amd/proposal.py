@@ -128,16 +128,44 @@ def from_net(y, params):
     return [b[i].reshape(params.ltrot, params.nly * params.nlx, 2) for i in range(b.shape[0])]
 
 
+def active_channels(params):
+    """Channels whose HS field couples (U1 and U2 nonzero); the others are left unmoved."""
+    return np.array([abs(params.u1) > 1e-10, abs(params.u2) > 1e-10], dtype=np.float64)
+
+
+def train_on_sweeps(net, x, t, epochs, lr, rng, holdout=0.25, batch=None):
+    """Fit the network to the one-sweep displacement t of fields x (both in the
+    network layout) by minibatch Adam; returns the per-channel residual
+    variance on the held-out part and the held-out R^2 (variance explained)."""
+    batch = batch or net.max_batch
+    n = len(x)
+    ntr = n - max(batch, int(n * holdout))
+    for ep in range(epochs):
+        idx = rng.permutation(ntr)
+        for b in range(0, ntr - batch + 1, batch):
+            j = idx[b:b + batch]
+            net.train(x[j], t[j], 1, lr if ep < (3 * epochs) // 4 else lr / 3)
+    res = []
+    for i in range(ntr, n, batch):
+        res.append(net.forward(x[i:i + batch]) - t[i:i + batch])
+    r = np.concatenate(res).astype(np.float64)
+    var_c = r.var(axis=(0, 2, 3, 4))
+    r2 = 1.0 - float(np.mean(r * r)) / float(t[ntr:].astype(np.float64).var())
+    return var_c, r2
+
+
 class LearnedProposal:
     """phi' = phi + lam f(phi) + sqrt(lam) sigma xi, f the network (0 when net is
-    None).  With f the mean one-sweep displacement and sigma^2 its residual
-    variance, lam = 1 imitates one local sweep and lam > 1 extrapolates it
-    (a Langevin step of lam times the sweep's diffusion)."""
+    None), sigma per channel.  With f the mean one-sweep displacement and
+    sigma^2 its residual variance, lam = 1 imitates one local sweep and
+    lam > 1 extrapolates it (a Langevin step of lam times the sweep's
+    diffusion).  Channels whose field does not couple are not moved."""
 
     def __init__(self, net, params, sigma, lam=1.0):
         self.net, self.params = net, params
         self.lam = float(lam)
-        self.sigma = float(sigma) * np.sqrt(self.lam)
+        self.mask = active_channels(params)
+        self.sigma = np.broadcast_to(np.asarray(sigma, dtype=np.float64), (2,)) * np.sqrt(self.lam) * self.mask
 
     def drift(self, phis):
         if self.net is None:
@@ -146,14 +174,15 @@ class LearnedProposal:
         for i in range(0, len(phis), self.net.max_batch):
             part = phis[i:i + self.net.max_batch]
             out += from_net(self.net.forward(to_net(part, self.params)), self.params)
-        return [self.lam * d for d in out]
+        return [self.lam * d * self.mask for d in out]
 
     def propose(self, phis, rng):
         """(proposals, log q(phi|phi') - log q(phi'|phi)) for every chain."""
         f0 = self.drift(phis)
         props = [p + d + self.sigma * rng.normal(size=p.shape) for p, d in zip(phis, f0)]
         f1 = self.drift(props)
-        s2 = 2.0 * self.sigma ** 2
-        lq = np.array([(-np.sum((p - q - d1) ** 2) + np.sum((q - p - d0) ** 2)) / s2
+        on = self.sigma > 0
+        w = np.where(on, 1.0 / (2.0 * np.where(on, self.sigma, 1.0) ** 2), 0.0)
+        lq = np.array([float(np.sum(w * (-(p - q - d1) ** 2 + (q - p - d0) ** 2)))
                        for p, q, d0, d1 in zip(phis, props, f0, f1)])
         return props, lq
