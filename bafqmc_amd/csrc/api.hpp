@@ -2,6 +2,7 @@
 // CPU emulator build.  A back end provides the rt_* functions below.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -24,7 +25,21 @@ void rt_free(void* p);
 int rt_h2d(void* dst, const void* src, size_t bytes);
 int rt_d2h(void* dst, const void* src, size_t bytes);
 int rt_memset(void* dst, int v, size_t bytes);
-int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms);
+// One persistent launch per path: chains a.ids[0..a.count) taken from the
+// queue a.queue (zeroed by the back end before the kernel) by `grid`
+// workgroups.  rt_launch_paths launches the paths of a batch together and
+// keeps back-end state (HIP graphs) in *cache, released by rt_release_launches.
+struct PathLaunch {
+    PathId path;
+    LaunchArgs a;
+    int grid, block;
+    size_t smem;
+};
+int rt_launch_paths(void** cache, KernelId k, const PathLaunch* pl, int n, double* ms);
+void rt_release_launches(void* cache);
+int rt_launch_warmup(const LaunchArgs& a, int grid, int block, double* ms);
+// workgroups of a kernel that fit on the device at once (<= 0 on error)
+int rt_resident_groups(KernelId k, PathId path, int block, size_t smem);
 double rt_clock_hz();          // rate of Ctx::clock()
 int rt_sync();
 const char* rt_error();
@@ -69,6 +84,10 @@ struct bafq_handle {
     size_t max_lds[2] = {0, 0};
     int* d_path_ids[2] = {nullptr, nullptr};
     int path_count[2] = {0, 0};
+    int* d_queue = nullptr;                   // one work counter per path
+    int grid[2][2] = {{0, 0}, {0, 0}};        // persistent grid per [kernel][path]
+    std::vector<bafq_schedule> schedule;      // in effect, per model
+    void* launches = nullptr;                 // back-end launch cache
     double last_ms = 0.0;
 };
 
@@ -124,6 +143,11 @@ extern "C" {
 
 int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chain_model, int n_chains,
                 bafq_handle** out) {
+    return bafq_create_scheduled(models, n_models, chain_model, n_chains, nullptr, out);
+}
+
+int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int32_t* chain_model, int n_chains,
+                          const bafq_schedule* schedule, bafq_handle** out) {
     using namespace bafq;
     *out = nullptr;
     bafq_handle* h = new bafq_handle();
@@ -160,20 +184,45 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
         m.active[0] = std::fabs(d.u1) > kZero;
         m.active[1] = std::fabs(d.u2) > kZero;
         lattice_tables(d.nlx, d.nly, bonds[k], phases[k], m.has_k);
-        // delayed local updates on the LDS path: the widest buffers (<= 16 columns,
-        // a multiple of 4) that fit beside G in LDS.  BAFQ_DELAYED=0 selects
-        // per-site updates, BAFQ_DELAYED=4/8/12 caps the width (tests).
+        // ---- schedule: path and delayed-update width.  The LDS path needs every
+        // MFMA output tile of an N x N matrix covered by the waves and G, the
+        // work matrix and the update buffers within the LDS of a workgroup;
+        // delayed updates use the widest buffers (<= 16 columns, a multiple of
+        // 4) that fit unless the schedule fixes the width.
         {
-            const char* ev = std::getenv("BAFQ_DELAYED");
-            const int cap = ev ? std::atoi(ev) : 16;
+            const bafq_schedule want = schedule ? schedule[k] : bafq_schedule{-1, -1};
+            if (want.path < -1 || want.path > 1 || want.delayed < -1 || want.delayed > 16 ||
+                (want.delayed > 0 && want.delayed % 4)) {
+                *out = h;
+                return fail(h, "invalid schedule (path -1/0/1, delayed -1/0/4/8/12/16)");
+            }
             const int lds_threads = rt_lds_block_threads();
+            const int lds_waves = lds_threads / 64;
+            const int ntile = (m.n + 15) / 16;
+            const size_t limit = rt_lds_limit() - kSlotBytes;
+            const bool lds_fits = m.n <= kLdsMaxN && ntile * ntile <= lds_waves * kMaxTiles &&
+                                  lds_bytes(m.n, lds_waves, true, 0) <= limit;
+            const bool delay_ok = lds_fits && 2 * m.n <= lds_threads;
+            if (want.path == 1 && !lds_fits) { *out = h; return fail(h, "schedule asks for the LDS path but N does not fit"); }
+            const int path = want.path == -1 ? (lds_fits ? 1 : 0) : want.path;
             m.dk = 0;
-            if (m.n <= kLdsMaxN && 2 * m.n <= lds_threads)
-                for (int dk = (cap / 4) * 4 < 16 ? (cap / 4) * 4 : 16; dk >= 4; dk -= 4)
-                    if (lds_bytes(m.n, lds_threads / 64, true, dk) <= rt_lds_limit()) {
-                        m.dk = dk;
-                        break;
+            if (path == 1 && want.delayed != 0) {
+                if (want.delayed > 0) {
+                    if (!delay_ok || lds_bytes(m.n, lds_waves, true, want.delayed) > limit) {
+                        *out = h;
+                        return fail(h, "schedule's delayed-update width does not fit in LDS");
                     }
+                    m.dk = want.delayed;
+                } else if (delay_ok) {
+                    for (int dk = 16; dk >= 4; dk -= 4)
+                        if (lds_bytes(m.n, lds_waves, true, dk) <= limit) {
+                            m.dk = dk;
+                            break;
+                        }
+                }
+            }
+            if (path == 0 && want.delayed > 0) { *out = h; return fail(h, "delayed updates need the LDS path"); }
+            h->schedule.push_back(bafq_schedule{path, m.dk});
         }
     }
     // ---- layout of the device allocation
@@ -212,6 +261,7 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
     const size_t off_pids0 = b.take(sizeof(int) * n_chains);
     const size_t off_pids1 = b.take(sizeof(int) * n_chains);
     const size_t off_seeds = b.take(sizeof(int) * n_chains);
+    const size_t off_queue = b.take(sizeof(int) * 2);
     int max_ns = 0;
     for (int c = 0; c < n_chains; ++c) {
         if (chain_model[c] < 0 || chain_model[c] >= n_models) { *out = h; return fail(h, "chain_model out of range"); }
@@ -266,6 +316,7 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
     h->d_chains = (ChainDev*)(base + off_chains);
     h->d_ids = (int*)(base + off_ids);
     h->d_seeds = (int*)(base + off_seeds);
+    h->d_queue = (int*)(base + off_queue);
     h->d_obs = (double*)(base + off_obs);
     h->d_stats = (double*)(base + off_stats);
     h->d_phi = (double*)(base + off_phi);
@@ -293,26 +344,32 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
         h->chain_n.push_back(m.n);
         h->chain_ns.push_back(m.ns);
         h->chain_lt.push_back(m.ltrot);
-        // BAFQ_FORCE_GLOBAL=1 runs every chain on the global-memory path (tests of that path)
-        const char* fg = std::getenv("BAFQ_FORCE_GLOBAL");
-        const bool force_global = fg && fg[0] == '1';
-        // the LDS path needs every MFMA output tile of an N x N matrix covered by the waves
-        const int lds_waves = rt_lds_block_threads() / 64;
-        const int ntile = (m.n + 15) / 16;
-        const bool lds_ok = m.n <= kLdsMaxN && ntile * ntile <= lds_waves * kMaxTiles &&
-                            lds_bytes(m.n, lds_waves, true, m.dk) <= rt_lds_limit();
-        const int path = (lds_ok && !force_global) ? P_LDS : P_GLOBAL;
+        const int path = h->schedule[chain_model[c]].path;
         pids[path].push_back(c);
-        const size_t lb = path == P_LDS ? lds_bytes(m.n, lds_waves, true, m.dk) : lds_bytes(m.n, nw_glb, false);
-        h->max_lds[path] = std::max(h->max_lds[path], lb);
+        const size_t lb = path == P_LDS ? lds_bytes(m.n, rt_lds_block_threads() / 64, true, m.dk)
+                                        : lds_bytes(m.n, nw_glb, false);
+        h->max_lds[path] = std::max(h->max_lds[path], lb + kSlotBytes);
     }
     rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * n_chains);
     rt_h2d(h->d_ids, ids.data(), sizeof(int) * n_chains);
     h->d_path_ids[P_GLOBAL] = (int*)(base + off_pids0);
     h->d_path_ids[P_LDS] = (int*)(base + off_pids1);
     for (int p = 0; p < 2; ++p) {
+        // longest processing time first: a sweep costs ~ N^3 Ltrot
+        std::stable_sort(pids[p].begin(), pids[p].end(), [&](int x, int y) {
+            const double cx = (double)h->chain_n[x] * h->chain_n[x] * h->chain_n[x] * h->chain_lt[x];
+            const double cy = (double)h->chain_n[y] * h->chain_n[y] * h->chain_n[y] * h->chain_lt[y];
+            return cx > cy;
+        });
         h->path_count[p] = (int)pids[p].size();
         if (!pids[p].empty()) rt_h2d(h->d_path_ids[p], pids[p].data(), sizeof(int) * pids[p].size());
+        for (int k = 0; k < 2; ++k) {
+            if (pids[p].empty()) continue;
+            const int block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
+            const int res = rt_resident_groups((KernelId)k, (PathId)p, block, h->max_lds[p]);
+            if (res <= 0) { *out = h; return fail(h, rt_error()); }
+            h->grid[k][p] = std::min(res, h->path_count[p]);
+        }
     }
     if (rt_sync()) { *out = h; return fail(h, rt_error()); }
     *out = h;
@@ -321,8 +378,15 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
 
 void bafq_destroy(bafq_handle* h) {
     if (!h) return;
+    bafq::rt_release_launches(h->launches);
     if (h->dev) bafq::rt_free(h->dev);
     delete h;
+}
+
+int bafq_get_schedule(const bafq_handle* h, int model, bafq_schedule* out) {
+    if (!h || !out || model < 0 || model >= (int)h->schedule.size()) return 1;
+    *out = h->schedule[model];
+    return 0;
 }
 
 const char* bafq_last_error(const bafq_handle* h) { return h ? h->err.c_str() : "null handle"; }
@@ -363,22 +427,29 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
     a.chains = h->d_chains;
     a.nsweep = nsweep;
     a.measure = measure;
+    a.queue = nullptr;
     double total_ms = 0.0;
     if (k == K_WARMUP) {
         a.ids = h->d_ids;
         a.count = (int)h->chains.size();
         const int block = 64, grid = (a.count + block - 1) / block;
-        if (rt_launch(k, P_GLOBAL, a, grid, block, 0, &total_ms)) return fail(h, rt_error());
+        if (rt_launch_warmup(a, grid, block, &total_ms)) return fail(h, rt_error());
     } else {
+        PathLaunch pl[2];
+        int n = 0;
         for (int p = 0; p < 2; ++p) {
             if (h->path_count[p] == 0) continue;
-            a.ids = h->d_path_ids[p];
-            a.count = h->path_count[p];
-            const int block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
-            double ms = 0.0;
-            if (rt_launch(k, (PathId)p, a, a.count, block, h->max_lds[p], &ms)) return fail(h, rt_error());
-            total_ms += ms;
+            PathLaunch& l = pl[n++];
+            l.path = (PathId)p;
+            l.a = a;
+            l.a.ids = h->d_path_ids[p];
+            l.a.count = h->path_count[p];
+            l.a.queue = h->d_queue + p;
+            l.grid = h->grid[k][p];
+            l.block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
+            l.smem = h->max_lds[p];
         }
+        if (rt_launch_paths(&h->launches, k, pl, n, &total_ms)) return fail(h, rt_error());
     }
     h->last_ms = total_ms;
     if (rt_sync()) return fail(h, rt_error());

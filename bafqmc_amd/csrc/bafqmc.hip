@@ -18,26 +18,30 @@ static int check(hipError_t e, const char* what) {
 
 __global__ void __launch_bounds__(512) k_prepare(LaunchArgs a) {
     extern __shared__ __align__(16) char smem[];
-    Ctx c{smem};
-    body_prepare<false>(c, a);
+    Ctx c{smem + kSlotBytes};
+    body_prepare<false>(c, a, (int*)smem);
 }
 
 __global__ void __launch_bounds__(512) k_sweep(LaunchArgs a) {
     extern __shared__ __align__(16) char smem[];
-    Ctx c{smem};
-    body_sweep<false>(c, a);
+    Ctx c{smem + kSlotBytes};
+    body_sweep<false>(c, a, (int*)smem);
 }
 
 __global__ void __launch_bounds__(512) k_prepare_lds(LaunchArgs a) {
     extern __shared__ __align__(16) char smem[];
-    Ctx c{smem};
-    body_prepare<true>(c, a);
+    Ctx c{smem + kSlotBytes};
+    body_prepare<true>(c, a, (int*)smem);
 }
 
 __global__ void __launch_bounds__(512) k_sweep_lds(LaunchArgs a) {
     extern __shared__ __align__(16) char smem[];
-    Ctx c{smem};
-    body_sweep<true>(c, a);
+    Ctx c{smem + kSlotBytes};
+    body_sweep<true>(c, a, (int*)smem);
+}
+
+__global__ void __launch_bounds__(64) k_reset_queue(int* q) {
+    if (threadIdx.x == 0) *q = 0;
 }
 
 __global__ void __launch_bounds__(64) k_warmup(LaunchArgs a) {
@@ -247,35 +251,131 @@ size_t rt_lds_limit() {
     return (size_t)v;
 }
 
-int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
-    static hipEvent_t ev[2] = {nullptr, nullptr};
-    if (!ev[0]) {
-        if (check(hipEventCreate(&ev[0]), "hipEventCreate") || check(hipEventCreate(&ev[1]), "hipEventCreate"))
+static hipEvent_t g_ev[2] = {nullptr, nullptr};
+
+static int timed_begin() {
+    if (!g_ev[0]) {
+        if (check(hipEventCreate(&g_ev[0]), "hipEventCreate") || check(hipEventCreate(&g_ev[1]), "hipEventCreate"))
             return 1;
     }
-    const void* fn = nullptr;
-    if (k == K_PREPARE) fn = path == P_LDS ? (const void*)k_prepare_lds : (const void*)k_prepare;
-    if (k == K_SWEEP) fn = path == P_LDS ? (const void*)k_sweep_lds : (const void*)k_sweep;
-    if (fn && smem > 65536 &&
-        check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem), "hipFuncSetAttribute"))
-        return 1;
-    if (check(hipEventRecord(ev[0], 0), "hipEventRecord")) return 1;
-    if (k == K_WARMUP) {
-        hipLaunchKernelGGL(k_warmup, dim3(grid), dim3(block), 0, 0, a);
-    } else if (k == K_PREPARE) {
-        if (path == P_LDS) hipLaunchKernelGGL(k_prepare_lds, dim3(grid), dim3(block), smem, 0, a);
-        else hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(block), smem, 0, a);
-    } else {
-        if (path == P_LDS) hipLaunchKernelGGL(k_sweep_lds, dim3(grid), dim3(block), smem, 0, a);
-        else hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(block), smem, 0, a);
-    }
-    if (check(hipGetLastError(), "kernel launch")) return 1;
-    if (check(hipEventRecord(ev[1], 0), "hipEventRecord") || check(hipEventSynchronize(ev[1]), "hipEventSynchronize"))
+    return check(hipEventRecord(g_ev[0], 0), "hipEventRecord");
+}
+
+static int timed_end(double* ms) {
+    if (check(hipEventRecord(g_ev[1], 0), "hipEventRecord") || check(hipEventSynchronize(g_ev[1]), "hipEventSynchronize"))
         return 1;
     float f = 0.0f;
-    if (check(hipEventElapsedTime(&f, ev[0], ev[1]), "hipEventElapsedTime")) return 1;
+    if (check(hipEventElapsedTime(&f, g_ev[0], g_ev[1]), "hipEventElapsedTime")) return 1;
     if (ms) *ms = (double)f;
     return 0;
+}
+
+static const void* kernel_of(KernelId k, PathId path) {
+    if (k == K_PREPARE) return path == P_LDS ? (const void*)k_prepare_lds : (const void*)k_prepare;
+    return path == P_LDS ? (const void*)k_sweep_lds : (const void*)k_sweep;
+}
+
+int rt_launch_warmup(const LaunchArgs& a, int grid, int block, double* ms) {
+    if (timed_begin()) return 1;
+    hipLaunchKernelGGL(k_warmup, dim3(grid), dim3(block), 0, 0, a);
+    if (check(hipGetLastError(), "kernel launch")) return 1;
+    return timed_end(ms);
+}
+
+int rt_resident_groups(KernelId k, PathId path, int block, size_t smem) {
+    const void* fn = kernel_of(k, path);
+    if (smem > 65536 &&
+        check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem), "hipFuncSetAttribute"))
+        return -1;
+    int per_cu = 0, cus = 0;
+    if (check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, smem), "hipOccupancy") ||
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute"))
+        return -1;
+    return per_cu > 0 && cus > 0 ? per_cu * cus : 1;
+}
+
+// Instantiated graphs of one handle, keyed by (kernel, sweeps, measure): each
+// holds, per path, a queue reset followed by the persistent kernel; the paths
+// are independent branches and run concurrently.
+struct GraphCache {
+    struct Entry {
+        int kind, nsweep, measure;
+        std::vector<LaunchArgs> args;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Entry> entries;
+    void clear() {
+        for (Entry& e : entries) {
+            if (e.exec) (void)hipGraphExecDestroy(e.exec);
+            if (e.graph) (void)hipGraphDestroy(e.graph);
+        }
+        entries.clear();
+    }
+};
+
+void rt_release_launches(void* cache) {
+    if (!cache) return;
+    GraphCache* gc = (GraphCache*)cache;
+    gc->clear();
+    delete gc;
+}
+
+static int build_graph(GraphCache::Entry& e, KernelId k, const PathLaunch* pl, int n) {
+    if (check(hipGraphCreate(&e.graph, 0), "hipGraphCreate")) return 1;
+    e.args.resize(n);
+    for (int i = 0; i < n; ++i) {
+        e.args[i] = pl[i].a;
+        const void* fn = kernel_of(k, pl[i].path);
+        if (pl[i].smem > 65536 &&
+            check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl[i].smem),
+                  "hipFuncSetAttribute"))
+            return 1;
+        hipGraphNode_t reset, kern;
+        void* rargs[] = {(void*)&e.args[i].queue};
+        hipKernelNodeParams rp{};
+        rp.func = (void*)k_reset_queue;
+        rp.gridDim = dim3(1);
+        rp.blockDim = dim3(64);
+        rp.sharedMemBytes = 0;
+        rp.kernelParams = rargs;
+        rp.extra = nullptr;
+        if (check(hipGraphAddKernelNode(&reset, e.graph, nullptr, 0, &rp), "hipGraphAddKernelNode")) return 1;
+        void* kargs[] = {(void*)&e.args[i]};
+        hipKernelNodeParams kp{};
+        kp.func = (void*)fn;
+        kp.gridDim = dim3(pl[i].grid);
+        kp.blockDim = dim3(pl[i].block);
+        kp.sharedMemBytes = (unsigned)pl[i].smem;
+        kp.kernelParams = kargs;
+        kp.extra = nullptr;
+        if (check(hipGraphAddKernelNode(&kern, e.graph, &reset, 1, &kp), "hipGraphAddKernelNode")) return 1;
+    }
+    return check(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+int rt_launch_paths(void** cache, KernelId k, const PathLaunch* pl, int n, double* ms) {
+    if (!*cache) *cache = new GraphCache();
+    GraphCache* gc = (GraphCache*)*cache;
+    const int nsweep = n ? pl[0].a.nsweep : 0, measure = n ? pl[0].a.measure : 0;
+    GraphCache::Entry* hit = nullptr;
+    for (GraphCache::Entry& e : gc->entries)
+        if (e.kind == (int)k && e.nsweep == nsweep && e.measure == measure) hit = &e;
+    if (!hit) {
+        if (gc->entries.size() >= 8) gc->clear();
+        gc->entries.emplace_back();
+        hit = &gc->entries.back();
+        hit->kind = (int)k;
+        hit->nsweep = nsweep;
+        hit->measure = measure;
+        if (build_graph(*hit, k, pl, n)) {
+            gc->clear();
+            return 1;
+        }
+    }
+    if (timed_begin()) return 1;
+    if (check(hipGraphLaunch(hit->exec, 0), "hipGraphLaunch")) return 1;
+    return timed_end(ms);
 }
 
 int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c) {

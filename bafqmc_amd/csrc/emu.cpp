@@ -68,11 +68,23 @@ struct Sched {
     bool reverse = false;                  // run threads in descending order between barriers
     const std::function<void(EmuCtx&)>* body = nullptr;
     EmuGroup* grp = nullptr;
+    std::vector<char> done;                // thread has returned from the body
+    bool mismatch = false;
 };
+
+// Set when the threads of a workgroup passed different numbers of barriers
+// (a thread waited at a barrier that a finished thread never reached): on the
+// GPU that is undefined behaviour; here the launch fails with an error.
+std::atomic<bool> g_barrier_mismatch{false};
 
 thread_local Sched* tl_sched = nullptr;
 
 void switch_to(Sched* s, int from, int to) {
+    if (s->done[to]) {
+        // abandon the workgroup: back to the launcher, never resumed
+        s->mismatch = true;
+        bafq_emu_switch(&s->sp[from], s->main_sp);
+    }
     s->cur = to;
     bafq_emu_switch(&s->sp[from], s->sp[to]);
 }
@@ -98,11 +110,11 @@ void wave_barrier(EmuGroup* g, int t) {
     const int t = s->cur;
     EmuCtx c{s->grp, t, s->grp->smem};
     (*s->body)(c);
+    s->done[t] = 1;
     // finished: the next thread resumes; the last one returns to the launcher
     const int next = s->reverse ? t - 1 : t + 1;
     if (next >= 0 && next < s->nthreads) {
-        s->cur = next;
-        bafq_emu_switch(&s->sp[t], s->sp[next]);
+        switch_to(s, t, next);
     } else {
         bafq_emu_switch(&s->sp[t], s->main_sp);
     }
@@ -140,6 +152,7 @@ void run_one_group(int g, int grid, int block, size_t smem, const std::function<
     s.body = &body;
     s.grp = &grp;
     s.sp.resize(block);
+    s.done.assign(block, 0);
     while ((int)stacks.size() < block) stacks.emplace_back(new char[kFiberStack]);
     for (int t = 0; t < block; ++t) s.sp[t] = make_context(stacks[t].get(), kFiberStack);
     grp.sched = &s;
@@ -149,6 +162,9 @@ void run_one_group(int g, int grid, int block, size_t smem, const std::function<
     s.cur = s.reverse ? block - 1 : 0;
     bafq_emu_switch(&s.main_sp, s.sp[s.cur]);
     tl_sched = nullptr;
+    for (int t = 0; t < block && !s.mismatch; ++t)
+        if (!s.done[t]) s.mismatch = true;
+    if (s.mismatch) g_barrier_mismatch = true;
 }
 
 }  // namespace
@@ -204,34 +220,53 @@ int rt_lds_block_threads() {
 
 size_t rt_lds_limit() { return 160 * 1024; }
 
-int rt_launch(KernelId k, PathId path, const LaunchArgs& a, int grid, int block, size_t smem, double* ms) {
+int rt_launch_warmup(const LaunchArgs& a, int grid, int block, double* ms) {
     if (ms) *ms = 0.0;
-    if (k == K_WARMUP) {
-        // one thread per chain; no workgroup cooperation
-        for (int g = 0; g < grid; ++g)
-            for (int t = 0; t < block; ++t) {
-                EmuGroup grp;
-                grp.nthreads = block;
-                grp.block = g;
-                grp.nblocks = grid;
-                grp.smem = nullptr;
-                grp.group_barrier = nullptr;
-                grp.wave_barrier = nullptr;
-                grp.sched = nullptr;
-                EmuCtx c{&grp, t, nullptr};
-                body_warmup(c, a);
-            }
-        return 0;
-    }
-    run_groups(grid, block, smem, [&](EmuCtx& c) {
-        if (k == K_PREPARE) {
-            if (path == P_LDS) body_prepare<true>(c, a);
-            else body_prepare<false>(c, a);
-        } else {
-            if (path == P_LDS) body_sweep<true>(c, a);
-            else body_sweep<false>(c, a);
+    // one thread per chain; no workgroup cooperation
+    for (int g = 0; g < grid; ++g)
+        for (int t = 0; t < block; ++t) {
+            EmuGroup grp;
+            grp.nthreads = block;
+            grp.block = g;
+            grp.nblocks = grid;
+            grp.smem = nullptr;
+            grp.group_barrier = nullptr;
+            grp.wave_barrier = nullptr;
+            grp.sched = nullptr;
+            EmuCtx c{&grp, t, nullptr};
+            body_warmup(c, a);
         }
-    });
+    return 0;
+}
+
+// A small persistent grid, so that every workgroup takes several chains from
+// the queue as on the GPU.
+int rt_resident_groups(KernelId, PathId, int, size_t) { return 3; }
+
+void rt_release_launches(void*) {}
+
+int rt_launch_paths(void**, KernelId k, const PathLaunch* pl, int n, double* ms) {
+    if (ms) *ms = 0.0;
+    g_barrier_mismatch = false;
+    for (int i = 0; i < n; ++i) {
+        const PathLaunch& l = pl[i];
+        *l.a.queue = 0;
+        run_groups(l.grid, l.block, l.smem, [&](EmuCtx& c0) {
+            EmuCtx c{c0.g, c0.t, c0.smem + kSlotBytes};
+            int* slot = (int*)c0.smem;
+            if (k == K_PREPARE) {
+                if (l.path == P_LDS) body_prepare<true>(c, l.a, slot);
+                else body_prepare<false>(c, l.a, slot);
+            } else {
+                if (l.path == P_LDS) body_sweep<true>(c, l.a, slot);
+                else body_sweep<false>(c, l.a, slot);
+            }
+        });
+    }
+    if (g_barrier_mismatch) {
+        g_err = "emulator: the threads of a workgroup passed different numbers of barriers";
+        return 1;
+    }
     return 0;
 }
 

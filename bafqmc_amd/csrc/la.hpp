@@ -254,7 +254,8 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
             for (int jb = k; jb < n; jb += 64) {
                 const int j = jb + c.lane();
                 if (j < n) {
-                    const double v = s.nrm[j];
+                    // a NaN norm (a chain that has blown up) wins, so the pivot is always a column
+                    const double v = s.nrm[j] == s.nrm[j] ? s.nrm[j] : INFINITY;
                     if (v > best) { best = v; bi = j; }
                 }
             }
@@ -536,7 +537,8 @@ DEV double col_candidate(const Ctx& c, const cd (&x)[R], int n, int k, int gl, d
 // Largest pivoting norm among the wavefront's active columns (ties to the
 // lower column), published in rec[2 w], rec[2 w + 1] for the next pivot search.
 DEV void publish_best(const Ctx& c, LDS_AS double* rec, bool active, double a2, int col, int n) {
-    double bv = active ? a2 : -1.0;
+    // a NaN norm (a chain that has blown up) wins, so every step has an owner
+    double bv = active ? (a2 == a2 ? a2 : INFINITY) : -1.0;
     int bi = active ? col : n;
     c.wave_argmax(bv, bi);
     if (c.lane() == 0) {
@@ -576,6 +578,7 @@ DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, doub
     cd t, scale;
     double a2 = col_candidate<R>(c, x, n, 0, gl, beta, t, scale);
     publish_best(c, rec, active, a2, col, n);
+    if (c.tid() < n) perm[c.tid()] = c.tid();   // defined even if a step had no owner
     c.sync();                                    // columns in registers: A may be overwritten
 #pragma unroll 1
     for (int k = 0; k < n; ++k) {

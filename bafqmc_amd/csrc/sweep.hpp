@@ -86,7 +86,12 @@ struct LaunchArgs {
     int count;
     int nsweep;
     int measure;
+    int* queue;          // work counter of the launch (zeroed before it), see next_item
 };
+
+// Bytes at the start of a workgroup's dynamic LDS that hold the work-queue
+// slot; the per-chain LDS layout (carve) starts after them.
+constexpr size_t kSlotBytes = 16;
 
 // ---------------------------------------------------------------- RNG
 constexpr int64_t kMod = 2147483648LL;
@@ -591,7 +596,12 @@ DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch
             cols = 0;
         }
     }
-    if (cols > 0) delayed_flush(c, G_, L.du, L.dw, n, cols);
+    if (cols > 0) {
+        // wavefronts still deciding the trailing rejected sites read G: they
+        // must be done before any wavefront's flush writes its tiles
+        c.sync();
+        delayed_flush(c, G_, L.du, L.dw, n, cols);
+    }
     for (int i = tid; i < ns; i += c.nthreads()) {
         auto pp = props + i;
         if (pp->pad != 0.0) phi_t[i * 2 + nf - 1] = pp->phi_new;
@@ -1334,9 +1344,21 @@ DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 }
 
 // ---------------------------------------------------------------- kernel bodies
+// Persistent workgroups take chains from a queue: lane 0 of the workgroup
+// increments the launch's counter and shares the index through LDS.  The
+// chain list is ordered by decreasing cost, so the first free workgroup always
+// takes the most expensive remaining chain (longest-processing-time order).
+DEV int next_item(const Ctx& c, int* queue, int* slot) {
+    if (c.tid() == 0) *slot = c.atomic_add(queue, 1);
+    c.sync();
+    const int q = *slot;
+    c.sync();
+    return q;
+}
+
 template <bool kLds>
-DEV void body_prepare(const Ctx& c, const LaunchArgs& a) {
-    for (int q = c.block(); q < a.count; q += c.nblocks()) {
+DEV void body_prepare(const Ctx& c, const LaunchArgs& a, int* slot) {
+    for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
         const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, kLds ? m.dk : 0);
@@ -1346,8 +1368,8 @@ DEV void body_prepare(const Ctx& c, const LaunchArgs& a) {
 }
 
 template <bool kLds>
-DEV void body_sweep(const Ctx& c, const LaunchArgs& a) {
-    for (int q = c.block(); q < a.count; q += c.nblocks()) {
+DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
+    for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
         const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, kLds ? m.dk : 0);

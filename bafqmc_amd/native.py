@@ -28,6 +28,12 @@ class ModelDesc(C.Structure):
     ]
 
 
+class Schedule(C.Structure):
+    """bafq_schedule: path -1 choose / 0 device memory / 1 LDS; delayed -1
+    widest / 0 per-site / 4, 8, 12, 16."""
+    _fields_ = [("path", C.c_int32), ("delayed", C.c_int32)]
+
+
 class NetDesc(C.Structure):
     _fields_ = [("lt", C.c_int32), ("ly", C.c_int32), ("lx", C.c_int32),
                 ("channels", C.c_int32), ("hidden", C.c_int32), ("layers", C.c_int32)]
@@ -37,6 +43,9 @@ c_float_p = C.POINTER(C.c_float)
 
 SYMBOLS = {
     "bafq_create": (C.c_int, [C.POINTER(ModelDesc), C.c_int, c_int32_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "bafq_create_scheduled": (C.c_int, [C.POINTER(ModelDesc), C.c_int, c_int32_p, C.c_int, C.POINTER(Schedule),
+                                        C.POINTER(C.c_void_p)]),
+    "bafq_get_schedule": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Schedule)]),
     "bafq_destroy": (None, [C.c_void_p]),
     "bafq_last_error": (C.c_char_p, [C.c_void_p]),
     "bafq_dims": (C.c_int, [C.c_void_p, C.c_int, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),

@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from .model import ModelSpec, NAUX
-from .native import ModelDesc, NativeLibrary, load_library, dptr, iptr
+from .native import ModelDesc, NativeLibrary, Schedule, load_library, dptr, iptr
 
 # observable slots of the per-chain accumulator block (csrc/sweep.hpp ObsIndex)
 OBS_SLOTS = {
@@ -48,7 +48,16 @@ class SolverError(RuntimeError):
 
 
 class BatchedSolver:
-    def __init__(self, models, chain_model, lib: NativeLibrary | None = None):
+    """A batch of Markov chains (one per entry of chain_model) on one GPU.
+
+    schedule: None (the library chooses), one (path, delayed) pair for every
+    model, or a list of pairs per model -- see bafq_schedule in
+    include/bafqmc.h; bafqmc_amd.autotune chooses one by timing.
+    default_schedule applies when schedule is None."""
+
+    default_schedule = None
+
+    def __init__(self, models, chain_model, lib: NativeLibrary | None = None, schedule=None):
         self.lib = lib or load_library()
         self.models = [m if isinstance(m, ModelSpec) else ModelSpec(m) for m in models]
         self.chain_model = np.ascontiguousarray(chain_model, dtype=np.int32)
@@ -65,8 +74,17 @@ class BatchedSolver:
             self._keep += [m.expk_p, m.expk_m]
             descs[k].expk_p = dptr(m.expk_p)
             descs[k].expk_m = dptr(m.expk_m)
+        schedule = self.default_schedule if schedule is None else schedule
+        sched = None
+        if schedule is not None:
+            if len(schedule) == 2 and all(isinstance(v, (int, np.integer)) for v in schedule):
+                schedule = [tuple(schedule)] * len(self.models)
+            sched = (Schedule * len(self.models))()
+            for k, (path, delayed) in enumerate(schedule):
+                sched[k].path, sched[k].delayed = int(path), int(delayed)
         h = C.c_void_p()
-        rc = self.lib.bafq_create(descs, len(self.models), iptr(self.chain_model), self.nchains, C.byref(h))
+        rc = self.lib.bafq_create_scheduled(descs, len(self.models), iptr(self.chain_model), self.nchains,
+                                            sched, C.byref(h))
         self.h = h
         if rc:
             msg = self.lib.bafq_last_error(h).decode() if h else "bafq_create failed"
@@ -84,6 +102,15 @@ class BatchedSolver:
         self.obs_stride = os_.value
         self.stats_stride = ss.value
         self.field_offsets = np.concatenate([[0], np.cumsum(self.nfields)])
+
+    def schedule(self):
+        """(path, delayed) in effect per model."""
+        out = []
+        sc = Schedule()
+        for k in range(len(self.models)):
+            self._check(self.lib.bafq_get_schedule(self.h, k, C.byref(sc)), "bafq_get_schedule")
+            out.append((sc.path, sc.delayed))
+        return out
 
     # ------------------------------------------------------------------ utils
     def _check(self, rc, what):

@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--schedule", default="auto",
+                    help="'auto' (autotuner, before the timed region), 'default' (library choice) or 'PATH,DELAYED'")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -150,7 +152,18 @@ def main():
     p = make_params(cfg["p"])
     spec = ModelSpec(p)
     phis, seeds = conf_set_batch(p, chain_seeds(13579113, nchains))
-    s = BatchedSolver([spec], [0] * nchains)
+    tuned = None
+    if args.schedule == "auto":
+        from bafqmc_amd import autotune
+        from bafqmc_amd.native import load_library
+        tuned = autotune.tune_model(load_library(),
+                                    p, chains=nchains, sweeps=2)
+        schedule = [tuned.schedule]
+    elif args.schedule == "default":
+        schedule = None
+    else:
+        schedule = [tuple(int(v) for v in args.schedule.split(","))]
+    s = BatchedSolver([spec], [0] * nchains, schedule=schedule)
     s.set_fields(list(phis), seeds)
     s.prepare()
     for _ in range(args.warmup):
@@ -194,7 +207,10 @@ def main():
         "config": {"workload": cfg["name"], "chains": nchains, "matrix_n": p.n, "ltrot": p.ltrot,
                    "nwrap": p.nwrap, "sweeps_per_s": sweeps_per_s, "tau_int_sf_k": tau,
                    "unstable_chains": nchains - n_stable,
-                   "tau_window": window, "kernel": "k_sweep_lds" if p.n <= 96 else "k_sweep",
+                   "tau_window": window,
+                   "kernel": "k_sweep_lds" if s.schedule()[0][0] == 1 else "k_sweep",
+                   "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1],
+                                "autotune_ms_per_sweep": tuned.table if tuned else None},
                    "phase_ms_per_chain_sweep": {k: 1e3 * v / (nchains * args.steps) for k, v in
                                                 zip(("site", "kinetic", "measure", "wrap", "hs", "wrap_products",
                                                      "wrap_stab_u", "wrap_green"), phase)}},

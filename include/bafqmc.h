@@ -53,6 +53,26 @@ int bafq_create(const bafq_model_desc* models, int n_models, const int32_t* chai
 void bafq_destroy(bafq_handle* h);
 const char* bafq_last_error(const bafq_handle* h);
 
+/* Execution schedule of the chains of one model (no counterpart in the
+ * reference, which runs one chain per MPI rank with LAPACK/BLAS calls):
+ * path  -1 choose, 0 Green's function and work matrices in device memory,
+ *       1 G and the stabilisation work matrix resident in LDS (N <= 96);
+ * delayed  -1 widest that fits, 0 per-site rank-1/rank-2 updates, or 4, 8,
+ *       12, 16 accepted updates buffered before one MFMA flush (LDS path).
+ * Every schedule computes the same Markov chain (identical up to rounding);
+ * bafqmc_amd.autotune picks one by timing.  bafq_create uses {-1, -1}. */
+typedef struct bafq_schedule {
+    int32_t path;
+    int32_t delayed;
+} bafq_schedule;
+
+/* bafq_create with a schedule per model (schedule may be NULL).  A schedule
+ * that cannot run (LDS path for a model that does not fit) is an error. */
+int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int32_t* chain_model,
+                          int n_chains, const bafq_schedule* schedule, bafq_handle** out);
+/* The schedule in effect for a model. */
+int bafq_get_schedule(const bafq_handle* h, int model, bafq_schedule* out);
+
 /* Sizes: matrix dimension N of a chain, doubles per chain in bafq_read_bin's
  * obs block, doubles per chain in its stats block, fields per chain. */
 int bafq_dims(const bafq_handle* h, int chain, int32_t* n, int32_t* obs_stride,
@@ -71,7 +91,10 @@ int bafq_warmup(bafq_handle* h, int n);
 int bafq_prepare(bafq_handle* h);
 
 /* n_sweeps x (sweep_L; sweep_R) = Local_sweep with Nsweep = n_sweeps
- * (local_sweep.f90:136-161); measures every slice when measure != 0. */
+ * (local_sweep.f90:136-161); measures every slice when measure != 0.
+ * Chains are taken from a work queue in decreasing order of cost by a
+ * persistent grid (one launch per path; the launches of a (n_sweeps, measure)
+ * pair are captured once in a HIP graph and replayed). */
 int bafq_sweep(bafq_handle* h, int n_sweeps, int measure);
 
 /* Per-chain observable sums, sample counts and counters accumulated since the

@@ -48,7 +48,7 @@ def test_emu_sweep_matches_oracle(emu_lib, case):
 
 @pytest.mark.parametrize("case", [1, 3, 5])
 def test_emu_global_path_matches_oracle(emu_lib, case, monkeypatch):
-    monkeypatch.setenv("BAFQ_FORCE_GLOBAL", "1")
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (0, -1))
     check_sweep_vs_oracle(emu_lib, case)
 
 
@@ -57,15 +57,16 @@ def test_emu_matches_reference_fortran(emu_lib, name):
     check_golden(emu_lib, name)
 
 
-@pytest.mark.parametrize("width", ["0", "4", "8"])
+@pytest.mark.parametrize("width", [0, 4, 8])
 @pytest.mark.parametrize("case", [3, 5])
 def test_emu_local_update_variants(emu_lib, case, width, monkeypatch):
-    monkeypatch.setenv("BAFQ_DELAYED", width)
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (1, width))
     check_sweep_vs_oracle(emu_lib, case)
 
 
-def test_emu_uneven_batch(emu_lib):
-    check_uneven_batch(emu_lib)
+@pytest.mark.parametrize("mixed", [False, True])
+def test_emu_uneven_batch(emu_lib, mixed):
+    check_uneven_batch(emu_lib, mixed)
 
 
 @pytest.mark.parametrize("pairing,delta", [(False, 0.0), (True, 0.2)])
@@ -77,11 +78,11 @@ def test_emu_lane_ops(emu_lib):
     check_lane_ops(emu_lib)
 
 
-@pytest.mark.parametrize("case,force_global", [(3, "0"), (5, "0"), (4, "1")])
-def test_emu_thread_order_independence(emu_lib, case, force_global, monkeypatch):
+@pytest.mark.parametrize("case,path", [(3, -1), (5, -1), (4, 0)])
+def test_emu_thread_order_independence(emu_lib, case, path, monkeypatch):
     """Race check: the same sweep with the threads of every workgroup run in
     ascending and in descending order between barriers gives identical bits."""
-    monkeypatch.setenv("BAFQ_FORCE_GLOBAL", force_global)
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (path, -1))
     d = CASES[case]
     ns = d["nlx"] * d["nly"]
     phi = fixed_fields(d["ltrot"], ns)
@@ -97,3 +98,41 @@ def test_emu_thread_order_independence(emu_lib, case, force_global, monkeypatch)
         s.close()
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_emu_wavefronts_agree_on_a_divergent_chain(emu_lib):
+    """Regression: with mu above the band bottom the chain's G grows without
+    bound (drift ~1e6 after one sweep), and the decisions of trailing rejected
+    sites are sensitive to every bit of G.  A wavefront's final delayed-update
+    flush used to overwrite G while other wavefronts were still deciding those
+    sites; their decisions then differed and so did their barrier counts (the
+    emulator reports that as an error)."""
+    from bafqmc_amd.analysis import chain_seeds
+    from bafqmc_amd.model import conf_set_batch
+    d = dict(pairing=True, u1=1.0, u2=-0.5, mu=-3.0, rdelta=0.5, nlx=6, nly=6, ltrot=40, beta=4.0, nwrap=10,
+             shift_loc=1.0)
+    p = solver_params(d)
+    phis, seeds = conf_set_batch(p, chain_seeds(97, 1))
+    s = BatchedSolver([ModelSpec(p)], [0], lib=emu_lib, schedule=[(1, 16)])
+    s.set_fields(list(phis), seeds)
+    s.prepare()
+    s.sweep(1, measure=False)
+    assert s.read_bin()[0]["_stats"]["xmax"] > 1.0       # the chain has blown up
+    s.close()
+
+
+@pytest.mark.parametrize("schedule", [(1, 16), (1, 0), (0, 0)])
+def test_emu_nonfinite_fields(emu_lib, schedule):
+    """Non-finite fields give non-finite results, never an index out of range:
+    a NaN column norm wins the QR pivot search, so every step has a pivot."""
+    d = dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=3, nly=3, ltrot=8, beta=1.0, nwrap=4,
+             shift_loc=1.0)
+    phis = [np.full((8, 9, 2), np.nan), np.random.default_rng(1).normal(size=(8, 9, 2))]
+    phis[1][3, 5, 0] = np.inf
+    s = BatchedSolver([ModelSpec(solver_params(d))], [0, 0], lib=emu_lib, schedule=[schedule])
+    s.set_fields(phis, [11, 12])
+    s.prepare()
+    s.sweep(1)
+    b = s.read_bin()
+    assert not np.isfinite(b[0]["density_total"]) and not np.isfinite(b[1]["density_total"])
+    s.close()

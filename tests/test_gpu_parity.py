@@ -221,41 +221,46 @@ def check_golden(lib, name):
 @pytest.mark.parametrize("name", ["nc_traj_3x3_u2", "pairing_traj_3x3", "pairing_traj_6x6_c2"])
 def test_global_memory_path_matches_reference(hip_lib, name, monkeypatch):
     """The global-memory path (used for N > 96) on the golden cases."""
-    monkeypatch.setenv("BAFQ_FORCE_GLOBAL", "1")
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (0, -1))
     check_golden(hip_lib, name)
 
 
-@pytest.mark.parametrize("width", ["0", "4"])
+@pytest.mark.parametrize("width", [0, 4])
 @pytest.mark.parametrize("name", ["nc_traj_3x3_u2", "pairing_traj_3x3", "pairing_traj_6x6_c2"])
 def test_local_update_variants_match_reference(hip_lib, name, width, monkeypatch):
     """Per-site rank updates (0) and delayed updates flushed every 4 columns (4)."""
-    monkeypatch.setenv("BAFQ_DELAYED", width)
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (1, width))
     check_golden(hip_lib, name)
 
 
-def test_uneven_batch_equals_single_chains(hip_lib):
-    check_uneven_batch(hip_lib)
+@pytest.mark.parametrize("mixed", [False, True])
+def test_uneven_batch_equals_single_chains(hip_lib, mixed):
+    check_uneven_batch(hip_lib, mixed)
 
 
-def check_uneven_batch(hip_lib):
-    """Chains of different lattices/solvers in one launch evolve exactly as alone."""
+def check_uneven_batch(hip_lib, mixed=False):
+    """Chains of different lattices/solvers in one launch evolve exactly as alone
+    (mixed: half of the models on the device-memory path, so both persistent
+    kernels run in one launch, against the single chains on their own paths)."""
     specs = [CASES[1], CASES[3], CASES[4], CASES[5]]
     models = [ModelSpec(solver_params(d)) for d in specs]
-    chain_model = [0, 1, 2, 3, 2, 0]
+    chain_model = [0, 1, 2, 3, 2, 0, 1, 3, 3]
+    sched = [(0, -1), (1, -1), (0, -1), (1, 4)] if mixed else None
     phis = [fixed_fields(specs[m]["ltrot"], specs[m]["nlx"] * specs[m]["nly"]) for m in chain_model]
     seeds = [1111 + 7 * c for c in range(len(chain_model))]
-    s = BatchedSolver(models, chain_model, lib=hip_lib)
+    s = BatchedSolver(models, chain_model, lib=hip_lib, schedule=sched)
     s.set_fields(phis, seeds)
     s.prepare()
     s.sweep(1)
+    s.sweep(1)          # replays the captured launch
     batch = s.read_bin()
     gb = [s.green(c) for c in range(len(chain_model))]
     s.close()
     for c, m in enumerate(chain_model):
-        s1 = BatchedSolver([models[m]], [0], lib=hip_lib)
+        s1 = BatchedSolver([models[m]], [0], lib=hip_lib, schedule=[sched[m]] if sched else None)
         s1.set_fields([phis[c]], [seeds[c]])
         s1.prepare()
-        s1.sweep(1)
+        s1.sweep(2)
         one = s1.read_bin()[0]
         for k in ("density_total", "energy_density", "sf_K", "dw_K"):
             assert batch[c][k] == one[k]
