@@ -116,6 +116,111 @@ def cpu_baseline(cfg, tau, budget_s=12.0):
                           f"reference binary unavailable: {e}"}
 
 
+def _efficiency(series, wall, stable):
+    """Per-chain bin series -> (tau_int, effective samples/s, 1 / (SE^2 wall)).
+    SE is the standard error of the mean over the stable chains' time
+    averages (independent chains), so 1 / (SE^2 wall) compares methods at
+    equal error bars: its ratio is the ratio of the times to reach one."""
+    from bafqmc_amd.analysis import tau_int
+    x = series[stable]
+    tau, _, _ = tau_int(x)
+    steps = x.shape[1]
+    eff = len(x) * (steps / wall) / (2.0 * tau)
+    means = x.mean(axis=1)
+    se = float(np.std(means, ddof=1) / math.sqrt(len(means)))
+    return tau, eff, 1.0 / (se * se * wall), se
+
+
+def run_learned(args, cfg, p, spec, nchains):
+    """Plain sweeps vs sweeps interleaved with learned global moves on the same
+    chains (the second phase continues from the first)."""
+    from bafqmc_amd import global_moves as GM
+    from bafqmc_amd.analysis import chain_seeds
+    from bafqmc_amd.model import conf_set_batch
+    from bafqmc_amd.proposal import ConvNet, LearnedProposal, to_net, train_on_sweeps
+    from bafqmc_amd.solver import BatchedSolver
+    phis, seeds = conf_set_batch(p, chain_seeds(13579113, nchains))
+    s = BatchedSolver([spec], [0] * nchains)
+    s.set_fields(list(phis), seeds)
+    s.prepare()
+    for _ in range(args.warmup):
+        s.sweep(1)
+        s.read_bin()
+    # ---- plain phase (also the training data: fields before/after each sweep)
+    K = args.steps
+    sf0 = np.zeros((nchains, K))
+    xs, ts = [], []
+    t0 = time.perf_counter()
+    for k in range(K):
+        s.sweep(1)
+        obs, st = s.read_bin_raw()
+        sf0[:, k] = obs[:, 20] / obs[:, 32]
+    wall0 = time.perf_counter() - t0
+    ntrain = max(1, min(12, 8192 // nchains + 1))
+    for _ in range(ntrain):
+        f0, _ = s.get_fields()
+        s.sweep(1, measure=False)
+        f1, _ = s.get_fields()
+        xs += f0
+        ts += [b - a for a, b in zip(f0, f1)]
+    s.read_bin()
+    t1 = time.perf_counter()
+    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, 8, 3, max_batch=64, seed=1)
+    var_c, r2 = train_on_sweeps(net, to_net(xs, p), to_net(ts, p), 20, 3e-3, np.random.default_rng(0))
+    t_train = time.perf_counter() - t1
+    sigma = np.sqrt(var_c)
+    rng = np.random.default_rng(2468)
+    # step size: the largest mean squared jump over a few trial moves
+    fields, sd = s.get_fields()
+    trials = {}
+    for lam in (0.1, 0.25, 0.5, 1.0):
+        s.set_fields(fields, sd)
+        s.prepare()
+        prop = LearnedProposal(net, p, sigma, lam)
+        f, _ = s.get_fields()
+        props, lq = prop.propose(f, rng)
+        acc = GM.metropolis_global(s, props, rng, lq)
+        trials[lam] = (float(acc.mean()), sum(float(np.sum((a - b) ** 2)) for a, b, ok in zip(props, f, acc) if ok)
+                       / (nchains * f[0].size))
+    lam = max(trials, key=lambda l: trials[l][1])
+    s.set_fields(fields, sd)
+    s.prepare()
+    prop = LearnedProposal(net, p, sigma, lam)
+    # ---- hybrid phase: one learned global move + one measured sweep per step
+    sf1 = np.zeros((nchains, K))
+    accepted = 0
+    t0 = time.perf_counter()
+    for k in range(K):
+        f, _ = s.get_fields()
+        props, lq = prop.propose(f, rng)
+        accepted += int(GM.metropolis_global(s, props, rng, lq).sum())
+        s.sweep(1)
+        obs, st = s.read_bin_raw()
+        sf1[:, k] = obs[:, 20] / obs[:, 32]
+    wall1 = time.perf_counter() - t0
+    stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf0), axis=1) & np.all(np.isfinite(sf1), axis=1)
+    tau0, eff0, e0, se0 = _efficiency(sf0, wall0, stable)
+    tau1, eff1, e1, se1 = _efficiency(sf1, wall1, stable)
+    out = {
+        "metric": "effective S_SF(K) samples/sec, learned-proposal hybrid vs plain batched sweep, 1 MI355X",
+        "value": eff1, "unit": "effective S_SF(K) samples/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall1 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (HS fields i.i.d. N(0,1) from per-chain seeds)",
+        "config": {"workload": cfg["name"], "chains": nchains, "stable_chains": int(stable.sum()),
+                   "plain": {"ms_per_step": 1000.0 * wall0 / K, "tau_int_sf_k": tau0, "eff_samples_per_s": eff0,
+                             "stderr_sf_k": se0, "inverse_variance_per_s": e0},
+                   "learned": {"ms_per_step": 1000.0 * wall1 / K, "tau_int_sf_k": tau1, "eff_samples_per_s": eff1,
+                               "stderr_sf_k": se1, "inverse_variance_per_s": e1,
+                               "global_acceptance": accepted / (nchains * K), "lambda": lam,
+                               "lambda_trials": {str(k): v for k, v in trials.items()},
+                               "train_s": t_train, "train_samples": len(xs), "heldout_r2": r2},
+                   "gain_eff_samples": eff1 / eff0, "gain_at_equal_error_bars": e1 / e0},
+    }
+    s.close()
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +231,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", default="auto",
                     help="'auto' (autotuner, before the timed region), 'default' (library choice) or 'PATH,DELAYED'")
+    ap.add_argument("--mode", default="plain", choices=["plain", "learned"],
+                    help="plain: the batched local sweep; learned: plain sweeps, then the network trained on "
+                         "their displacements, then sweeps interleaved with learned global moves, compared "
+                         "at equal error bars")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -151,6 +260,9 @@ def main():
     nchains = args.chains or cfg["chains"]
     p = make_params(cfg["p"])
     spec = ModelSpec(p)
+    if args.mode == "learned":
+        run_learned(args, cfg, p, spec, nchains)
+        return
     phis, seeds = conf_set_batch(p, chain_seeds(13579113, nchains))
     tuned = None
     if args.schedule == "auto":
