@@ -198,30 +198,32 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
             }
             const int lds_threads = rt_lds_block_threads();
             const int lds_waves = lds_threads / 64;
+            const int glb_waves = rt_block_threads() / 64;
             const int ntile = (m.n + 15) / 16;
             const size_t limit = rt_lds_limit() - kSlotBytes;
             const bool lds_fits = m.n <= kLdsMaxN && ntile * ntile <= lds_waves * kMaxTiles &&
                                   lds_bytes(m.n, lds_waves, true, 0) <= limit;
-            const bool delay_ok = lds_fits && 2 * m.n <= lds_threads;
             if (want.path == 1 && !lds_fits) { *out = h; return fail(h, "schedule asks for the LDS path but N does not fit"); }
             const int path = want.path == -1 ? (lds_fits ? 1 : 0) : want.path;
+            // U and W of the delayed updates live in LDS on both paths
+            auto fits = [&](int dk) {
+                return path == 1 ? lds_bytes(m.n, lds_waves, true, dk) <= limit
+                                 : lds_bytes(m.n, glb_waves, false, dk) <= limit;
+            };
             m.dk = 0;
-            if (path == 1 && want.delayed != 0) {
-                if (want.delayed > 0) {
-                    if (!delay_ok || lds_bytes(m.n, lds_waves, true, want.delayed) > limit) {
-                        *out = h;
-                        return fail(h, "schedule's delayed-update width does not fit in LDS");
-                    }
-                    m.dk = want.delayed;
-                } else if (delay_ok) {
-                    for (int dk = 16; dk >= 4; dk -= 4)
-                        if (lds_bytes(m.n, lds_waves, true, dk) <= limit) {
-                            m.dk = dk;
-                            break;
-                        }
+            if (want.delayed > 0) {
+                if (!fits(want.delayed)) {
+                    *out = h;
+                    return fail(h, "schedule's delayed-update width does not fit in LDS");
                 }
+                m.dk = want.delayed;
+            } else if (want.delayed == -1) {
+                for (int dk = 16; dk >= 4; dk -= 4)
+                    if (fits(dk)) {
+                        m.dk = dk;
+                        break;
+                    }
             }
-            if (path == 0 && want.delayed > 0) { *out = h; return fail(h, "delayed updates need the LDS path"); }
             h->schedule.push_back(bafq_schedule{path, m.dk});
         }
     }
@@ -347,7 +349,7 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
         const int path = h->schedule[chain_model[c]].path;
         pids[path].push_back(c);
         const size_t lb = path == P_LDS ? lds_bytes(m.n, rt_lds_block_threads() / 64, true, m.dk)
-                                        : lds_bytes(m.n, nw_glb, false);
+                                        : lds_bytes(m.n, nw_glb, false, m.dk);
         h->max_lds[path] = std::max(h->max_lds[path], lb + kSlotBytes);
     }
     rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * n_chains);

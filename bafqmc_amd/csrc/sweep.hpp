@@ -425,9 +425,10 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 // Both factors are stored with the long index contiguous (Ut[j][r] = U[r][j],
 // W[j][c]) so that consecutive lanes touch consecutive LDS addresses.
 
-// G0 -= U[:, :cols] W[:cols, :] on FP64 MFMA 16x16x4 (each output tile owned by one wavefront).
-DEVFN void delayed_flush(const Ctx& c, cd* G_, const cd* Ut_, const cd* W_, int n, int cols) {
-    auto G = as_lds(G_);
+// G0 -= U[:, :cols] W[:cols, :] on FP64 MFMA 16x16x4 (each output tile owned by
+// one wavefront); G in LDS or in device memory, U and W in LDS.
+template <class GP>
+DEV void delayed_flush_t(const Ctx& c, GP G, const cd* Ut_, const cd* W_, int n, int cols) {
     auto Ut = as_lds(Ut_);
     auto W = as_lds(W_);
     const int nt = (n + 15) >> 4;
@@ -472,16 +473,27 @@ DEVFN void delayed_flush(const Ctx& c, cd* G_, const cd* Ut_, const cd* W_, int 
     c.sync();
 }
 
+DEVFN void delayed_flush(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
+    delayed_flush_t(c, as_lds(G), Ut, W, n, cols);
+}
+DEVFN void delayed_flush_glb(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
+    delayed_flush_t(c, as_global(G), Ut, W, n, cols);
+}
+
 // Same decisions, RNG use and result as site_loop_t (LocalU_metro + LocalU_prop_L/R).
-// Requires dk <= 16, n <= 128 and 2 n <= threads.
-DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t,
+// G in LDS (kGlb false) or in device memory; U and W in LDS.  Requires dk <= 16.
+template <bool kGlb>
+DEV void site_loop_delayed_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t,
                              int nf, bool ascending, int& seed) {
     const int n = m.n, ns = m.ns, dk = m.dk;
     const bool pairing = m.kind == 1;
     const int ksec = pairing ? 2 : 1;
     const int tid = c.tid(), lane = c.lane();
     const int wbase = c.nthreads() >= 256 ? 128 : c.nthreads() / 2;
-    auto G = as_lds(G_);
+    auto G = [&] {
+        if constexpr (kGlb) return as_global(G_);
+        else return as_lds(G_);
+    }();
     auto Ut = as_lds(L.du);
     auto W = as_lds(L.dw);
     double* phi_t = ch.phi + (size_t)t * ns * 2;
@@ -548,15 +560,14 @@ DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch
             m11 = (p00 * idet) * P.d1;
         }
         if (tid == 0) (props + i)->pad = 1.0;                // accepted: phi is written after the loop
-        // New factors: threads [0, n) form row r of the new U columns, G[r,S] M with
-        // G[r,S] = G0[r,S] - U[r,:] W[:,S]; threads [wbase, wbase + n) column cb of
-        // the new W rows, I_S - G[S,cb] (wbase = 128: the two halves run on
-        // different SIMDs).  The inner index runs in chunks of four whose loads are
-        // issued together (rows beyond cols are read from inside the buffer and
-        // replaced by zero).
-        if (tid < n || (tid >= wbase && tid < wbase + n)) {
-            const bool urow = tid < n;
-            const int x = urow ? tid : tid - wbase;                // r or cb
+        // New factors: threads [0, wbase) form rows r of the new U columns, G[r,S] M
+        // with G[r,S] = G0[r,S] - U[r,:] W[:,S]; threads [wbase, 2 wbase) columns cb
+        // of the new W rows, I_S - G[S,cb] (wbase = 128: the two halves run on
+        // different SIMDs; rows r, r + wbase, ... per thread).  The inner index
+        // runs in chunks of four whose loads are issued together (rows beyond cols
+        // are read from inside the buffer and replaced by zero).
+        const bool urow = tid < wbase;
+        for (int x = urow ? tid : tid - wbase; tid < 2 * wbase && x < n; x += wbase) {   // r or cb
             const int xa = urow ? x * n + s0 : s0 * n + x;         // G0[r,s0] or G0[s0,cb]
             const int xb = urow ? x * n + s1 : s1 * n + x;
             cd a0 = ldc(G, xa), a1 = cd{0.0, 0.0};
@@ -592,7 +603,8 @@ DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch
         cols += ksec;
         c.sync();
         if (cols + ksec > dk) {
-            delayed_flush(c, G_, L.du, L.dw, n, cols);
+            if (kGlb) delayed_flush_glb(c, G_, L.du, L.dw, n, cols);
+            else delayed_flush(c, G_, L.du, L.dw, n, cols);
             cols = 0;
         }
     }
@@ -600,7 +612,8 @@ DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch
         // wavefronts still deciding the trailing rejected sites read G: they
         // must be done before any wavefront's flush writes its tiles
         c.sync();
-        delayed_flush(c, G_, L.du, L.dw, n, cols);
+        if (kGlb) delayed_flush_glb(c, G_, L.du, L.dw, n, cols);
+        else delayed_flush(c, G_, L.du, L.dw, n, cols);
     }
     for (int i = tid; i < ns; i += c.nthreads()) {
         auto pp = props + i;
@@ -611,6 +624,15 @@ DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch
         ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
     }
     c.sync();
+}
+
+DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t,
+                             int nf, bool ascending, int& seed) {
+    site_loop_delayed_t<false>(c, m, ch, L, G, t, nf, ascending, seed);
+}
+DEVFN void site_loop_delayed_glb(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t,
+                                 int nf, bool ascending, int& seed) {
+    site_loop_delayed_t<true>(c, m, ch, L, G, t, nf, ascending, seed);
 }
 
 // ---------------------------------------------------------------- MFMA GEMM in LDS
@@ -1133,7 +1155,8 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
-            site_loop(c, m, ch, L, ch.g, t, nf, false, seed);
+            if (m.dk > 0) site_loop_delayed_glb(c, m, ch, L, ch.g, t, nf, false, seed);
+            else site_loop(c, m, ch, L, ch.g, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.g, n, L.finv, L.f);      // G <- e^{-V} G e^{V}
@@ -1159,7 +1182,8 @@ DEV void sweep_right_global(const Ctx& c, const ModelDev& m, const ChainDev& ch,
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.g, n, L.f, L.finv);      // G <- e^{V} G e^{-V} (old fields)
             tm.stop(c, ch, S_T_HS);
-            site_loop(c, m, ch, L, ch.g, t, nf, true, seed);
+            if (m.dk > 0) site_loop_delayed_glb(c, m, ch, L, ch.g, t, nf, true, seed);
+            else site_loop(c, m, ch, L, ch.g, t, nf, true, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.ur, n, L.f, nullptr);    // UR <- e^{V} UR (new fields)
@@ -1361,7 +1385,7 @@ DEV void body_prepare(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, kLds ? m.dk : 0);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.dk);
         if (kLds) prepare_lds(c, m, ch, L);
         else prepare_global(c, m, ch, L);
     }
@@ -1372,7 +1396,7 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, kLds ? m.dk : 0);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.dk);
         int seed = *ch.seed;
         if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
         for (int s = 0; s < a.nsweep; ++s) {

@@ -9,6 +9,11 @@ chain, 2 Ltrot measurements each) followed by reading the bin.
 
 Metric: effective independent S_SF(K) samples/s = sum over chains of
 (bins/s) / (2 tau_int), tau_int measured on the per-chain Re S_SF(K) bin series.
+Default (--mode learned): the chains run K plain measured sweeps, the proposal
+network is trained on the displacements of 12 more, then the chains run K
+steps of one learned global move + one measured sweep; `value` is the
+learned form's rate, the plain rate, both error bars and their ratio at equal
+error bars are in `config`.  --mode plain reports the plain sweep alone.
 Chains whose stabilisation drift reaches the reference's "ortho unstable"
 threshold (5.5e-5, src/number_conserving/src/stabilization.f90:320,358) -- the
 bosonic weight drives some chains towards poles of (1 - B)^-1 -- are run but
@@ -42,6 +47,18 @@ CONFIGS = {
     "c4": dict(name="number-conserving 9x9 beta=8 Ltrot=80 U1=1 U2=0, 256 chains",
                p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=80, beta=8.0, nwrap=10,
                       shift_loc=1.5), chains=256),
+    "uneven": dict(name="uneven batch: number-conserving 3x3/6x6/9x9/12x12 x beta 4/8 x U2 0/-0.5, "
+                        "U1=1 mu=-5 dtau=0.1, 16 chains each, 256 chains",
+                   models=[dict(pairing=False, u1=1.0, u2=u2, mu=-5.0, nlx=L, nly=L, ltrot=int(round(b / 0.1)),
+                                beta=b, nwrap=10, shift_loc=1.5)
+                           for L in (3, 6, 9, 12) for b in (4.0, 8.0) for u2 in (0.0, -0.5)],
+                   chains=256),
+    "c5": dict(name="pairing 12x12 beta=10 Ltrot=100 U1=1 U2=-0.5 Delta=0.3, 128 chains",
+               p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12, ltrot=100, beta=10.0,
+                      nwrap=10, shift_loc=1.0), chains=128),
+    "nc12": dict(name="number-conserving 12x12 beta=8 Ltrot=80 U1=1 U2=-0.5, 96 chains",
+                 p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=12, nly=12, ltrot=80, beta=8.0, nwrap=10,
+                        shift_loc=1.5), chains=96),
     "c1": dict(name="number-conserving 3x3 beta=4 Ltrot=40 U1=1 U2=0, 512 chains",
                p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=3, nly=3, ltrot=40, beta=4.0, nwrap=10,
                       shift_loc=1.5), chains=512),
@@ -116,6 +133,31 @@ def cpu_baseline(cfg, tau, budget_s=12.0):
                           f"reference binary unavailable: {e}"}
 
 
+METRIC = "effective S_SF(K) samples/sec, learned vs plain sweep, 1 MI355X at equal \u03c3"
+
+
+def roofline(p, nchains, kernel_ms, config):
+    """FP64 roofline of the sweep kernel from its HIP-event launch times."""
+    nch = int(abs(p.u1) > 1e-10) + int(abs(p.u2) > 1e-10)
+    f_chain = flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch)
+    avg_ms = float(np.mean(kernel_ms))
+    achieved = f_chain * nchains / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else float("nan")
+    return {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(config, nchains),
+            "kernel_ms": avg_ms, "flops_per_launch": f_chain * nchains}
+
+
+def choose_schedule(args, p, nchains):
+    if args.schedule == "auto":
+        from bafqmc_amd import autotune
+        from bafqmc_amd.native import load_library
+        tuned = autotune.tune_model(load_library(), p, chains=nchains, sweeps=2)
+        return [tuned.schedule], tuned
+    if args.schedule == "default":
+        return None, None
+    return [tuple(int(v) for v in args.schedule.split(","))], None
+
+
 def _efficiency(series, wall, stable):
     """Per-chain bin series -> (tau_int, effective samples/s, 1 / (SE^2 wall)).
     SE is the standard error of the mean over the stable chains' time
@@ -140,7 +182,8 @@ def run_learned(args, cfg, p, spec, nchains):
     from bafqmc_amd.proposal import ConvNet, LearnedProposal, to_net, train_on_sweeps
     from bafqmc_amd.solver import BatchedSolver
     phis, seeds = conf_set_batch(p, chain_seeds(13579113, nchains))
-    s = BatchedSolver([spec], [0] * nchains)
+    schedule, tuned = choose_schedule(args, p, nchains)
+    s = BatchedSolver([spec], [0] * nchains, schedule=schedule)
     s.set_fields(list(phis), seeds)
     s.prepare()
     for _ in range(args.warmup):
@@ -150,9 +193,11 @@ def run_learned(args, cfg, p, spec, nchains):
     K = args.steps
     sf0 = np.zeros((nchains, K))
     xs, ts = [], []
+    kernel_ms = []
     t0 = time.perf_counter()
     for k in range(K):
         s.sweep(1)
+        kernel_ms.append(s.last_kernel_ms())
         obs, st = s.read_bin_raw()
         sf0[:, k] = obs[:, 20] / obs[:, 32]
     wall0 = time.perf_counter() - t0
@@ -165,7 +210,7 @@ def run_learned(args, cfg, p, spec, nchains):
         ts += [b - a for a, b in zip(f0, f1)]
     s.read_bin()
     t1 = time.perf_counter()
-    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, 8, 3, max_batch=64, seed=1)
+    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, args.hidden, args.layers, max_batch=64, seed=1)
     var_c, r2 = train_on_sweeps(net, to_net(xs, p), to_net(ts, p), 20, 3e-3, np.random.default_rng(0))
     t_train = time.perf_counter() - t1
     sigma = np.sqrt(var_c)
@@ -202,22 +247,106 @@ def run_learned(args, cfg, p, spec, nchains):
     tau0, eff0, e0, se0 = _efficiency(sf0, wall0, stable)
     tau1, eff1, e1, se1 = _efficiency(sf1, wall1, stable)
     out = {
-        "metric": "effective S_SF(K) samples/sec, learned-proposal hybrid vs plain batched sweep, 1 MI355X",
+        "metric": METRIC,
         "value": eff1, "unit": "effective S_SF(K) samples/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": 1000.0 * wall1 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (HS fields i.i.d. N(0,1) from per-chain seeds)",
         "config": {"workload": cfg["name"], "chains": nchains, "stable_chains": int(stable.sum()),
+                   "value_is": "learned-proposal form (one learned global move + one measured sweep per step)",
+                   "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1],
+                                "autotune_ms_per_sweep": tuned.table if tuned else None},
                    "plain": {"ms_per_step": 1000.0 * wall0 / K, "tau_int_sf_k": tau0, "eff_samples_per_s": eff0,
                              "stderr_sf_k": se0, "inverse_variance_per_s": e0},
                    "learned": {"ms_per_step": 1000.0 * wall1 / K, "tau_int_sf_k": tau1, "eff_samples_per_s": eff1,
                                "stderr_sf_k": se1, "inverse_variance_per_s": e1,
                                "global_acceptance": accepted / (nchains * K), "lambda": lam,
                                "lambda_trials": {str(k): v for k, v in trials.items()},
+                               "network": {"layers": args.layers, "hidden": args.hidden, "params": net.nparam},
                                "train_s": t_train, "train_samples": len(xs), "heldout_r2": r2},
                    "gain_eff_samples": eff1 / eff0, "gain_at_equal_error_bars": e1 / e0},
+        "roofline": roofline(p, nchains, kernel_ms, args.config),
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, tau0)
     s.close()
     net.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_uneven(args, cfg, nchains):
+    """Plain sweeps of a batch mixing lattices, Ltrot and U2 in one handle: the
+    LDS path (N <= 96) and the device-memory path (12x12, N = 144) run as two
+    persistent kernels in one HIP graph; schedules from the autotuner."""
+    from bafqmc_amd import autotune
+    from bafqmc_amd.analysis import chain_seeds, tau_int
+    from bafqmc_amd.model import ModelSpec, conf_set_batch
+    from bafqmc_amd.native import load_library
+    from bafqmc_amd.solver import BatchedSolver
+    params = [make_params(d) for d in cfg["models"]]
+    chain_model = np.arange(nchains) % len(params)
+    t0 = time.perf_counter()
+    choices = autotune.tune(load_library(), params, chain_model, sweeps=1, max_sample=16) \
+        if args.schedule == "auto" else None
+    t_tune = time.perf_counter() - t0
+    s = BatchedSolver([ModelSpec(p) for p in params], chain_model,
+                      schedule=[c.schedule for c in choices] if choices else None)
+    phis, seeds = [], []
+    for k, p in enumerate(params):
+        idx = np.flatnonzero(chain_model == k)
+        f, sd = conf_set_batch(p, chain_seeds(24680 + 97 * k, len(idx)))
+        phis.append(list(f))
+        seeds.append(list(sd))
+    order = [0] * len(params)
+    all_f, all_s = [], []
+    for c in range(nchains):
+        k = chain_model[c]
+        all_f.append(phis[k][order[k]])
+        all_s.append(seeds[k][order[k]])
+        order[k] += 1
+    s.set_fields(all_f, all_s)
+    s.prepare()
+    for _ in range(args.warmup):
+        s.sweep(1)
+        s.read_bin()
+    sf = np.zeros((nchains, args.steps))
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.sweep(1)
+        kernel_ms.append(s.last_kernel_ms())
+        obs, st = s.read_bin_raw()
+        sf[:, k] = obs[:, 20] / obs[:, 32]
+    wall = time.perf_counter() - t0
+    stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf), axis=1)
+    # the models' S_SF(K) differ in scale: pool autocorrelations of standardised series
+    x = sf[stable] - sf[stable].mean(axis=1, keepdims=True)
+    sd = x.std(axis=1, keepdims=True)
+    tau, window, _ = tau_int(x / np.where(sd > 0, sd, 1.0))
+    value = int(stable.sum()) * (args.steps / wall) / (2.0 * tau)
+    flops = 0.0
+    for c in range(nchains):
+        p = params[chain_model[c]]
+        nch = int(abs(p.u1) > 1e-10) + int(abs(p.u2) > 1e-10)
+        flops += flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch)
+    avg_ms = float(np.mean(kernel_ms))
+    achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else float("nan")
+    out = {
+        "metric": METRIC, "value": value, "unit": "effective S_SF(K) samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000.0 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (HS fields i.i.d. N(0,1) from per-chain seeds)",
+        "config": {"workload": cfg["name"], "chains": nchains, "value_is": "plain batched sweep, uneven batch",
+                   "tau_int_sf_k": tau, "tau_window": window, "unstable_chains": int((~stable).sum()),
+                   "sweeps_per_s": nchains * args.steps / wall, "autotune_s": t_tune,
+                   "schedules": {f"{p.nlx}x{p.nly} b{p.beta:g} U2{p.u2:g}": list(sc)
+                                 for p, sc in zip(params, s.schedule())},
+                   "paths": {"lds_chains": int(sum(1 for c in range(nchains) if s.schedule()[chain_model[c]][0] == 1)),
+                             "hbm_chains": int(sum(1 for c in range(nchains) if s.schedule()[chain_model[c]][0] == 0))}},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel_ms": avg_ms,
+                     "flops_per_launch": flops},
+    }
+    s.close()
     print(json.dumps(out), flush=True)
 
 
@@ -231,7 +360,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", default="auto",
                     help="'auto' (autotuner, before the timed region), 'default' (library choice) or 'PATH,DELAYED'")
-    ap.add_argument("--mode", default="plain", choices=["plain", "learned"],
+    ap.add_argument("--layers", type=int, default=3, help="convolution layers of the proposal network")
+    ap.add_argument("--hidden", type=int, default=8, help="hidden channels of the proposal network")
+    ap.add_argument("--mode", default="learned", choices=["plain", "learned"],
                     help="plain: the batched local sweep; learned: plain sweeps, then the network trained on "
                          "their displacements, then sweeps interleaved with learned global moves, compared "
                          "at equal error bars")
@@ -258,23 +389,16 @@ def main():
 
     cfg = CONFIGS[args.config]
     nchains = args.chains or cfg["chains"]
+    if "models" in cfg:
+        run_uneven(args, cfg, nchains)
+        return
     p = make_params(cfg["p"])
     spec = ModelSpec(p)
     if args.mode == "learned":
         run_learned(args, cfg, p, spec, nchains)
         return
     phis, seeds = conf_set_batch(p, chain_seeds(13579113, nchains))
-    tuned = None
-    if args.schedule == "auto":
-        from bafqmc_amd import autotune
-        from bafqmc_amd.native import load_library
-        tuned = autotune.tune_model(load_library(),
-                                    p, chains=nchains, sweeps=2)
-        schedule = [tuned.schedule]
-    elif args.schedule == "default":
-        schedule = None
-    else:
-        schedule = [tuple(int(v) for v in args.schedule.split(","))]
+    schedule, tuned = choose_schedule(args, p, nchains)
     s = BatchedSolver([spec], [0] * nchains, schedule=schedule)
     s.set_fields(list(phis), seeds)
     s.prepare()
@@ -299,12 +423,8 @@ def main():
     tau, window, _ = tau_int(sf[stable]) if n_stable else (float("nan"), 0, None)
     sweeps_per_s = nchains * args.steps / wall
     value = n_stable * (args.steps / wall) / (2.0 * tau)
-    nch = int(abs(p.u1) > 1e-10) + int(abs(p.u2) > 1e-10)
-    f_chain = flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch)
-    avg_ms = float(np.mean(kernel_ms))
-    achieved = f_chain * nchains / (avg_ms * 1e-3) / 1e12
     out = {
-        "metric": "effective S_SF(K) samples/sec, plain batched sweep, 1 MI355X",
+        "metric": METRIC,
         "value": value,
         "unit": "effective S_SF(K) samples/s",
         "n_gpus": 1,
@@ -320,15 +440,14 @@ def main():
                    "nwrap": p.nwrap, "sweeps_per_s": sweeps_per_s, "tau_int_sf_k": tau,
                    "unstable_chains": nchains - n_stable,
                    "tau_window": window,
+                   "value_is": "plain batched sweep (no global moves)",
                    "kernel": "k_sweep_lds" if s.schedule()[0][0] == 1 else "k_sweep",
                    "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1],
                                 "autotune_ms_per_sweep": tuned.table if tuned else None},
                    "phase_ms_per_chain_sweep": {k: 1e3 * v / (nchains * args.steps) for k, v in
                                                 zip(("site", "kinetic", "measure", "wrap", "hs", "wrap_products",
                                                      "wrap_stab_u", "wrap_green"), phase)}},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.config, nchains),
-                     "kernel_ms": avg_ms, "flops_per_launch": f_chain * nchains},
+        "roofline": roofline(p, nchains, kernel_ms, args.config),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, tau)

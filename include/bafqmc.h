@@ -58,7 +58,8 @@ const char* bafq_last_error(const bafq_handle* h);
  * path  -1 choose, 0 Green's function and work matrices in device memory,
  *       1 G and the stabilisation work matrix resident in LDS (N <= 96);
  * delayed  -1 widest that fits, 0 per-site rank-1/rank-2 updates, or 4, 8,
- *       12, 16 accepted updates buffered before one MFMA flush (LDS path).
+ *       12, 16 accepted updates buffered in LDS before one MFMA flush into G
+ *       (either path).
  * Every schedule computes the same Markov chain (identical up to rounding);
  * bafqmc_amd.autotune picks one by timing.  bafq_create uses {-1, -1}. */
 typedef struct bafq_schedule {

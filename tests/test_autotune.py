@@ -45,7 +45,7 @@ def check_schedules_agree(lib, case):
             assert np.max(np.abs(gs[c] - ref[1][c])) < 1e-10 * np.max(np.abs(ref[1][c]))
             for k in ("density_total", "energy_density", "sf_K", "dw_K"):
                 assert abs(bins[c][k] - ref[0][c][k]) <= 1e-10 * max(1e-300, abs(ref[0][c][k])), (sched, k)
-    assert ran >= 3
+    assert ran >= 6
 
 
 @pytest.mark.parametrize("case", [3, 5])
@@ -55,7 +55,7 @@ def test_emu_schedules_agree(emu_lib, case):
 
 def test_emu_invalid_schedules_rejected(emu_lib):
     d = CASES[3]
-    for bad in ((2, 0), (1, 6), (0, 8), (1, 20)):
+    for bad in ((2, 0), (1, 6), (0, 6), (1, 20), (-2, 0)):
         with pytest.raises(SolverError):
             _run(emu_lib, d, [bad])
 

@@ -46,9 +46,10 @@ def test_emu_sweep_matches_oracle(emu_lib, case):
     check_sweep_vs_oracle(emu_lib, case)
 
 
+@pytest.mark.parametrize("delayed", [-1, 0])
 @pytest.mark.parametrize("case", [1, 3, 5])
-def test_emu_global_path_matches_oracle(emu_lib, case, monkeypatch):
-    monkeypatch.setattr(BatchedSolver, "default_schedule", (0, -1))
+def test_emu_global_path_matches_oracle(emu_lib, case, delayed, monkeypatch):
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (0, delayed))
     check_sweep_vs_oracle(emu_lib, case)
 
 

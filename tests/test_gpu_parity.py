@@ -218,10 +218,12 @@ def check_golden(lib, name):
     s.close()
 
 
+@pytest.mark.parametrize("delayed", [-1, 0])
 @pytest.mark.parametrize("name", ["nc_traj_3x3_u2", "pairing_traj_3x3", "pairing_traj_6x6_c2"])
-def test_global_memory_path_matches_reference(hip_lib, name, monkeypatch):
-    """The global-memory path (used for N > 96) on the golden cases."""
-    monkeypatch.setattr(BatchedSolver, "default_schedule", (0, -1))
+def test_global_memory_path_matches_reference(hip_lib, name, delayed, monkeypatch):
+    """The device-memory path (used for N > 96) on the golden cases, with
+    delayed (-1: widest) and per-site (0) local updates."""
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (0, delayed))
     check_golden(hip_lib, name)
 
 
