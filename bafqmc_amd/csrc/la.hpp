@@ -178,36 +178,53 @@ DEVFN void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool
         [&](int k, int j) { return load_op(B, opB, k, j, n); }, [&](int r, int j, cd v) { C[r * n + j] = v; });
 }
 
-// Real-by-complex: side 0: C = E * X ; side 1: C = X * E.  E real row-major.
-DEVFN void gemm_rc(const Ctx& c, cd* C, const double* E, const cd* X, int side, int n) {
-    const int nt = (n + 15) >> 4;
+// C = E X (side 0) or X E (side 1) on FP64 MFMA 16x16x4, E real N x N in MFMA
+// operand order (zero padded; side 0: Ez[ti][kk][lane] = E[16 ti + lane % 16][4 kk + lane / 16],
+// side 1: Ez[tj][kk][lane] = E[4 kk + lane / 16][16 tj + lane % 16]), X and C
+// complex N x N in device memory.  The loads of eight k-steps are issued
+// together before their products (the operands come from L2/HBM).
+DEVFN void gemm_rc(const Ctx& c, cd* C, const double* Ez, const cd* X, int side, int n) {
+    const int nt = (n + 15) >> 4, nk = (n + 3) >> 2;
     const int lane = c.lane();
     const int li = lane & 15, lk = lane >> 4;
+    auto Xg = as_global(X);
+    auto Cg = as_global(C);
     for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
         const int ti = t / nt, tj = t - ti * nt;
         const int ar = 16 * ti + li, bc = 16 * tj + li;
+        auto ez = as_global(Ez + (size_t)(side == 0 ? ti : tj) * nk * 64 + lane);
         d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < n; k0 += 4) {
-            const int k = k0 + lk;
-            if (side == 0) {
-                double a = 0.0;
-                cd b = cd{0.0, 0.0};
-                if (ar < n && k < n) a = E[ar * n + k];
-                if (bc < n && k < n) b = X[k * n + bc];
-                accr = c.mfma(a, b.r, accr);
-                acci = c.mfma(a, b.i, acci);
-            } else {
-                cd a = cd{0.0, 0.0};
-                double b = 0.0;
-                if (ar < n && k < n) a = X[ar * n + k];
-                if (bc < n && k < n) b = E[k * n + bc];
-                accr = c.mfma(a.r, b, accr);
-                acci = c.mfma(a.i, b, acci);
+        for (int kk0 = 0; kk0 < nk; kk0 += 8) {
+            double e[8];
+            cd x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int kk = kk0 + u, k = 4 * kk + lk;
+                e[u] = 0.0;
+                x[u] = cd{0.0, 0.0};
+                if (kk < nk) e[u] = ez[(size_t)kk * 64];
+                if (side == 0) {
+                    if (k < n && bc < n) x[u] = ldc(Xg, k * n + bc);
+                } else {
+                    if (ar < n && k < n) x[u] = ldc(Xg, ar * n + k);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (kk0 + u < nk) {
+                    if (side == 0) {
+                        accr = c.mfma(e[u], x[u].r, accr);
+                        acci = c.mfma(e[u], x[u].i, acci);
+                    } else {
+                        accr = c.mfma(x[u].r, e[u], accr);
+                        acci = c.mfma(x[u].i, e[u], acci);
+                    }
+                }
             }
         }
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
-            if (row < n && col < n) C[row * n + col] = cd{accr[r], acci[r]};
+            if (row < n && col < n) stc(Cg, row * n + col, cd{accr[r], acci[r]});
         }
     }
     c.sync();

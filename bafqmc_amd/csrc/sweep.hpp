@@ -1117,7 +1117,7 @@ DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.ur, n, L.f, nullptr);
         }
-        gemm_rc(c, ch.w1, m.ep, ch.ur, 0, n);
+        gemm_rc(c, ch.w1, m.ez[0], ch.ur, 0, n);
         mat_copy(c, ch.ur, ch.w1, n, false);
         if (nt % m.nwrap == 0) {
             load_ud(c, ch.wa, ch.ur, ch.dr, n, false);
@@ -1146,9 +1146,9 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
             tm.stop(c, ch, S_T_MEAS);
         }
         // propT_L: G <- e^{+dtau K} G e^{-dtau K}, UL <- UL e^{-dtau K}
-        gemm_rc(c, ch.w1, m.ep, ch.g, 1, n);
-        gemm_rc(c, ch.g, m.em, ch.w1, 0, n);
-        gemm_rc(c, ch.w1, m.ep, ch.ul, 1, n);
+        gemm_rc(c, ch.w1, m.ez[2], ch.g, 1, n);
+        gemm_rc(c, ch.g, m.ez[1], ch.w1, 0, n);
+        gemm_rc(c, ch.w1, m.ez[2], ch.ul, 1, n);
         mat_copy(c, ch.ul, ch.w1, n, false);
         tm.stop(c, ch, S_T_KIN);
         const int t = nt - 1;
@@ -1190,9 +1190,9 @@ DEV void sweep_right_global(const Ctx& c, const ModelDev& m, const ChainDev& ch,
             tm.stop(c, ch, S_T_HS);
         }
         // propT_R: G <- e^{-dtau K} G e^{+dtau K}, UR <- e^{-dtau K} UR
-        gemm_rc(c, ch.w1, m.ep, ch.g, 0, n);
-        gemm_rc(c, ch.g, m.em, ch.w1, 1, n);
-        gemm_rc(c, ch.w1, m.ep, ch.ur, 0, n);
+        gemm_rc(c, ch.w1, m.ez[0], ch.g, 0, n);
+        gemm_rc(c, ch.g, m.ez[3], ch.w1, 1, n);
+        gemm_rc(c, ch.w1, m.ez[0], ch.ur, 0, n);
         mat_copy(c, ch.ur, ch.w1, n, false);
         tm.stop(c, ch, S_T_KIN);
         if (nt % m.nwrap == 0) {
