@@ -231,15 +231,99 @@ DEVFN void gemm_rc(const Ctx& c, cd* C, const double* Ez, const cd* X, int side,
 }
 
 // ------------------------------------------------------ pivoted QR (ZGEQP3)
-// Scratch in LDS: tau[n] (cd), perm[n] (int), nrm[n] (double), bcast (>= 8 doubles).
+// Scratch in LDS: tau[n] (cd), perm[n] (int), nrm[n] (double), bcast (>= 8
+// doubles), v[n] (cd, the current reflector).
 struct QrScratch {
     cd* tau;
     int* perm;
     double* nrm;
     double* bcast;
+    cd* v;
 };
 
 constexpr int kGroup = 8;   // lanes cooperating on one column
+
+// H(k)^H = I - conj(tau) v v^H applied to columns j > k of A (device memory)
+// and their new squared norms below row k.  A group of NG lanes owns a column;
+// its rows k + gl + NG i are held in registers (slot i, R slots, n <= NG R)
+// between the dot product and the update, so the column is read and written
+// once; v comes from LDS.  With NG = kGroup the arithmetic and its order are
+// those of the two-pass loop of qrp_apply_loop.
+template <int R, int NG>
+DEV void qrp_apply_regs(const Ctx& c, cd* A, int n, int k, const QrScratch& s, cd ctau) {
+    constexpr int kGroup = NG;
+    const int tid = c.tid(), nth = c.nthreads();
+    const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
+    auto v = as_lds(s.v);
+    for (int jb = k + 1; jb < n; jb += ngrp) {
+        const int j = jb + grp;
+        const bool ok = j < n;
+        const int jc = ok ? j : n - 1;           // idle groups read a valid column, store nothing
+        cd x[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = k + gl + kGroup * i;
+            x[i] = cd{0.0, 0.0};
+            if (k + kGroup * i < n && r < n) x[i] = A[r * n + jc];
+        }
+        double wr = 0.0, wi = 0.0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = k + gl + kGroup * i;
+            if (k + kGroup * i < n && r < n) {
+                const cd p2 = conj(ldc(v, r)) * x[i];
+                wr += p2.r;
+                wi += p2.i;
+            }
+        }
+        wr = group_sum(c, wr, kGroup);
+        wi = group_sum(c, wi, kGroup);
+        const cd t = ctau * cd{wr, wi};
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = k + gl + kGroup * i;
+            if (k + kGroup * i < n && r < n) {
+                const cd y = x[i] - ldc(v, r) * t;
+                if (ok) A[r * n + j] = y;
+                if (r > k) a += abs2(y);
+            }
+        }
+        a = group_sum(c, a, kGroup);
+        if (ok && gl == 0) s.nrm[j] = a;
+    }
+}
+
+DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, const QrScratch& s, cd ctau) {
+    const int tid = c.tid(), nth = c.nthreads();
+    const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
+    auto v = as_lds(s.v);
+    for (int jb = k + 1; jb < n; jb += ngrp) {
+        const int j = jb + grp;
+        const bool ok = j < n;
+        double wr = 0.0, wi = 0.0;
+        if (ok) {
+            for (int r = k + gl; r < n; r += kGroup) {
+                const cd p2 = conj(ldc(v, r)) * A[r * n + j];
+                wr += p2.r;
+                wi += p2.i;
+            }
+        }
+        wr = group_sum(c, wr, kGroup);
+        wi = group_sum(c, wi, kGroup);
+        const cd t = ctau * cd{wr, wi};
+        double a = 0.0;
+        if (ok) {
+            for (int r = k + gl; r < n; r += kGroup) {
+                const cd x = A[r * n + j] - ldc(v, r) * t;
+                A[r * n + j] = x;
+                if (r > k) a += abs2(x);
+            }
+        }
+        a = group_sum(c, a, kGroup);
+        if (ok && gl == 0) s.nrm[j] = a;
+    }
+}
 
 // In-place Householder QR with column pivoting, A * P = Q * R.
 // On exit: strictly lower part of A holds the reflectors (v_k = 1 implied),
@@ -326,40 +410,24 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
         }
         c.sync();
         {
+            // the reflector, scaled, into A below the diagonal and into LDS
             const cd scale = cd{s.bcast[1], s.bcast[2]};
-            for (int r = k + 1 + tid; r < n; r += nth) A[r * n + k] = A[r * n + k] * scale;
-            if (tid == 0) A[k * n + k] = cd{s.bcast[3], 0.0};
+            for (int r = k + 1 + tid; r < n; r += nth) {
+                const cd x = A[r * n + k] * scale;
+                A[r * n + k] = x;
+                s.v[r] = x;
+            }
+            if (tid == 0) {
+                A[k * n + k] = cd{s.bcast[3], 0.0};
+                s.v[k] = cd{1.0, 0.0};
+            }
         }
         c.sync();
         // ---- apply H(k)^H = I - conj(tau) v v^H to columns j > k, new norms
         const cd ctau = conj(s.tau[k]);
-        for (int jb = k + 1; jb < n; jb += ngrp) {
-            const int j = jb + grp;
-            const bool ok = j < n;
-            double wr = 0.0, wi = 0.0;
-            if (ok) {
-                for (int r = k + gl; r < n; r += kGroup) {
-                    const cd v = (r == k) ? cd{1.0, 0.0} : A[r * n + k];
-                    const cd x = A[r * n + j];
-                    const cd p2 = conj(v) * x;
-                    wr += p2.r; wi += p2.i;
-                }
-            }
-            wr = group_sum(c, wr, kGroup);
-            wi = group_sum(c, wi, kGroup);
-            const cd t = ctau * cd{wr, wi};
-            double a = 0.0;
-            if (ok) {
-                for (int r = k + gl; r < n; r += kGroup) {
-                    const cd v = (r == k) ? cd{1.0, 0.0} : A[r * n + k];
-                    const cd x = A[r * n + j] - v * t;
-                    A[r * n + j] = x;
-                    if (r > k) a += abs2(x);
-                }
-            }
-            a = group_sum(c, a, kGroup);
-            if (ok && gl == 0) s.nrm[j] = a;
-        }
+        if (n <= 18 * kGroup) qrp_apply_regs<18, kGroup>(c, A, n, k, s, ctau);
+        else if (n <= 18 * 2 * kGroup) qrp_apply_regs<18, 2 * kGroup>(c, A, n, k, s, ctau);
+        else qrp_apply_loop(c, A, n, k, s, ctau);
         c.sync();
     }
     // ---- D and row normalisation of R

@@ -174,6 +174,7 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0) {
     L.f = (cd*)take((size_t)n * sizeof(cd));
     L.finv = (cd*)take((size_t)n * sizeof(cd));
     L.col = (cd*)take(4 * (size_t)n * sizeof(cd));
+    L.qr.v = L.col;                              // free while a QR runs
     L.row = (cd*)take(4 * (size_t)n * sizeof(cd));
     L.red = (double*)take((size_t)nwaves * 64 * sizeof(double));
     L.site = (cd*)take(6 * (size_t)n * sizeof(cd));

@@ -36,7 +36,7 @@ def test_emu_gemm(emu_lib):
                 assert rel_err(emu_lib.debug_gemm(a, b, oa, ob), A @ B) < 1e-13
 
 
-@pytest.mark.parametrize("n,variant", [(9, 0), (18, 0), (33, 0), (9, 1), (18, 1), (32, 1)])
+@pytest.mark.parametrize("n,variant", [(9, 0), (18, 0), (33, 0), (150, 0), (9, 1), (18, 1), (32, 1)])
 def test_emu_qrp(emu_lib, n, variant):
     check_qrp(emu_lib, n, variant)
 

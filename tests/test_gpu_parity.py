@@ -102,7 +102,7 @@ def check_qrp(lib, n, variant):
     np.testing.assert_array_equal(perm, pl)
 
 
-@pytest.mark.parametrize("n,variant", [(9, 0), (18, 0), (72, 0), (81, 0), (144, 0),
+@pytest.mark.parametrize("n,variant", [(9, 0), (18, 0), (72, 0), (81, 0), (144, 0), (200, 0), (288, 0),
                                        (9, 1), (18, 1), (72, 1), (81, 1), (88, 1)])
 def test_device_qrp(hip_lib, n, variant):
     check_qrp(hip_lib, n, variant)
