@@ -158,6 +158,11 @@ def choose_schedule(args, p, nchains):
     return [tuple(int(v) for v in args.schedule.split(","))], None
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _efficiency(series, wall, stable):
     """Per-chain bin series -> (tau_int, effective samples/s, 1 / (SE^2 wall)).
     SE is the standard error of the mean over the stable chains' time
@@ -194,12 +199,15 @@ def run_learned(args, cfg, p, spec, nchains):
     sf0 = np.zeros((nchains, K))
     xs, ts = [], []
     kernel_ms = []
+    progress(f"plain phase: {K} steps")
     t0 = time.perf_counter()
     for k in range(K):
         s.sweep(1)
         kernel_ms.append(s.last_kernel_ms())
         obs, st = s.read_bin_raw()
         sf0[:, k] = obs[:, 20] / obs[:, 32]
+        if (k + 1) % 10 == 0:
+            progress(f"plain step {k + 1}/{K}")
     wall0 = time.perf_counter() - t0
     ntrain = max(1, min(12, 8192 // nchains + 1))
     for _ in range(ntrain):
@@ -209,11 +217,13 @@ def run_learned(args, cfg, p, spec, nchains):
         xs += f0
         ts += [b - a for a, b in zip(f0, f1)]
     s.read_bin()
+    progress(f"training the network on {len(xs)} sweep displacements")
     t1 = time.perf_counter()
     net = ConvNet(p.ltrot, p.nly, p.nlx, 2, args.hidden, args.layers, max_batch=64, seed=1)
     var_c, r2 = train_on_sweeps(net, to_net(xs, p), to_net(ts, p), 20, 3e-3, np.random.default_rng(0))
     t_train = time.perf_counter() - t1
     sigma = np.sqrt(var_c)
+    progress(f"trained in {t_train:.1f} s, held-out R2 {r2:.4f}")
     rng = np.random.default_rng(2468)
     # step size: the largest mean squared jump over a few trial moves
     fields, sd = s.get_fields()
@@ -234,6 +244,7 @@ def run_learned(args, cfg, p, spec, nchains):
     # ---- hybrid phase: one learned global move + one measured sweep per step
     sf1 = np.zeros((nchains, K))
     accepted = 0
+    progress(f"learned phase: {K} steps, lambda {lam}")
     t0 = time.perf_counter()
     for k in range(K):
         f, _ = s.get_fields()
@@ -242,6 +253,8 @@ def run_learned(args, cfg, p, spec, nchains):
         s.sweep(1)
         obs, st = s.read_bin_raw()
         sf1[:, k] = obs[:, 20] / obs[:, 32]
+        if (k + 1) % 10 == 0:
+            progress(f"learned step {k + 1}/{K}")
     wall1 = time.perf_counter() - t0
     stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf0), axis=1) & np.all(np.isfinite(sf1), axis=1)
     tau0, eff0, e0, se0 = _efficiency(sf0, wall0, stable)
@@ -310,12 +323,14 @@ def run_uneven(args, cfg, nchains):
         s.read_bin()
     sf = np.zeros((nchains, args.steps))
     kernel_ms = []
+    progress(f"uneven batch: {args.steps} steps")
     t0 = time.perf_counter()
     for k in range(args.steps):
         s.sweep(1)
         kernel_ms.append(s.last_kernel_ms())
         obs, st = s.read_bin_raw()
         sf[:, k] = obs[:, 20] / obs[:, 32]
+        progress(f"step {k + 1}/{args.steps}")
     wall = time.perf_counter() - t0
     stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf), axis=1)
     # the models' S_SF(K) differ in scale: pool autocorrelations of standardised series
@@ -415,6 +430,8 @@ def main():
         obs, st = s.read_bin_raw()
         sf[:, k] = obs[:, 20] / obs[:, 32]
         phase += st[:, 12:20].sum(axis=0)
+        if (k + 1) % 20 == 0 or args.steps < 20:
+            progress(f"step {k + 1}/{args.steps}")
     wall = time.perf_counter() - t0
     ms_per_step = 1000.0 * wall / args.steps
     # the drift maximum S_XMAX is kept over the whole run

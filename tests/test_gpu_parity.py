@@ -240,6 +240,39 @@ def test_uneven_batch_equals_single_chains(hip_lib, mixed):
     check_uneven_batch(hip_lib, mixed)
 
 
+def test_uneven_batch_across_paths_equals_single_chains(hip_lib):
+    """Lattices from 3x3 to 12x12 (N = 9 ... 288) in one handle with the
+    library's own schedules: the 12x12 chains run on the device-memory path,
+    the others in LDS, as two persistent kernels of one launch graph; every
+    chain ends bit for bit as when run alone."""
+    specs = [CASES[1], CASES[4], CASES[6], CASES[7], CASES[5]]
+    models = [ModelSpec(solver_params(d)) for d in specs]
+    chain_model = [2, 0, 3, 1, 4, 2, 0, 3]
+    phis = [fixed_fields(specs[m]["ltrot"], specs[m]["nlx"] * specs[m]["nly"]) for m in chain_model]
+    seeds = [2222 + 11 * c for c in range(len(chain_model))]
+    s = BatchedSolver(models, chain_model, lib=hip_lib)
+    assert {p for p, _ in s.schedule()} == {0, 1}
+    s.set_fields(phis, seeds)
+    s.prepare()
+    s.sweep(2)
+    batch = s.read_bin()
+    gb = [s.green(c) for c in range(len(chain_model))]
+    fb, sb = s.get_fields()
+    s.close()
+    for c, m in enumerate(chain_model):
+        s1 = BatchedSolver([models[m]], [0], lib=hip_lib)
+        s1.set_fields([phis[c]], [seeds[c]])
+        s1.prepare()
+        s1.sweep(2)
+        one = s1.read_bin()[0]
+        for k in ("density_total", "energy_density", "sf_K", "dw_K"):
+            assert batch[c][k] == one[k], (c, k)
+        assert np.array_equal(gb[c], s1.green(0))
+        f1, s1d = s1.get_fields()
+        assert np.array_equal(fb[c], f1[0]) and sb[c] == s1d[0]
+        s1.close()
+
+
 def check_uneven_batch(hip_lib, mixed=False):
     """Chains of different lattices/solvers in one launch evolve exactly as alone
     (mixed: half of the models on the device-memory path, so both persistent
