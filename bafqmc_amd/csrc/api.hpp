@@ -205,16 +205,23 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
                                   lds_bytes(m.n, lds_waves, true, 0) <= limit;
             if (want.path == 1 && !lds_fits) { *out = h; return fail(h, "schedule asks for the LDS path but N does not fit"); }
             const int path = want.path == -1 ? (lds_fits ? 1 : 0) : want.path;
-            // U and W of the delayed updates live in LDS on both paths
+            // U and W of the delayed updates live in LDS; on the device-memory
+            // path a width that does not fit there uses the chain's work
+            // matrices w2, w3 instead (free during the local updates; N x dk
+            // entries each, so dk <= N)
             auto fits = [&](int dk) {
                 return path == 1 ? lds_bytes(m.n, lds_waves, true, dk) <= limit
                                  : lds_bytes(m.n, glb_waves, false, dk) <= limit;
             };
             m.dk = 0;
+            m.uw_glb = 0;
             if (want.delayed > 0) {
                 if (!fits(want.delayed)) {
-                    *out = h;
-                    return fail(h, "schedule's delayed-update width does not fit in LDS");
+                    if (path == 1 || want.delayed > m.n) {
+                        *out = h;
+                        return fail(h, "schedule's delayed-update width does not fit");
+                    }
+                    m.uw_glb = 1;
                 }
                 m.dk = want.delayed;
             } else if (want.delayed == -1) {
@@ -223,6 +230,11 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
                         m.dk = dk;
                         break;
                     }
+                if (path == 0 && m.dk < 16 && m.n >= 16) {
+                    // G streams from HBM at every flush: the widest buffers, in device memory
+                    m.dk = 16;
+                    m.uw_glb = 1;
+                }
             }
             h->schedule.push_back(bafq_schedule{path, m.dk});
         }
@@ -349,7 +361,7 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
         const int path = h->schedule[chain_model[c]].path;
         pids[path].push_back(c);
         const size_t lb = path == P_LDS ? lds_bytes(m.n, rt_lds_block_threads() / 64, true, m.dk)
-                                        : lds_bytes(m.n, nw_glb, false, m.dk);
+                                        : lds_bytes(m.n, nw_glb, false, m.uw_glb ? 0 : m.dk);
         h->max_lds[path] = std::max(h->max_lds[path], lb + kSlotBytes);
     }
     rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * n_chains);

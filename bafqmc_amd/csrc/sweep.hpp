@@ -53,7 +53,8 @@ constexpr int kMaxTiles = 5;      // ceil(36 / 8) output tiles per wavefront
 struct ModelDev {
     int kind;            // 0 number conserving, 1 Nambu pairing
     int n, ns, nlx, nly, ltrot, nwrap, nst, has_k;
-    int dk;              // LDS path: columns of the delayed-update buffers (0: per-site updates)
+    int dk;              // columns of the delayed-update buffers (0: per-site updates)
+    int uw_glb;          // the buffers live in the chain's work matrices w2, w3 (device memory), not LDS
     int active[2];
     double u1, u2, mu, rdelta, shift_loc;
     cd alpha[2];
@@ -428,10 +429,8 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 
 // G0 -= U[:, :cols] W[:cols, :] on FP64 MFMA 16x16x4 (each output tile owned by
 // one wavefront); G in LDS or in device memory, U and W in LDS.
-template <class GP>
-DEV void delayed_flush_t(const Ctx& c, GP G, const cd* Ut_, const cd* W_, int n, int cols) {
-    auto Ut = as_lds(Ut_);
-    auto W = as_lds(W_);
+template <class GP, class UP>
+DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols) {
     const int nt = (n + 15) >> 4;
     const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
     for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
@@ -475,15 +474,19 @@ DEV void delayed_flush_t(const Ctx& c, GP G, const cd* Ut_, const cd* W_, int n,
 }
 
 DEVFN void delayed_flush(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
-    delayed_flush_t(c, as_lds(G), Ut, W, n, cols);
+    delayed_flush_t(c, as_lds(G), as_lds(Ut), as_lds(W), n, cols);
 }
 DEVFN void delayed_flush_glb(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
-    delayed_flush_t(c, as_global(G), Ut, W, n, cols);
+    delayed_flush_t(c, as_global(G), as_lds(Ut), as_lds(W), n, cols);
+}
+DEVFN void delayed_flush_glb_uw(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
+    delayed_flush_t(c, as_global(G), as_global(Ut), as_global(W), n, cols);
 }
 
 // Same decisions, RNG use and result as site_loop_t (LocalU_metro + LocalU_prop_L/R).
-// G in LDS (kGlb false) or in device memory; U and W in LDS.  Requires dk <= 16.
-template <bool kGlb>
+// G in LDS (kGlb false) or in device memory; U and W in LDS, or (kUW) in the
+// chain's work matrices w2, w3 in device memory.  Requires dk <= 16, dk <= n.
+template <bool kGlb, bool kUW = false>
 DEV void site_loop_delayed_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t,
                              int nf, bool ascending, int& seed) {
     const int n = m.n, ns = m.ns, dk = m.dk;
@@ -495,8 +498,16 @@ DEV void site_loop_delayed_t(const Ctx& c, const ModelDev& m, const ChainDev& ch
         if constexpr (kGlb) return as_global(G_);
         else return as_lds(G_);
     }();
-    auto Ut = as_lds(L.du);
-    auto W = as_lds(L.dw);
+    cd* const du = kUW ? ch.w2 : L.du;
+    cd* const dw = kUW ? ch.w3 : L.dw;
+    auto Ut = [&] {
+        if constexpr (kUW) return as_global(du);
+        else return as_lds(du);
+    }();
+    auto W = [&] {
+        if constexpr (kUW) return as_global(dw);
+        else return as_lds(dw);
+    }();
     double* phi_t = ch.phi + (size_t)t * ns * 2;
     make_proposals(c, m, L, phi_t, nf, ascending, seed);
     seed = lcg_jump(seed, 2 * ns);
@@ -604,8 +615,9 @@ DEV void site_loop_delayed_t(const Ctx& c, const ModelDev& m, const ChainDev& ch
         cols += ksec;
         c.sync();
         if (cols + ksec > dk) {
-            if (kGlb) delayed_flush_glb(c, G_, L.du, L.dw, n, cols);
-            else delayed_flush(c, G_, L.du, L.dw, n, cols);
+            if (kUW) delayed_flush_glb_uw(c, G_, du, dw, n, cols);
+            else if (kGlb) delayed_flush_glb(c, G_, du, dw, n, cols);
+            else delayed_flush(c, G_, du, dw, n, cols);
             cols = 0;
         }
     }
@@ -613,8 +625,9 @@ DEV void site_loop_delayed_t(const Ctx& c, const ModelDev& m, const ChainDev& ch
         // wavefronts still deciding the trailing rejected sites read G: they
         // must be done before any wavefront's flush writes its tiles
         c.sync();
-        if (kGlb) delayed_flush_glb(c, G_, L.du, L.dw, n, cols);
-        else delayed_flush(c, G_, L.du, L.dw, n, cols);
+        if (kUW) delayed_flush_glb_uw(c, G_, du, dw, n, cols);
+        else if (kGlb) delayed_flush_glb(c, G_, du, dw, n, cols);
+        else delayed_flush(c, G_, du, dw, n, cols);
     }
     for (int i = tid; i < ns; i += c.nthreads()) {
         auto pp = props + i;
@@ -633,7 +646,8 @@ DEVFN void site_loop_delayed(const Ctx& c, const ModelDev& m, const ChainDev& ch
 }
 DEVFN void site_loop_delayed_glb(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t,
                                  int nf, bool ascending, int& seed) {
-    site_loop_delayed_t<true>(c, m, ch, L, G, t, nf, ascending, seed);
+    if (m.uw_glb) site_loop_delayed_t<true, true>(c, m, ch, L, G, t, nf, ascending, seed);
+    else site_loop_delayed_t<true, false>(c, m, ch, L, G, t, nf, ascending, seed);
 }
 
 // ---------------------------------------------------------------- MFMA GEMM in LDS
@@ -1386,7 +1400,7 @@ DEV void body_prepare(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.dk);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
         if (kLds) prepare_lds(c, m, ch, L);
         else prepare_global(c, m, ch, L);
     }
@@ -1397,7 +1411,7 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.dk);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
         int seed = *ch.seed;
         if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
         for (int s = 0; s < a.nsweep; ++s) {
