@@ -242,13 +242,11 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
     // ---- layout of the device allocation
     Bump b;
     const size_t off_models = b.take(sizeof(ModelDev) * n_models);
-    std::vector<size_t> off_ep(n_models), off_em(n_models), off_ph(n_models), off_bd(n_models);
+    std::vector<size_t> off_ph(n_models), off_bd(n_models);
     std::vector<size_t> off_ez(n_models);
     std::vector<std::vector<double>> ez(n_models);
     for (int k = 0; k < n_models; ++k) {
         const int n = h->models[k].n, ns = h->models[k].ns;
-        off_ep[k] = b.take(sizeof(double) * n * n);
-        off_em[k] = b.take(sizeof(double) * n * n);
         // MFMA operand order of e^{-+dtau A} (left: [ti][kk][lane] = E[16ti+l%16][4kk+l/16],
         // right: [tj][kk][lane] = E[4kk+l/16][16tj+l%16]), zero padded.
         const int nt = (n + 15) / 16, nk = (n + 3) / 4;
@@ -313,13 +311,8 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
     // ---- model tables
     for (int k = 0; k < n_models; ++k) {
         ModelDev& m = h->models[k];
-        const size_t nn = (size_t)m.n * m.n;
-        rt_h2d(base + off_ep[k], models[k].expk_p, sizeof(double) * nn);
-        rt_h2d(base + off_em[k], models[k].expk_m, sizeof(double) * nn);
         rt_h2d(base + off_ph[k], phases[k].data(), sizeof(cd) * m.ns);
         rt_h2d(base + off_bd[k], bonds[k].data(), sizeof(int) * 3 * m.ns);
-        m.ep = (const double*)(base + off_ep[k]);
-        m.em = (const double*)(base + off_em[k]);
         rt_h2d(base + off_ez[k], ez[k].data(), sizeof(double) * ez[k].size());
         for (int w = 0; w < 4; ++w)
             m.ez[w] = (const double*)(base + off_ez[k]) + (size_t)w * (ez[k].size() / 4);

@@ -58,8 +58,6 @@ struct ModelDev {
     int active[2];
     double u1, u2, mu, rdelta, shift_loc;
     cd alpha[2];
-    const double* ep;    // e^{-dtau A}, n x n real row-major
-    const double* em;    // e^{+dtau A}
     // MFMA-operand order: [0] e^{-dtau A} as left factor ([ti][kk][64]),
     // [1] e^{+dtau A} left, [2] e^{-dtau A} right ([tj][kk][64]), [3] e^{+dtau A} right
     const double* ez[4];
