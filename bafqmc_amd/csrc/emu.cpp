@@ -199,13 +199,13 @@ int rt_memset(void* dst, int v, size_t bytes) { std::memset(dst, v, bytes); retu
 int rt_sync() { return 0; }
 const char* rt_error() { return g_err.c_str(); }
 
-// Workgroup sizes default to the GPU's (256 threads on the global-memory
-// path, 512 on the LDS path); BAFQ_BLOCK / BAFQ_LDS_BLOCK select others in
-// tests (the host checks that the waves cover every MFMA tile).
+// Workgroup sizes default to the GPU's (512 threads on both paths);
+// BAFQ_BLOCK / BAFQ_LDS_BLOCK select others in tests (the host checks that
+// the waves cover every MFMA tile).
 int rt_block_threads() {
     const char* s = std::getenv("BAFQ_BLOCK");
-    int b = s ? std::atoi(s) : 256;
-    if (b < 64 || b > 1024 || (b % 64) != 0) b = 256;
+    int b = s ? std::atoi(s) : 512;
+    if (b < 64 || b > 1024 || (b % 64) != 0) b = 512;
     return b;
 }
 
