@@ -27,8 +27,9 @@ int rt_d2h(void* dst, const void* src, size_t bytes);
 int rt_memset(void* dst, int v, size_t bytes);
 // One persistent launch per path: chains a.ids[0..a.count) taken from the
 // queue a.queue (zeroed by the back end before the kernel) by `grid`
-// workgroups.  rt_launch_paths launches the paths of a batch together and
-// keeps back-end state (HIP graphs) in *cache, released by rt_release_launches.
+// workgroups.  rt_launch_paths launches the paths of a batch together; the
+// launch functions keep per-handle back-end state (timing events, HIP graphs)
+// in *cache, released by rt_release_launches.
 struct PathLaunch {
     PathId path;
     LaunchArgs a;
@@ -37,7 +38,7 @@ struct PathLaunch {
 };
 int rt_launch_paths(void** cache, KernelId k, const PathLaunch* pl, int n, double* ms);
 void rt_release_launches(void* cache);
-int rt_launch_warmup(const LaunchArgs& a, int grid, int block, double* ms);
+int rt_launch_warmup(void** cache, const LaunchArgs& a, int grid, int block, double* ms);
 // workgroups of a kernel that fit on the device at once (<= 0 on error)
 int rt_resident_groups(KernelId k, PathId path, int block, size_t smem);
 double rt_clock_hz();          // rate of Ctx::clock()
@@ -440,7 +441,7 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
         a.ids = h->d_ids;
         a.count = (int)h->chains.size();
         const int block = 64, grid = (a.count + block - 1) / block;
-        if (rt_launch_warmup(a, grid, block, &total_ms)) return fail(h, rt_error());
+        if (rt_launch_warmup(&h->launches, a, grid, block, &total_ms)) return fail(h, rt_error());
     } else {
         PathLaunch pl[2];
         int n = 0;

@@ -254,36 +254,11 @@ size_t rt_lds_limit() {
     return (size_t)v;
 }
 
-static hipEvent_t g_ev[2] = {nullptr, nullptr};
-
-static int timed_begin() {
-    if (!g_ev[0]) {
-        if (check(hipEventCreate(&g_ev[0]), "hipEventCreate") || check(hipEventCreate(&g_ev[1]), "hipEventCreate"))
-            return 1;
-    }
-    return check(hipEventRecord(g_ev[0], 0), "hipEventRecord");
-}
-
-static int timed_end(double* ms) {
-    if (check(hipEventRecord(g_ev[1], 0), "hipEventRecord") || check(hipEventSynchronize(g_ev[1]), "hipEventSynchronize"))
-        return 1;
-    float f = 0.0f;
-    if (check(hipEventElapsedTime(&f, g_ev[0], g_ev[1]), "hipEventElapsedTime")) return 1;
-    if (ms) *ms = (double)f;
-    return 0;
-}
-
 static const void* kernel_of(KernelId k, PathId path) {
     if (k == K_PREPARE) return path == P_LDS ? (const void*)k_prepare_lds : (const void*)k_prepare;
     return path == P_LDS ? (const void*)k_sweep_lds : (const void*)k_sweep;
 }
 
-int rt_launch_warmup(const LaunchArgs& a, int grid, int block, double* ms) {
-    if (timed_begin()) return 1;
-    hipLaunchKernelGGL(k_warmup, dim3(grid), dim3(block), 0, 0, a);
-    if (check(hipGetLastError(), "kernel launch")) return 1;
-    return timed_end(ms);
-}
 
 int rt_resident_groups(KernelId k, PathId path, int block, size_t smem) {
     const void* fn = kernel_of(k, path);
@@ -297,10 +272,12 @@ int rt_resident_groups(KernelId k, PathId path, int block, size_t smem) {
     return per_cu > 0 && cus > 0 ? per_cu * cus : 1;
 }
 
-// Instantiated graphs of one handle, keyed by (kernel, sweeps, measure): each
-// holds, per path, a queue reset followed by the persistent kernel; the paths
-// are independent branches and run concurrently.
+// Launch state of one handle: its timing events and instantiated graphs, keyed
+// by (kernel, sweeps, measure); each graph holds, per path, a queue reset
+// followed by the persistent kernel, the paths as independent branches that
+// run concurrently.
 struct GraphCache {
+    hipEvent_t ev[2] = {nullptr, nullptr};
     struct Entry {
         int kind, nsweep, measure;
         std::vector<LaunchArgs> args;
@@ -321,7 +298,40 @@ void rt_release_launches(void* cache) {
     if (!cache) return;
     GraphCache* gc = (GraphCache*)cache;
     gc->clear();
+    for (hipEvent_t& e : gc->ev)
+        if (e) (void)hipEventDestroy(e);
     delete gc;
+}
+
+static GraphCache* launch_state(void** cache) {
+    if (!*cache) *cache = new GraphCache();
+    return (GraphCache*)*cache;
+}
+
+static int timed_begin(GraphCache* gc) {
+    if (!gc->ev[0]) {
+        if (check(hipEventCreate(&gc->ev[0]), "hipEventCreate") || check(hipEventCreate(&gc->ev[1]), "hipEventCreate"))
+            return 1;
+    }
+    return check(hipEventRecord(gc->ev[0], 0), "hipEventRecord");
+}
+
+static int timed_end(GraphCache* gc, double* ms) {
+    if (check(hipEventRecord(gc->ev[1], 0), "hipEventRecord") ||
+        check(hipEventSynchronize(gc->ev[1]), "hipEventSynchronize"))
+        return 1;
+    float f = 0.0f;
+    if (check(hipEventElapsedTime(&f, gc->ev[0], gc->ev[1]), "hipEventElapsedTime")) return 1;
+    if (ms) *ms = (double)f;
+    return 0;
+}
+
+int rt_launch_warmup(void** cache, const LaunchArgs& a, int grid, int block, double* ms) {
+    GraphCache* gc = launch_state(cache);
+    if (timed_begin(gc)) return 1;
+    hipLaunchKernelGGL(k_warmup, dim3(grid), dim3(block), 0, 0, a);
+    if (check(hipGetLastError(), "kernel launch")) return 1;
+    return timed_end(gc, ms);
 }
 
 static int build_graph(GraphCache::Entry& e, KernelId k, const PathLaunch* pl, int n) {
@@ -358,8 +368,7 @@ static int build_graph(GraphCache::Entry& e, KernelId k, const PathLaunch* pl, i
 }
 
 int rt_launch_paths(void** cache, KernelId k, const PathLaunch* pl, int n, double* ms) {
-    if (!*cache) *cache = new GraphCache();
-    GraphCache* gc = (GraphCache*)*cache;
+    GraphCache* gc = launch_state(cache);
     const int nsweep = n ? pl[0].a.nsweep : 0, measure = n ? pl[0].a.measure : 0;
     GraphCache::Entry* hit = nullptr;
     for (GraphCache::Entry& e : gc->entries)
@@ -376,9 +385,9 @@ int rt_launch_paths(void** cache, KernelId k, const PathLaunch* pl, int n, doubl
             return 1;
         }
     }
-    if (timed_begin()) return 1;
+    if (timed_begin(gc)) return 1;
     if (check(hipGraphLaunch(hit->exec, 0), "hipGraphLaunch")) return 1;
-    return timed_end(ms);
+    return timed_end(gc, ms);
 }
 
 int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c) {

@@ -220,7 +220,7 @@ int rt_lds_block_threads() {
 
 size_t rt_lds_limit() { return 160 * 1024; }
 
-int rt_launch_warmup(const LaunchArgs& a, int grid, int block, double* ms) {
+int rt_launch_warmup(void**, const LaunchArgs& a, int grid, int block, double* ms) {
     if (ms) *ms = 0.0;
     // one thread per chain; no workgroup cooperation
     for (int g = 0; g < grid; ++g)

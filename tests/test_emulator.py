@@ -69,7 +69,7 @@ def test_emu_local_update_variants(emu_lib, case, width, monkeypatch):
 
 @pytest.mark.parametrize("mixed", [False, True])
 def test_emu_uneven_batch(emu_lib, mixed):
-    check_uneven_batch(emu_lib, mixed)
+    check_uneven_batch(emu_lib, mixed, light=True)
 
 
 @pytest.mark.parametrize("pairing,delta", [(False, 0.0), (True, 0.2)])
