@@ -48,6 +48,17 @@ def log_weights(solver, fields=None):
     if fields is None:
         fields, _ = solver.get_fields()
     st = solver.stats()
+    shapes = {np.shape(f) for f in fields}
+    if len(shapes) == 1 and len({id(solver.params(c)) for c in range(solver.nchains)}) == 1:
+        # one model: all chains at once
+        F = np.stack(fields)
+        p = solver.params(0)
+        lw = -0.5 * np.sum(F * F, axis=(1, 2, 3))
+        if p.pairing:
+            dtau = p.beta / p.ltrot
+            a2 = math.sqrt(-2.0 * p.u2 * dtau) if p.u2 < -1e-10 else 0.0
+            return lw - a2 * np.sum(F[:, :, :, 1], axis=(1, 2)) + st[:, LOGDET]
+        return lw + 2.0 * st[:, LOGDET]
     return np.array([log_weight(solver.params(c), fields[c], st[c, LOGDET]) for c in range(solver.nchains)])
 
 

@@ -137,7 +137,7 @@ def train_on_sweeps(net, x, t, epochs, lr, rng, holdout=0.25, batch=None):
     """Fit the network to the one-sweep displacement t of fields x (both in the
     network layout) by minibatch Adam; returns the per-channel residual
     variance on the held-out part and the held-out R^2 (variance explained)."""
-    batch = batch or net.max_batch
+    batch = min(batch or net.max_batch, net.max_batch)
     n = len(x)
     ntr = n - max(batch, int(n * holdout))
     for ep in range(epochs):
@@ -167,22 +167,26 @@ class LearnedProposal:
         self.mask = active_channels(params)
         self.sigma = np.broadcast_to(np.asarray(sigma, dtype=np.float64), (2,)) * np.sqrt(self.lam) * self.mask
 
-    def drift(self, phis):
+    def drift(self, F):
+        """lam f(F) * mask for stacked fields F [B, Ltrot, Ns, 2]."""
         if self.net is None:
-            return [np.zeros_like(f) for f in phis]
-        out = []
-        for i in range(0, len(phis), self.net.max_batch):
-            part = phis[i:i + self.net.max_batch]
-            out += from_net(self.net.forward(to_net(part, self.params)), self.params)
-        return [self.lam * d * self.mask for d in out]
+            return np.zeros_like(F)
+        out = np.empty_like(F)
+        for i in range(0, len(F), self.net.max_batch):
+            part = F[i:i + self.net.max_batch]
+            out[i:i + len(part)] = np.stack(from_net(self.net.forward(to_net(part, self.params)), self.params))
+        return self.lam * out * self.mask
 
     def propose(self, phis, rng):
-        """(proposals, log q(phi|phi') - log q(phi'|phi)) for every chain."""
-        f0 = self.drift(phis)
-        props = [p + d + self.sigma * rng.normal(size=p.shape) for p, d in zip(phis, f0)]
-        f1 = self.drift(props)
+        """(proposals, log q(phi|phi') - log q(phi'|phi)) for every chain; all
+        chains at once as one array (the noise is drawn in FP32 and used exactly
+        as drawn in both the move and its density)."""
+        F = np.stack([np.asarray(f, dtype=np.float64) for f in phis])
+        f0 = self.drift(F)
+        xi = rng.standard_normal(size=F.shape, dtype=np.float32)
+        P = F + f0 + self.sigma * xi
+        f1 = self.drift(P)
         on = self.sigma > 0
         w = np.where(on, 1.0 / (2.0 * np.where(on, self.sigma, 1.0) ** 2), 0.0)
-        lq = np.array([float(np.sum(w * (-(p - q - d1) ** 2 + (q - p - d0) ** 2)))
-                       for p, q, d0, d1 in zip(phis, props, f0, f1)])
-        return props, lq
+        lq = np.sum(w * ((P - F - f0) ** 2 - (F - P - f1) ** 2), axis=(1, 2, 3))
+        return list(P), lq

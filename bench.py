@@ -8,7 +8,8 @@ from per-chain seeds.  One step = one batched sweep (sweep_L + sweep_R of every
 chain, 2 Ltrot measurements each) followed by reading the bin.
 
 Metric: effective independent S_SF(K) samples/s = sum over chains of
-(bins/s) / (2 tau_int), tau_int measured on the per-chain Re S_SF(K) bin series.
+(bins/s) / (2 tau_int), tau_int measured on the per-chain Re S_SF(K) bin
+series (each standardised, then pooled over chains).
 Default (--mode learned): the chains run K plain measured sweeps, the proposal
 network is trained on the displacements of 12 more, then the chains run K
 steps of one learned global move + one measured sweep; `value` is the
@@ -163,14 +164,24 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def pooled_tau(series):
+    """tau_int of per-chain bin series pooled over chains, each chain's series
+    standardised first: chains near a pole of (1 - B)^-1 have S_SF(K)
+    fluctuations orders of magnitude above the rest and would otherwise set
+    the pooled autocorrelation alone."""
+    from bafqmc_amd.analysis import tau_int
+    x = series - series.mean(axis=1, keepdims=True)
+    sd = x.std(axis=1, keepdims=True)
+    return tau_int(x / np.where(sd > 0, sd, 1.0))
+
+
 def _efficiency(series, wall, stable):
     """Per-chain bin series -> (tau_int, effective samples/s, 1 / (SE^2 wall)).
     SE is the standard error of the mean over the stable chains' time
     averages (independent chains), so 1 / (SE^2 wall) compares methods at
     equal error bars: its ratio is the ratio of the times to reach one."""
-    from bafqmc_amd.analysis import tau_int
     x = series[stable]
-    tau, _, _ = tau_int(x)
+    tau, _, _ = pooled_tau(x)
     steps = x.shape[1]
     eff = len(x) * (steps / wall) / (2.0 * tau)
     means = x.mean(axis=1)
@@ -219,8 +230,8 @@ def run_learned(args, cfg, p, spec, nchains):
     s.read_bin()
     progress(f"training the network on {len(xs)} sweep displacements")
     t1 = time.perf_counter()
-    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, args.hidden, args.layers, max_batch=64, seed=1)
-    var_c, r2 = train_on_sweeps(net, to_net(xs, p), to_net(ts, p), 20, 3e-3, np.random.default_rng(0))
+    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, args.hidden, args.layers, max_batch=min(512, nchains), seed=1)
+    var_c, r2 = train_on_sweeps(net, to_net(xs, p), to_net(ts, p), 20, 3e-3, np.random.default_rng(0), batch=64)
     t_train = time.perf_counter() - t1
     sigma = np.sqrt(var_c)
     progress(f"trained in {t_train:.1f} s, held-out R2 {r2:.4f}")
@@ -333,10 +344,7 @@ def run_uneven(args, cfg, nchains):
         progress(f"step {k + 1}/{args.steps}")
     wall = time.perf_counter() - t0
     stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf), axis=1)
-    # the models' S_SF(K) differ in scale: pool autocorrelations of standardised series
-    x = sf[stable] - sf[stable].mean(axis=1, keepdims=True)
-    sd = x.std(axis=1, keepdims=True)
-    tau, window, _ = tau_int(x / np.where(sd > 0, sd, 1.0))
+    tau, window, _ = pooled_tau(sf[stable])
     value = int(stable.sum()) * (args.steps / wall) / (2.0 * tau)
     flops = 0.0
     for c in range(nchains):
@@ -437,7 +445,7 @@ def main():
     # the drift maximum S_XMAX is kept over the whole run
     stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf), axis=1)
     n_stable = int(stable.sum())
-    tau, window, _ = tau_int(sf[stable]) if n_stable else (float("nan"), 0, None)
+    tau, window, _ = pooled_tau(sf[stable]) if n_stable else (float("nan"), 0, None)
     sweeps_per_s = nchains * args.steps / wall
     value = n_stable * (args.steps / wall) / (2.0 * tau)
     out = {
