@@ -13,6 +13,7 @@
 #include "../../include/bafqmc.h"
 #include "sweep.hpp"
 #include "net.hpp"
+#include "gmove.hpp"
 
 namespace bafq {
 
@@ -25,6 +26,7 @@ void rt_free(void* p);
 int rt_h2d(void* dst, const void* src, size_t bytes);
 int rt_d2h(void* dst, const void* src, size_t bytes);
 int rt_memset(void* dst, int v, size_t bytes);
+int rt_d2d(void* dst, const void* src, size_t bytes);
 // One persistent launch per path: chains a.ids[0..a.count) taken from the
 // queue a.queue (zeroed by the back end before the kernel) by `grid`
 // workgroups.  rt_launch_paths launches the paths of a batch together; the
@@ -41,6 +43,7 @@ void rt_release_launches(void* cache);
 int rt_launch_warmup(void** cache, const LaunchArgs& a, int grid, int block, double* ms);
 // workgroups of a kernel that fit on the device at once (<= 0 on error)
 int rt_resident_groups(KernelId k, PathId path, int block, size_t smem);
+
 double rt_clock_hz();          // rate of Ctx::clock()
 int rt_sync();
 const char* rt_error();
@@ -60,6 +63,13 @@ int rt_conv_bwd_weight(const float* gout, int cout, const float* in, int cin, fl
                        NetShape s);
 int rt_mse_grad(const float* y, const float* t, float* g, size_t n, float scale);
 int rt_adam(float* p, const float* g, float* m, float* v, float lr, int step, size_t n);
+// learned global moves (gmove.hpp); device pointers, y may be null (no network)
+int rt_gm_to_net(const double* phi, float* x, const GmArgs& a);
+int rt_gm_propose(double* phi, double* old, double* f0, const float* y, const GmArgs& a);
+int rt_gm_sums(const double* phi, const double* old, const double* f0, const float* y, const GmArgs& a, double* out);
+int rt_gm_restore(double* phi, const double* old, const int* accept, int per, size_t n);
+// words [c * per, (c + 1) * per) of dst <- src for chains c with accept[c] == 0
+int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n);
 
 constexpr double kZero = 1.0e-10;   // src/number_conserving/src/calc_basic.f90:5
 
@@ -89,6 +99,11 @@ struct bafq_handle {
     int grid[2][2] = {{0, 0}, {0, 0}};        // persistent grid per [kernel][path]
     std::vector<bafq_schedule> schedule;      // in effect, per model
     void* launches = nullptr;                 // back-end launch cache
+    // learned global moves: old fields, forward drift, per-chain sums, flags, subset ids
+    double *gm_old = nullptr, *gm_f0 = nullptr, *gm_out = nullptr, *gm_stats = nullptr;
+    uint64_t* gm_state = nullptr;             // copy of every chain's matrices before the move
+    int* gm_accept = nullptr;
+    size_t chain_block_bytes = 0;             // matrix block of chain 0 (with one model: of every chain)
     double last_ms = 0.0;
 };
 
@@ -304,6 +319,7 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
         o.ull = b.take(mat * lists); o.vll = b.take(mat * lists);
         o.drl = b.take(vec * lists); o.dll = b.take(vec * lists);
         o.wa = b.take(mat); o.w1 = b.take(mat); o.w2 = b.take(mat); o.w3 = b.take(mat);
+        if (c == 0) h->chain_block_bytes = b.off - o.g;          // the same for every chain of one model
     }
     h->dev_bytes = b.off;
     if (rt_malloc(&h->dev, h->dev_bytes)) { *out = h; return fail(h, std::string("device allocation failed: ") + rt_error()); }
@@ -386,6 +402,8 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
 
 void bafq_destroy(bafq_handle* h) {
     if (!h) return;
+    if (h->gm_old) bafq::rt_free(h->gm_old);
+    if (h->gm_state) bafq::rt_free(h->gm_state);
     bafq::rt_release_launches(h->launches);
     if (h->dev) bafq::rt_free(h->dev);
     delete h;
@@ -706,6 +724,91 @@ int bafq_net_train(bafq_net* h, int batch, const float* x, const float* target, 
         // the input activation is overwritten by nothing: act[0] stays x
     }
     return rt_sync() ? net_fail(h, rt_error()) : 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ learned global moves
+extern "C" {
+
+int bafq_global_propose(bafq_handle* h, bafq_net* net, double lam, const double* sigma, uint64_t seed,
+                        double* out) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (h->models.size() != 1) return fail(h, "device global moves need a handle with one model");
+    if (!sigma || !out || !(lam >= 0.0)) return fail(h, "invalid global move arguments");
+    const ModelDev& m = h->models[0];
+    const int nc = (int)h->chains.size();
+    const NetShape s{m.ltrot, m.nly, m.nlx};
+    if (net && (net->d.lt != m.ltrot || net->d.ly != m.nly || net->d.lx != m.nlx || net->d.channels != 2))
+        return fail(h, "the network's lattice is not the model's");
+    const size_t per = (size_t)gm_per(s), total = per * nc;
+    if (!h->gm_old) {
+        // old fields, forward drift, 5 sums per chain (doubles); acceptance flags
+        const size_t bytes = sizeof(double) * (2 * total + 5 * (size_t)nc) + sizeof(int) * (size_t)nc;
+        if (rt_malloc((void**)&h->gm_old, bytes)) return fail(h, rt_error());
+        h->gm_f0 = h->gm_old + total;
+        h->gm_out = h->gm_f0 + total;
+        h->gm_accept = (int*)(h->gm_out + 5 * (size_t)nc);
+    }
+    // every chain's matrices (G, UDV factors and checkpoint lists, work matrices:
+    // one block of the same size per chain, consecutive) and counters, so that
+    // rejected chains get them back without a rebuild
+    const size_t block = h->chain_block_bytes;
+    if (!h->gm_state) {
+        if (rt_malloc((void**)&h->gm_state, block * nc + sizeof(double) * S_LEN * (size_t)nc))
+            return fail(h, rt_error());
+        h->gm_stats = (double*)((char*)h->gm_state + block * nc);
+    }
+    if (rt_d2d(h->gm_state, h->chains[0].g, block * nc) ||
+        rt_d2d(h->gm_stats, h->d_stats, sizeof(double) * S_LEN * (size_t)nc))
+        return fail(h, rt_error());
+    GmArgs a;
+    a.s = s;
+    a.lam = lam;
+    a.seed = seed;
+    for (int ch = 0; ch < 2; ++ch) {
+        a.mask[ch] = m.active[ch] ? 1.0 : 0.0;
+        a.sigma[ch] = m.active[ch] && sigma[ch] > 0.0 ? sigma[ch] : 0.0;
+        a.w[ch] = a.sigma[ch] > 0.0 ? 1.0 / (2.0 * a.sigma[ch] * a.sigma[ch]) : 0.0;
+    }
+    const int chunk = net ? net->max_batch : nc;
+    for (int c0 = 0; c0 < nc; c0 += chunk) {
+        a.c0 = c0;
+        a.batch = std::min(chunk, nc - c0);
+        const float* y = nullptr;
+        if (net) {
+            if (rt_gm_to_net(h->d_phi, net->act[0], a)) return fail(h, rt_error());
+            if (net_forward_dev(net, a.batch)) return fail(h, net->err);
+            y = net->act[net->d.layers];
+        }
+        if (rt_gm_propose(h->d_phi, h->gm_old, h->gm_f0, y, a)) return fail(h, rt_error());
+        if (net) {
+            if (rt_gm_to_net(h->d_phi, net->act[0], a)) return fail(h, rt_error());
+            if (net_forward_dev(net, a.batch)) return fail(h, net->err);
+        }
+        if (rt_gm_sums(h->d_phi, h->gm_old, h->gm_f0, y, a, h->gm_out)) return fail(h, rt_error());
+    }
+    if (rt_sync() || rt_d2h(out, h->gm_out, sizeof(double) * 5 * (size_t)nc)) return fail(h, rt_error());
+    return launch_all(h, K_PREPARE, 0, 0);
+}
+
+int bafq_global_finish(bafq_handle* h, const int32_t* accept) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (!h->gm_old || !accept) return fail(h, "no device global move to finish");
+    const ModelDev& m = h->models[0];
+    const int nc = (int)h->chains.size();
+    const int per = gm_per(NetShape{m.ltrot, m.nly, m.nlx});
+    if (rt_h2d(h->gm_accept, accept, sizeof(int) * nc)) return fail(h, rt_error());
+    if (rt_gm_restore(h->d_phi, h->gm_old, h->gm_accept, per, (size_t)per * nc)) return fail(h, rt_error());
+    // matrices and counters of the rejected chains from the copy taken before the move
+    const size_t words = h->chain_block_bytes / 8;
+    if (rt_gm_restore_words((uint64_t*)h->chains[0].g, h->gm_state, h->gm_accept, words, words * nc) ||
+        rt_gm_restore_words((uint64_t*)h->d_stats, (const uint64_t*)h->gm_stats, h->gm_accept, S_LEN,
+                            (size_t)S_LEN * nc))
+        return fail(h, rt_error());
+    return rt_sync() ? fail(h, rt_error()) : 0;
 }
 
 }  // extern "C"

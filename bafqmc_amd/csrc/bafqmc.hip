@@ -179,7 +179,69 @@ __global__ void __launch_bounds__(256) k_adam(float* p, const float* g, float* m
         adam_elem(p, g, m, v, lr, step, (int)e);
 }
 
+// ---- learned global moves (gmove.hpp)
+__global__ void __launch_bounds__(256) k_gm_to_net(const double* phi, float* x, GmArgs a) {
+    const size_t n = (size_t)a.batch * 2 * a.s.vol();
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        gm_to_net_elem(phi, x, a, (int)e);
+}
+__global__ void __launch_bounds__(256) k_gm_propose(double* phi, double* old, double* f0, const float* y, GmArgs a) {
+    const size_t n = (size_t)a.batch * gm_per(a.s);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        gm_propose_elem(phi, old, f0, y, a, (int)e);
+}
+// one workgroup per chain: strided partial sums, then a tree in LDS
+__global__ void __launch_bounds__(256) k_gm_sums(const double* phi, const double* old, const double* f0,
+                                                 const float* y, GmArgs a, double* out) {
+    __shared__ double part[5][256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    gm_sums(phi, old, f0, y, a, b, tid, gm_per(a.s), 256, acc);
+    for (int k = 0; k < 5; ++k) part[k][tid] = acc[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int k = 0; k < 5; ++k) part[k][tid] += part[k][tid + h];
+        __syncthreads();
+    }
+    if (tid < 5) out[(size_t)(a.c0 + b) * 5 + tid] = part[tid][0];
+}
+__global__ void __launch_bounds__(256) k_gm_restore(double* phi, const double* old, const int* accept, int per,
+                                                    size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        gm_restore_elem(phi, old, accept, per, (int)e);
+}
+
+__global__ void __launch_bounds__(256) k_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept,
+                                                          size_t per, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        if (!accept[e / per]) dst[e] = src[e];
+}
+
 static int grid_for(size_t n) { return (int)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535); }
+
+int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n) {
+    hipLaunchKernelGGL(k_gm_restore_words, dim3(grid_for(n)), dim3(256), 0, 0, dst, src, accept, per, n);
+    return check(hipGetLastError(), "k_gm_restore_words");
+}
+
+int rt_gm_to_net(const double* phi, float* x, const GmArgs& a) {
+    hipLaunchKernelGGL(k_gm_to_net, dim3(grid_for((size_t)a.batch * 2 * a.s.vol())), dim3(256), 0, 0, phi, x, a);
+    return check(hipGetLastError(), "k_gm_to_net");
+}
+int rt_gm_propose(double* phi, double* old, double* f0, const float* y, const GmArgs& a) {
+    hipLaunchKernelGGL(k_gm_propose, dim3(grid_for((size_t)a.batch * gm_per(a.s))), dim3(256), 0, 0, phi, old, f0, y,
+                       a);
+    return check(hipGetLastError(), "k_gm_propose");
+}
+int rt_gm_sums(const double* phi, const double* old, const double* f0, const float* y, const GmArgs& a, double* out) {
+    hipLaunchKernelGGL(k_gm_sums, dim3(a.batch), dim3(256), 0, 0, phi, old, f0, y, a, out);
+    return check(hipGetLastError(), "k_gm_sums");
+}
+int rt_gm_restore(double* phi, const double* old, const int* accept, int per, size_t n) {
+    hipLaunchKernelGGL(k_gm_restore, dim3(grid_for(n)), dim3(256), 0, 0, phi, old, accept, per, n);
+    return check(hipGetLastError(), "k_gm_restore");
+}
 
 int rt_conv_fwd(const float* in, int cin, const float* W, const float* b, float* out, int cout, int batch,
                 NetShape s, int relu) {
@@ -227,6 +289,9 @@ int rt_d2h(void* dst, const void* src, size_t bytes) {
     return check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
 }
 int rt_memset(void* dst, int v, size_t bytes) { return check(hipMemset(dst, v, bytes), "hipMemset"); }
+int rt_d2d(void* dst, const void* src, size_t bytes) {
+    return check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+}
 int rt_sync() { return check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 const char* rt_error() { return g_err.c_str(); }
 

@@ -196,6 +196,7 @@ void rt_free(void* p) { std::free(p); }
 int rt_h2d(void* dst, const void* src, size_t bytes) { std::memcpy(dst, src, bytes); return 0; }
 int rt_d2h(void* dst, const void* src, size_t bytes) { std::memcpy(dst, src, bytes); return 0; }
 int rt_memset(void* dst, int v, size_t bytes) { std::memset(dst, v, bytes); return 0; }
+int rt_d2d(void* dst, const void* src, size_t bytes) { std::memmove(dst, src, bytes); return 0; }
 int rt_sync() { return 0; }
 const char* rt_error() { return g_err.c_str(); }
 
@@ -267,6 +268,35 @@ int rt_launch_paths(void**, KernelId k, const PathLaunch* pl, int n, double* ms)
         g_err = "emulator: the threads of a workgroup passed different numbers of barriers";
         return 1;
     }
+    return 0;
+}
+
+// ------------------------------------------------------------------ learned global moves
+int rt_gm_to_net(const double* phi, float* x, const GmArgs& a) {
+    const size_t n = (size_t)a.batch * 2 * a.s.vol();
+    for (size_t e = 0; e < n; ++e) gm_to_net_elem(phi, x, a, (int)e);
+    return 0;
+}
+int rt_gm_propose(double* phi, double* old, double* f0, const float* y, const GmArgs& a) {
+    const size_t n = (size_t)a.batch * gm_per(a.s);
+    for (size_t e = 0; e < n; ++e) gm_propose_elem(phi, old, f0, y, a, (int)e);
+    return 0;
+}
+int rt_gm_sums(const double* phi, const double* old, const double* f0, const float* y, const GmArgs& a, double* out) {
+    for (int b = 0; b < a.batch; ++b) {
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        gm_sums(phi, old, f0, y, a, b, 0, gm_per(a.s), 1, acc);
+        for (int k = 0; k < 5; ++k) out[(size_t)(a.c0 + b) * 5 + k] = acc[k];
+    }
+    return 0;
+}
+int rt_gm_restore(double* phi, const double* old, const int* accept, int per, size_t n) {
+    for (size_t e = 0; e < n; ++e) gm_restore_elem(phi, old, accept, per, (int)e);
+    return 0;
+}
+int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n) {
+    for (size_t e = 0; e < n; ++e)
+        if (!accept[e / per]) dst[e] = src[e];
     return 0;
 }
 

@@ -98,3 +98,39 @@ def uniform_shift_proposals(solver, rng, width):
                 g[:, :, nf] += width * (rng.random() - 0.5)
         out.append(g)
     return out
+
+
+def metropolis_global_device(solver, net, lam, sigma, rng, seed=None):
+    """One exact global Metropolis-Hastings step for every chain with the
+    proposal of bafqmc_amd.proposal.LearnedProposal (phi' = phi + lam f(phi) +
+    sigma xi, sigma per channel, already scaled), drawn, evaluated and undone
+    on the device (bafq_global_propose / bafq_global_finish): no fields cross
+    to the host.  net: a proposal.ConvNet or None (no drift).  Returns the
+    boolean acceptance per chain; G is current afterwards."""
+    import ctypes as C
+    lib = solver.lib
+    n = solver.nchains
+    st0 = solver.stats()
+    out = np.zeros((n, 5))
+    sig = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, dtype=np.float64), (2,)))
+    if seed is None:
+        seed = int(rng.integers(0, 2 ** 63))
+    solver._check(lib.bafq_global_propose(solver.h, net.h if net is not None else None, float(lam),
+                                          sig.ctypes.data_as(C.POINTER(C.c_double)), C.c_uint64(seed),
+                                          out.ctypes.data_as(C.POINTER(C.c_double))), "bafq_global_propose")
+    st1 = solver.stats()
+    p = solver.params(0)
+    if p.pairing:
+        dtau = p.beta / p.ltrot
+        a2 = math.sqrt(-2.0 * p.u2 * dtau) if p.u2 < -1e-10 else 0.0
+        lw0 = -0.5 * out[:, 1] - a2 * out[:, 3] + st0[:, LOGDET]
+        lw1 = -0.5 * out[:, 2] - a2 * out[:, 4] + st1[:, LOGDET]
+    else:
+        lw0 = -0.5 * out[:, 1] + 2.0 * st0[:, LOGDET]
+        lw1 = -0.5 * out[:, 2] + 2.0 * st1[:, LOGDET]
+    u = rng.random(n)
+    accept = np.log(np.maximum(u, 1e-300)) < lw1 - lw0 + out[:, 0]
+    flags = np.ascontiguousarray(accept.astype(np.int32))
+    solver._check(lib.bafq_global_finish(solver.h, flags.ctypes.data_as(C.POINTER(C.c_int32))),
+                  "bafq_global_finish")
+    return accept

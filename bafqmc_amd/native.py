@@ -65,6 +65,8 @@ SYMBOLS = {
     "bafq_net_forward": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p]),
     "bafq_net_gradient": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, c_double_p, c_float_p]),
     "bafq_net_train": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, C.c_int, C.c_float, c_double_p]),
+    "bafq_global_propose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, c_double_p, C.c_uint64, c_double_p]),
+    "bafq_global_finish": (C.c_int, [C.c_void_p, c_int32_p]),
     "bafq_get_green": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),

@@ -170,6 +170,22 @@ int bafq_net_gradient(bafq_net* h, int batch, const float* x, const float* targe
 int bafq_net_train(bafq_net* h, int batch, const float* x, const float* target, int steps, float lr,
                    double* loss);
 
+/* A learned global move of every chain on the device (no counterpart in the
+ * reference; bafqmc_amd.global_moves.metropolis_global with
+ * bafqmc_amd.proposal.LearnedProposal is the host-side form).  For a handle
+ * with one model whose lattice is the network's (net may be NULL: no drift),
+ * every chain's fields become phi' = phi + lam f(phi) + sigma[ch] xi, f the
+ * network's output on phi (channels whose field does not couple are not
+ * moved), xi standard normal numbers from a counter-based generator keyed by
+ * seed, and G is rebuilt for phi' (as bafq_prepare).  out[c][0..4]:
+ * log q(phi|phi') - log q(phi'|phi), sum phi^2, sum phi'^2, sum phi_2 and
+ * sum phi'_2 (the Gaussian parts of log W).  bafq_global_finish then keeps
+ * phi' where accept[c] != 0 and elsewhere restores phi, G, the checkpoint
+ * lists and the counters from the device copy bafq_global_propose took. */
+int bafq_global_propose(bafq_handle* h, bafq_net* net, double lam, const double* sigma, uint64_t seed,
+                        double* out);
+int bafq_global_finish(bafq_handle* h, const int32_t* accept);
+
 #ifdef __cplusplus
 }
 #endif

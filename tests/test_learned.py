@@ -104,3 +104,123 @@ def test_net_rejects_bad_batch(emu_lib):
 def test_net_gpu(hip_lib):
     _check_net(hip_lib)
     _check_net(hip_lib, dict(lt=40, ly=3, lx=3, channels=2, hidden=8, layers=3), batch=4)
+
+
+def _device_move_case(lib, pairing):
+    import ctypes as C
+    from bafqmc_amd import global_moves as GM
+    from bafqmc_amd.model import ModelSpec
+    from bafqmc_amd.proposal import from_net, to_net
+    from bafqmc_amd.solver import BatchedSolver
+    from tests.helpers import solver_params
+    d = dict(pairing=pairing, u1=0.8, u2=-0.3, mu=-4.0, rdelta=0.4 if pairing else 0.0, nlx=3, nly=3, ltrot=8,
+             beta=1.0, nwrap=4, shift_loc=1.0)
+    p = solver_params(d)
+    rng = np.random.default_rng(21)
+    phis = [rng.normal(size=(8, 9, 2)) * 0.5 for _ in range(4)]
+    s = BatchedSolver([ModelSpec(p)], [0] * 4, lib=lib)
+    s.set_fields(phis, [5, 6, 7, 8])
+    s.prepare()
+    net = ConvNet(8, 3, 3, 2, 3, 3, max_batch=3, seed=4, lib=lib)    # 3 < 4 chains: two chunks
+    lam, sigma = 0.7, np.array([0.05, 0.08])
+    out = np.zeros((4, 5))
+    sig = np.ascontiguousarray(sigma)
+    s._check(lib.bafq_global_propose(s.h, net.h, lam, sig.ctypes.data_as(C.POINTER(C.c_double)), C.c_uint64(99),
+                                     out.ctypes.data_as(C.POINTER(C.c_double))), "propose")
+    P, _ = s.get_fields()
+    F = np.stack(phis)
+    P = np.stack(P)
+    def fwd(X):
+        return np.concatenate([np.stack(from_net(net.forward(to_net(list(X[i:i + 3]), p)), p))
+                               for i in range(0, len(X), 3)])
+    f0 = lam * fwd(F)
+    f1 = lam * fwd(P)
+    w = 1.0 / (2.0 * sigma ** 2)
+    lq = np.sum(w * ((P - F - f0) ** 2 - (F - P - f1) ** 2), axis=(1, 2, 3))
+    np.testing.assert_allclose(out[:, 0], lq, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(out[:, 1], np.sum(F ** 2, axis=(1, 2, 3)), rtol=1e-12)
+    np.testing.assert_allclose(out[:, 2], np.sum(P ** 2, axis=(1, 2, 3)), rtol=1e-12)
+    np.testing.assert_allclose(out[:, 3], np.sum(F[..., 1], axis=(1, 2)), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(out[:, 4], np.sum(P[..., 1], axis=(1, 2)), rtol=1e-12, atol=1e-12)
+    xi = (P - F - f0) / sigma                                  # the device's normal numbers
+    assert abs(xi.mean()) < 0.15 and abs(xi.var() - 1.0) < 0.2
+    # G of the proposals is current, then rejected chains get phi and its G back
+    g_prop = [s.green(c) for c in range(4)]
+    accept = np.array([1, 0, 1, 0], dtype=np.int32)
+    s._check(lib.bafq_global_finish(s.h, accept.ctypes.data_as(C.POINTER(C.c_int32))), "finish")
+    fin, seeds = s.get_fields()
+    ref = BatchedSolver([ModelSpec(p)], [0] * 4, lib=lib)
+    ref.set_fields(phis, [5, 6, 7, 8])
+    ref.prepare()
+    for c in range(4):
+        np.testing.assert_array_equal(fin[c], P[c] if accept[c] else F[c])
+        g = s.green(c)
+        if accept[c]:
+            np.testing.assert_array_equal(g, g_prop[c])
+        else:
+            np.testing.assert_allclose(g, ref.green(c), rtol=0, atol=1e-12 * np.max(np.abs(g)))
+    assert list(seeds) == [5, 6, 7, 8]
+    # the acceptance built from these pieces is the host one's
+    assert GM.metropolis_global_device(s, net, lam, sigma, np.random.default_rng(3)).dtype == bool
+    s.close()
+    ref.close()
+    net.close()
+
+
+@pytest.mark.parametrize("pairing", [False, True])
+def test_device_move_emulator(emu_lib, pairing):
+    _device_move_case(emu_lib, pairing)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pairing", [False, True])
+def test_device_move_gpu(hip_lib, pairing):
+    _device_move_case(hip_lib, pairing)
+
+
+def _rejected_move_is_invisible(lib):
+    """A device move whose chains are all rejected leaves no trace: the next
+    sweep is bit for bit that of a batch that never moved."""
+    import ctypes as C
+    from bafqmc_amd.model import ModelSpec
+    from bafqmc_amd.solver import BatchedSolver
+    from tests.helpers import solver_params
+    d = dict(pairing=True, u1=0.8, u2=-0.3, mu=-4.0, rdelta=0.4, nlx=3, nly=3, ltrot=8, beta=1.0, nwrap=4,
+             shift_loc=1.0)
+    p = solver_params(d)
+    rng = np.random.default_rng(8)
+    phis = [rng.normal(size=(8, 9, 2)) * 0.5 for _ in range(3)]
+    runs = []
+    for move in (False, True):
+        s = BatchedSolver([ModelSpec(p)], [0] * 3, lib=lib)
+        s.set_fields(phis, [15, 16, 17])
+        s.prepare()
+        s.sweep(1)
+        s.read_bin()
+        if move:
+            out = np.zeros((3, 5))
+            sig = np.array([0.1, 0.1])
+            s._check(lib.bafq_global_propose(s.h, None, 1.0, sig.ctypes.data_as(C.POINTER(C.c_double)),
+                                             C.c_uint64(5), out.ctypes.data_as(C.POINTER(C.c_double))), "propose")
+            flags = np.zeros(3, dtype=np.int32)
+            s._check(lib.bafq_global_finish(s.h, flags.ctypes.data_as(C.POINTER(C.c_int32))), "finish")
+        s.sweep(1)
+        obs, st = s.read_bin_raw()
+        runs.append((obs, st, [s.green(c) for c in range(3)], s.get_fields()))
+        s.close()
+    (o0, s0, g0, f0), (o1, s1, g1, f1) = runs
+    np.testing.assert_array_equal(o0, o1)
+    np.testing.assert_array_equal(s0[:, :8], s1[:, :8])
+    for a, b in zip(g0, g1):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(f0[0], f1[0]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_rejected_device_move_is_invisible_emulator(emu_lib):
+    _rejected_move_is_invisible(emu_lib)
+
+
+@pytest.mark.gpu
+def test_rejected_device_move_is_invisible_gpu(hip_lib):
+    _rejected_move_is_invisible(hip_lib)

@@ -145,3 +145,33 @@ def test_learned_moves_sample_the_ed_answer(trained, hip_lib):
         ed = FIXTURE[CASE]["observables"][names[k]]["ed"]
         print(f"{k}: {m:.6g} +- {se:.2g}  ED {ed:.6g}  z {(m - ed) / se:+.2f}", flush=True)
         assert abs(m - ed) <= 2.0 * se, (k, m, se, ed)
+
+
+def test_device_learned_moves_sample_the_ed_answer(trained, hip_lib):
+    """The same learned moves drawn, evaluated and undone on the device
+    (bafq_global_propose / bafq_global_finish) also reproduce ED."""
+    p, _, net, sigma, _ = trained
+    s = _solver(hip_lib, p, 1234)
+    prop = LearnedProposal(net, p, sigma, 0.5)
+    rng = np.random.default_rng(2028)
+    sums = {"density_total": np.zeros(NCH), "energy_density": np.zeros(NCH), "sf_K": np.zeros(NCH),
+            "dw_K": np.zeros(NCH)}
+    meas, accepted = 100, 0
+    for _ in range(meas):
+        accepted += int(GM.metropolis_global_device(s, net, prop.lam, prop.sigma, rng).sum())
+        s.sweep(1)
+        for c, b in enumerate(s.read_bin()):
+            for k in sums:
+                sums[k][c] += b[k].real if isinstance(b[k], complex) else b[k]
+    s.close()
+    rate = accepted / (NCH * meas)
+    print(f"device learned global move acceptance {rate:.3f}", flush=True)
+    assert rate > 0.1
+    names = {"density_total": "density_total", "energy_density": "energy_density", "sf_K": "S_SF_K",
+             "dw_K": "S_DW_K"}
+    for k, v in sums.items():
+        x = v / meas
+        m, se = float(np.mean(x)), float(np.std(x, ddof=1) / math.sqrt(NCH))
+        ed = FIXTURE[CASE]["observables"][names[k]]["ed"]
+        print(f"{k}: {m:.6g} +- {se:.2g}  ED {ed:.6g}  z {(m - ed) / se:+.2f}", flush=True)
+        assert abs(m - ed) <= 2.0 * se, (k, m, se, ed)
