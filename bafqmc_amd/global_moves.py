@@ -20,8 +20,12 @@ here are new, and the local sweep stays the sampler that they interleave with.
 A move: propose phi' for every chain, rebuild G(phi') from scratch (the
 stabilised B-product of bafq_prepare gives log|det G'|), accept with
 min(1, W'/W q(phi|phi') / q(phi'|phi)), and restore phi for rejected chains.
-The acceptance uniforms and the proposals come from a host generator, so the
-chains' own LCG streams are not touched.
+metropolis_global takes proposals made on the host (any proposal density);
+metropolis_global_device draws the learned proposal on the device and
+restores rejected chains from a device copy of their state.  The acceptance
+uniforms (and host proposals) come from a host generator, the device's normal
+numbers from a counter-based generator: the chains' own LCG streams are never
+touched.
 """
 from __future__ import annotations
 
