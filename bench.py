@@ -257,7 +257,8 @@ def run_learned(args, cfg, p, spec, nchains):
     progress(f"learned phase: {K} steps, lambda {lam}")
     t0 = time.perf_counter()
     for k in range(K):
-        accepted += int(GM.metropolis_global_device(s, net, prop.lam, prop.sigma, rng).sum())
+        for _ in range(args.moves):
+            accepted += int(GM.metropolis_global_device(s, net, prop.lam, prop.sigma, rng).sum())
         s.sweep(1)
         obs, st = s.read_bin_raw()
         sf1[:, k] = obs[:, 20] / obs[:, 32]
@@ -281,7 +282,8 @@ def run_learned(args, cfg, p, spec, nchains):
                              "stderr_sf_k": se0, "inverse_variance_per_s": e0},
                    "learned": {"ms_per_step": 1000.0 * wall1 / K, "tau_int_sf_k": tau1, "eff_samples_per_s": eff1,
                                "stderr_sf_k": se1, "inverse_variance_per_s": e1,
-                               "global_acceptance": accepted / (nchains * K), "lambda": lam,
+                               "global_acceptance": accepted / (nchains * K * args.moves), "lambda": lam,
+                               "moves_per_sweep": args.moves,
                                "lambda_trials": {str(k): v for k, v in trials.items()},
                                "network": {"layers": args.layers, "hidden": args.hidden, "params": net.nparam},
                                "train_s": t_train, "train_samples": len(xs), "heldout_r2": r2},
@@ -381,6 +383,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", default="auto",
                     help="'auto' (autotuner, before the timed region), 'default' (library choice) or 'PATH,DELAYED'")
+    ap.add_argument("--moves", type=int, default=1, help="learned global moves per sweep (learned mode)")
     ap.add_argument("--layers", type=int, default=3, help="convolution layers of the proposal network")
     ap.add_argument("--hidden", type=int, default=8, help="hidden channels of the proposal network")
     ap.add_argument("--mode", default="learned", choices=["plain", "learned"],
