@@ -17,7 +17,7 @@
 
 namespace bafq {
 
-enum KernelId { K_PREPARE = 0, K_SWEEP = 1, K_WARMUP = 2 };
+enum KernelId { K_PREPARE = 0, K_SWEEP = 1, K_WARMUP = 2, K_GRAD = 3 };
 enum PathId { P_GLOBAL = 0, P_LDS = 1 };
 
 // back-end hooks (0 on success)
@@ -102,6 +102,7 @@ struct bafq_handle {
     // learned global moves: old fields, forward drift, per-chain sums, flags, subset ids
     double *gm_old = nullptr, *gm_f0 = nullptr, *gm_out = nullptr, *gm_stats = nullptr;
     uint64_t* gm_state = nullptr;             // copy of every chain's matrices before the move
+    double* d_grad = nullptr;                 // d log W / d phi, laid out like the fields
     int* gm_accept = nullptr;
     size_t chain_block_bytes = 0;             // matrix block of chain 0 (with one model: of every chain)
     double last_ms = 0.0;
@@ -404,6 +405,7 @@ void bafq_destroy(bafq_handle* h) {
     if (!h) return;
     if (h->gm_old) bafq::rt_free(h->gm_old);
     if (h->gm_state) bafq::rt_free(h->gm_state);
+    if (h->d_grad) bafq::rt_free(h->d_grad);
     bafq::rt_release_launches(h->launches);
     if (h->dev) bafq::rt_free(h->dev);
     delete h;
@@ -454,6 +456,8 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
     a.nsweep = nsweep;
     a.measure = measure;
     a.queue = nullptr;
+    a.grad = h->d_grad;
+    a.phi0 = h->d_phi;
     double total_ms = 0.0;
     if (k == K_WARMUP) {
         a.ids = h->d_ids;
@@ -471,7 +475,7 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
             l.a.ids = h->d_path_ids[p];
             l.a.count = h->path_count[p];
             l.a.queue = h->d_queue + p;
-            l.grid = h->grid[k][p];
+            l.grid = h->grid[k == K_GRAD ? K_SWEEP : k][p];
             l.block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
             l.smem = h->max_lds[p];
         }
@@ -809,6 +813,21 @@ int bafq_global_finish(bafq_handle* h, const int32_t* accept) {
                             (size_t)S_LEN * nc))
         return fail(h, rt_error());
     return rt_sync() ? fail(h, rt_error()) : 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ gradient of log W
+extern "C" {
+
+int bafq_grad_log_weight(bafq_handle* h, double* grad) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (!grad) return fail(h, "null output");
+    if (!h->d_grad && rt_malloc((void**)&h->d_grad, sizeof(double) * h->phi_total)) return fail(h, rt_error());
+    if (launch_all(h, K_GRAD, 1, 0)) return 1;
+    if (rt_d2h(grad, h->d_grad, sizeof(double) * h->phi_total)) return fail(h, rt_error());
+    return 0;
 }
 
 }  // extern "C"

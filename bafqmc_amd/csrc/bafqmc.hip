@@ -40,6 +40,18 @@ __global__ void __launch_bounds__(512) k_sweep_lds(LaunchArgs a) {
     body_sweep<true>(c, a, (int*)smem);
 }
 
+__global__ void __launch_bounds__(512) k_grad(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem + kSlotBytes};
+    body_grad<false>(c, a, (int*)smem);
+}
+
+__global__ void __launch_bounds__(512) k_grad_lds(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem + kSlotBytes};
+    body_grad<true>(c, a, (int*)smem);
+}
+
 __global__ void __launch_bounds__(64) k_reset_queue(int* q) {
     if (threadIdx.x == 0) *q = 0;
 }
@@ -321,6 +333,7 @@ size_t rt_lds_limit() {
 
 static const void* kernel_of(KernelId k, PathId path) {
     if (k == K_PREPARE) return path == P_LDS ? (const void*)k_prepare_lds : (const void*)k_prepare;
+    if (k == K_GRAD) return path == P_LDS ? (const void*)k_grad_lds : (const void*)k_grad;
     return path == P_LDS ? (const void*)k_sweep_lds : (const void*)k_sweep;
 }
 

@@ -258,6 +258,9 @@ int rt_launch_paths(void**, KernelId k, const PathLaunch* pl, int n, double* ms)
             if (k == K_PREPARE) {
                 if (l.path == P_LDS) body_prepare<true>(c, l.a, slot);
                 else body_prepare<false>(c, l.a, slot);
+            } else if (k == K_GRAD) {
+                if (l.path == P_LDS) body_grad<true>(c, l.a, slot);
+                else body_grad<false>(c, l.a, slot);
             } else {
                 if (l.path == P_LDS) body_sweep<true>(c, l.a, slot);
                 else body_sweep<false>(c, l.a, slot);

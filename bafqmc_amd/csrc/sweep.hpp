@@ -86,6 +86,8 @@ struct LaunchArgs {
     int nsweep;
     int measure;
     int* queue;          // work counter of the launch (zeroed before it), see next_item
+    double* grad;        // K_GRAD: d log W / d phi, laid out like the fields starting at phi0
+    const double* phi0;
 };
 
 // Bytes at the start of a workgroup's dynamic LDS that hold the work-queue
@@ -200,6 +202,31 @@ DEV int nsec(const ModelDev& m) { return m.kind == 1 ? 2 : 1; }
 // Exponent sign of sector `sec` in channel nf (1: relative U1, 2: total U2).
 DEV double sec_sign(const ModelDev& m, int nf, int sec) {
     return (m.kind == 1 && nf == 2 && sec == 1) ? -1.0 : 1.0;
+}
+
+// d log W / d phi_{t,i,nf} from the Green's function at the point of the
+// local update of slice t, channel nf (where a field change e^{s alpha dphi}
+// of site i multiplies det(1 - B) by det(1 + Delta (1 - G_SS))):
+//   number conserving  -phi - 2 Re[alpha (1 - G_ii)]             (W ~ |det G|^2)
+//   pairing            -phi - Re sum_sec s alpha (1 - G_rr) [- alpha_2 for nf = 2]
+// (the normal-ordering constant of the U2 channel, W ~ |det G_2Ns|).
+template <class GP>
+DEV void grad_extract(const Ctx& c, const ModelDev& m, const double* phi_t, GP G, int t, int nf, double* grad) {
+    const int ns = m.ns, n = m.n;
+    const double cpow = m.kind == 1 ? 1.0 : 2.0;
+    const cd a = m.alpha[nf - 1];
+    for (int i = c.tid(); i < ns; i += c.nthreads()) {
+        double det = 0.0;
+        for (int sec = 0; sec < nsec(m); ++sec) {
+            const int r = sec * ns + i;
+            const cd g = ldc(G, r * n + r);
+            det += sec_sign(m, nf, sec) * (a.r * (1.0 - g.r) + a.i * g.i);
+        }
+        double v = -phi_t[i * 2 + nf - 1] - cpow * det;
+        if (m.kind == 1 && nf == 2) v -= m.alpha[1].r;
+        grad[((size_t)t * ns + i) * 2 + nf - 1] = v;
+    }
+    c.sync();
 }
 
 // f[r] = exp(s_r alpha_nf phi_r), finv = 1 / f, for all rows (LDS).
@@ -1144,7 +1171,7 @@ DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
 }
 
 DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
-                           bool meas) {
+                           bool meas, double* grad = nullptr) {
     const int n = m.n;
     Timer tm;
     live_identity(c, n, ch.ul, ch.dl, ch.vl);     // Wrap_L(Ltrot): L list slot Nst is the identity
@@ -1168,7 +1195,8 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
-            if (m.dk > 0) site_loop_delayed_glb(c, m, ch, L, ch.g, t, nf, false, seed);
+            if (grad) grad_extract(c, m, phi_t, as_global(ch.g), t, nf, grad);
+            else if (m.dk > 0) site_loop_delayed_glb(c, m, ch, L, ch.g, t, nf, false, seed);
             else site_loop(c, m, ch, L, ch.g, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
@@ -1311,7 +1339,7 @@ DEV void prepare_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
 }
 
 DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
-                        bool meas) {
+                        bool meas, double* grad = nullptr) {
     const int n = m.n;
     cd* G = L.mat;
     Timer tm;
@@ -1334,7 +1362,8 @@ DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
-            if (m.dk > 0) site_loop_delayed(c, m, ch, L, G, t, nf, false, seed);
+            if (grad) grad_extract(c, m, phi_t, as_lds(G), t, nf, grad);
+            else if (m.dk > 0) site_loop_delayed(c, m, ch, L, G, t, nf, false, seed);
             else site_loop_lds(c, m, ch, L, G, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
@@ -1423,6 +1452,28 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
         }
         if (kLds) global_from_lds(c, ch.g, L.mat, m.n);
         if (c.tid() == 0) *ch.seed = seed;
+        c.sync();
+    }
+}
+
+// d log W / d phi of every chain's current fields: one left sweep without
+// updates (the Green's function of every slice at the point of its local
+// updates), into a.grad at the chain's field offset.
+template <bool kLds>
+DEV void body_grad(const Ctx& c, const LaunchArgs& a, int* slot) {
+    for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
+        const ChainDev& ch = a.chains[a.ids[q]];
+        const ModelDev& m = a.models[ch.model];
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
+        int seed = *ch.seed;
+        double* grad = a.grad + (ch.phi - a.phi0);
+        if (kLds) {
+            lds_from_global(c, L.mat, ch.g, m.n);
+            sweep_left_lds(c, m, ch, L, seed, false, grad);
+            global_from_lds(c, ch.g, L.mat, m.n);
+        } else {
+            sweep_left_global(c, m, ch, L, seed, false, grad);
+        }
         c.sync();
     }
 }

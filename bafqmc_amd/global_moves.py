@@ -138,3 +138,17 @@ def metropolis_global_device(solver, net, lam, sigma, rng, seed=None):
     solver._check(lib.bafq_global_finish(solver.h, flags.ctypes.data_as(C.POINTER(C.c_int32))),
                   "bafq_global_finish")
     return accept
+
+
+def grad_log_weights(solver):
+    """d log W / d phi of every chain's current fields (bafq_grad_log_weight),
+    one array [Ltrot, Ns, 2] per chain."""
+    import ctypes as C
+    out = np.zeros(int(solver.field_offsets[-1]))
+    solver._check(solver.lib.bafq_grad_log_weight(solver.h, out.ctypes.data_as(C.POINTER(C.c_double))),
+                  "bafq_grad_log_weight")
+    res = []
+    for c in range(solver.nchains):
+        p = solver.params(c)
+        res.append(out[solver.field_offsets[c]:solver.field_offsets[c + 1]].reshape(p.ltrot, p.nlx * p.nly, 2))
+    return res

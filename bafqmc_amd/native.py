@@ -67,6 +67,7 @@ SYMBOLS = {
     "bafq_net_train": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, C.c_int, C.c_float, c_double_p]),
     "bafq_global_propose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, c_double_p, C.c_uint64, c_double_p]),
     "bafq_global_finish": (C.c_int, [C.c_void_p, c_int32_p]),
+    "bafq_grad_log_weight": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_get_green": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),

@@ -10,11 +10,13 @@ chain, 2 Ltrot measurements each) followed by reading the bin.
 Metric: effective independent S_SF(K) samples/s = sum over chains of
 (bins/s) / (2 tau_int), tau_int measured on the per-chain Re S_SF(K) bin
 series (each standardised, then pooled over chains).
-Default (--mode learned): the chains run K plain measured sweeps, the proposal
-network is trained on the displacements of 12 more, then the chains run K
-steps of one learned global move + one measured sweep; `value` is the
-learned form's rate, the plain rate, both error bars and their ratio at equal
-error bars are in `config`.  --mode plain reports the plain sweep alone.
+Default (--mode learned): the chains run K plain measured sweeps; the
+proposal network is trained on d log W / d phi of the fields of 12 more (the
+device's gradient pass); then the chains run K steps of 8 learned global
+moves (Metropolis-adjusted Langevin with the network's gradient, drawn and
+accepted on the device) + one measured sweep.  `value` is the learned form's
+rate; the plain rate, both error bars and their ratio at equal error bars are
+in `config`.  --mode plain reports the plain sweep alone.
 Chains whose stabilisation drift reaches the reference's "ortho unstable"
 threshold (5.5e-5, src/number_conserving/src/stabilization.f90:320,358) -- the
 bosonic weight drives some chains towards poles of (1 - B)^-1 -- are run but
@@ -223,12 +225,21 @@ def run_learned(args, cfg, p, spec, nchains):
     ntrain = max(1, min(12, 8192 // nchains + 1))
     for _ in range(ntrain):
         f0, _ = s.get_fields()
+        if args.target == "grad":
+            # chains near a pole of (1 - B)^-1 have gradients orders of magnitude
+            # above the rest: they are left out of the fit
+            for f, g in zip(f0, GM.grad_log_weights(s)):
+                if np.all(np.isfinite(g)) and np.max(np.abs(g)) < 50.0:
+                    xs.append(f)
+                    ts.append(g)
+            s.sweep(1, measure=False)
+            continue
         s.sweep(1, measure=False)
         f1, _ = s.get_fields()
         xs += f0
         ts += [b - a for a, b in zip(f0, f1)]
     s.read_bin()
-    progress(f"training the network on {len(xs)} sweep displacements")
+    progress(f"training the network on {len(xs)} " + ("gradients" if args.target == "grad" else "sweep displacements"))
     t1 = time.perf_counter()
     net = ConvNet(p.ltrot, p.nly, p.nlx, 2, args.hidden, args.layers, max_batch=min(512, nchains), seed=1)
     var_c, r2 = train_on_sweeps(net, to_net(xs, p), to_net(ts, p), 20, 3e-3, np.random.default_rng(0), batch=64)
@@ -236,10 +247,17 @@ def run_learned(args, cfg, p, spec, nchains):
     sigma = np.sqrt(var_c)
     progress(f"trained in {t_train:.1f} s, held-out R2 {r2:.4f}")
     rng = np.random.default_rng(2468)
-    # step size: the largest mean squared jump over a few trial moves
+    # step size: the largest mean squared jump over a few trial moves.  Sweep
+    # target: phi' = phi + lam f + sqrt(lam) sigma xi; gradient target (MALA
+    # with step h = 2 lam): phi' = phi + (h / 2) grad + sqrt(h) xi
+    if args.target == "grad":
+        sigma = np.array([math.sqrt(2.0), math.sqrt(2.0)])
+        lams = (0.005, 0.015, 0.05, 0.15)
+    else:
+        lams = (0.1, 0.25, 0.5, 1.0)
     fields, sd = s.get_fields()
     trials = {}
-    for lam in (0.1, 0.25, 0.5, 1.0):
+    for lam in lams:
         s.set_fields(fields, sd)
         s.prepare()
         prop = LearnedProposal(net, p, sigma, lam)
@@ -285,7 +303,8 @@ def run_learned(args, cfg, p, spec, nchains):
                                "global_acceptance": accepted / (nchains * K * args.moves), "lambda": lam,
                                "moves_per_sweep": args.moves,
                                "lambda_trials": {str(k): v for k, v in trials.items()},
-                               "network": {"layers": args.layers, "hidden": args.hidden, "params": net.nparam},
+                               "network": {"layers": args.layers, "hidden": args.hidden, "params": net.nparam,
+                                           "target": args.target},
                                "train_s": t_train, "train_samples": len(xs), "heldout_r2": r2},
                    "gain_eff_samples": eff1 / eff0, "gain_at_equal_error_bars": e1 / e0},
         "roofline": roofline(p, nchains, kernel_ms, args.config),
@@ -383,7 +402,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", default="auto",
                     help="'auto' (autotuner, before the timed region), 'default' (library choice) or 'PATH,DELAYED'")
-    ap.add_argument("--moves", type=int, default=1, help="learned global moves per sweep (learned mode)")
+    ap.add_argument("--moves", type=int, default=8, help="learned global moves per sweep (learned mode)")
+    ap.add_argument("--target", default="grad", choices=["sweep", "grad"],
+                    help="what the network learns: the one-sweep displacement (a Langevin move of the sweep's "
+                         "own scale) or d log W / d phi (a Metropolis-adjusted Langevin move, step h)")
     ap.add_argument("--layers", type=int, default=3, help="convolution layers of the proposal network")
     ap.add_argument("--hidden", type=int, default=8, help="hidden channels of the proposal network")
     ap.add_argument("--mode", default="learned", choices=["plain", "learned"],

@@ -186,6 +186,13 @@ int bafq_global_propose(bafq_handle* h, bafq_net* net, double lam, const double*
                         double* out);
 int bafq_global_finish(bafq_handle* h, const int32_t* accept);
 
+/* d log W / d phi for every field of every chain (laid out like the fields
+ * of bafq_set_fields; W the weight of the global moves), from one left sweep
+ * without updates: at each slice and channel, the diagonal of the Green's
+ * function at the point of its local updates.  The chains' fields and RNG
+ * streams are not changed.  No counterpart in the reference. */
+int bafq_grad_log_weight(bafq_handle* h, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -132,3 +132,34 @@ def test_global_moves_sample_the_ed_answer(hip_lib):
         ed = case["observables"][names[k]]["ed"]
         print(f"{k}: {m:.6g} +- {se:.2g}  ED {ed:.6g}  z {(m - ed) / se:+.2f}", flush=True)
         assert abs(m - ed) <= 2.0 * se, (k, m, se, ed)
+
+
+@pytest.mark.parametrize("pairing,path", [(False, -1), (True, -1), (True, 0)])
+def test_grad_log_weight_central_differences(emu_lib, pairing, path):
+    """bafq_grad_log_weight against central differences of log W (each from a
+    fresh rebuild of G), on both channels and on both paths."""
+    d = dict(pairing=pairing, u1=0.8, u2=-0.3, mu=-4.0, rdelta=0.4 if pairing else 0.0, nlx=3, nly=3, ltrot=8,
+             beta=1.0, nwrap=4, shift_loc=1.0)
+    p = solver_params(d)
+    rng = np.random.default_rng(31)
+    phis = [rng.normal(size=(8, 9, 2)) * 0.6 for _ in range(2)]
+    s = BatchedSolver([ModelSpec(p)], [0, 0], lib=emu_lib, schedule=[(path, -1)])
+    s.set_fields(phis, [3, 4])
+    s.prepare()
+    s.sweep(1, measure=False)                 # the pass must also work after a sweep
+    fields, seeds = s.get_fields()
+    grads = GM.grad_log_weights(s)
+    after, seeds2 = s.get_fields()
+    assert all(np.array_equal(a, b) for a, b in zip(fields, after)) and list(seeds) == list(seeds2)
+    h = 1e-5
+    for (t, i, nf) in ((0, 0, 0), (3, 4, 1), (7, 8, 1), (5, 2, 0)):
+        lw = []
+        for sgn in (1.0, -1.0):
+            f2 = [f.copy() for f in fields]
+            f2[1][t, i, nf] += sgn * h
+            s.set_fields(f2, seeds)
+            s.prepare()
+            lw.append(GM.log_weights(s)[1])
+        fd = (lw[0] - lw[1]) / (2 * h)
+        assert abs(fd - grads[1][t, i, nf]) < 1e-6 * max(1.0, abs(fd)), ((t, i, nf), fd, grads[1][t, i, nf])
+    s.close()
