@@ -175,3 +175,84 @@ def test_device_learned_moves_sample_the_ed_answer(trained, hip_lib):
         ed = FIXTURE[CASE]["observables"][names[k]]["ed"]
         print(f"{k}: {m:.6g} +- {se:.2g}  ED {ed:.6g}  z {(m - ed) / se:+.2f}", flush=True)
         assert abs(m - ed) <= 2.0 * se, (k, m, se, ed)
+
+
+@pytest.fixture(scope="module")
+def trained_grad(hip_lib):
+    """The network trained on d log W / d phi (bafq_grad_log_weight) of the
+    plain sampler's fields."""
+    p = _params()
+    s = _solver(hip_lib, p, 5151)
+    xs, ts = [], []
+    for _ in range(12):
+        f0, _ = s.get_fields()
+        for f, g in zip(f0, GM.grad_log_weights(s)):
+            if np.all(np.isfinite(g)) and np.max(np.abs(g)) < 50.0:
+                xs.append(f)
+                ts.append(g)
+        s.sweep(1, measure=False)
+    s.read_bin()
+    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, 8, 3, max_batch=256, seed=2, lib=hip_lib)
+    _, r2 = train_on_sweeps(net, to_net(xs, p), to_net(ts, p), 30, 3e-3, np.random.default_rng(1), batch=64)
+    print(f"gradient network: held-out R2 {r2:.4f} on {len(xs)} samples", flush=True)
+    yield p, s, net, r2
+    s.close()
+    net.close()
+
+
+def test_gradient_network_mala_gain(trained_grad):
+    """Metropolis-adjusted Langevin moves with the learned gradient at step h:
+    the network's drift gives a far higher acceptance and jump than the same
+    step without it (a Gaussian random walk of width sqrt(h))."""
+    p, s, net, r2 = trained_grad
+    assert r2 > 0.9
+    fields, seeds = s.get_fields()
+    rng = np.random.default_rng(17)
+    h = 0.1
+    out = {}
+    for name, use in (("no network", None), ("learned gradient", net)):
+        s.set_fields(fields, seeds)
+        s.prepare()
+        prop = LearnedProposal(use, p, math.sqrt(2.0), h / 2)
+        acc_n, jump = 0, 0.0
+        for _ in range(4):
+            f, _ = s.get_fields()
+            acc = GM.metropolis_global_device(s, use, prop.lam, prop.sigma, rng)
+            g, _ = s.get_fields()
+            acc_n += int(acc.sum())
+            jump += sum(float(np.sum((a - b) ** 2)) for a, b in zip(g, f))
+        out[name] = (acc_n / (4 * NCH), jump / (4 * NCH * fields[0].size))
+        print(f"MALA step {h}, {name}: acceptance {out[name][0]:.3f}, jump {out[name][1]:.3g}", flush=True)
+    s.set_fields(fields, seeds)
+    s.prepare()
+    assert out["learned gradient"][0] > 0.3
+    assert out["learned gradient"][1] >= 10.0 * max(out["no network"][1], 1e-12)
+
+
+def test_gradient_network_moves_sample_the_ed_answer(trained_grad, hip_lib):
+    p, _, net, _ = trained_grad
+    s = _solver(hip_lib, p, 4321)
+    prop = LearnedProposal(net, p, math.sqrt(2.0), 0.05)
+    rng = np.random.default_rng(2029)
+    sums = {"density_total": np.zeros(NCH), "energy_density": np.zeros(NCH), "sf_K": np.zeros(NCH),
+            "dw_K": np.zeros(NCH)}
+    meas, accepted = 100, 0
+    for _ in range(meas):
+        for _ in range(4):
+            accepted += int(GM.metropolis_global_device(s, net, prop.lam, prop.sigma, rng).sum())
+        s.sweep(1)
+        for c, b in enumerate(s.read_bin()):
+            for k in sums:
+                sums[k][c] += b[k].real if isinstance(b[k], complex) else b[k]
+    s.close()
+    rate = accepted / (NCH * meas * 4)
+    print(f"gradient-network MALA acceptance {rate:.3f}", flush=True)
+    assert rate > 0.3
+    names = {"density_total": "density_total", "energy_density": "energy_density", "sf_K": "S_SF_K",
+             "dw_K": "S_DW_K"}
+    for k, v in sums.items():
+        x = v / meas
+        m, se = float(np.mean(x)), float(np.std(x, ddof=1) / math.sqrt(NCH))
+        ed = FIXTURE[CASE]["observables"][names[k]]["ed"]
+        print(f"{k}: {m:.6g} +- {se:.2g}  ED {ed:.6g}  z {(m - ed) / se:+.2f}", flush=True)
+        assert abs(m - ed) <= 2.0 * se, (k, m, se, ed)
