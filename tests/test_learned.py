@@ -104,6 +104,8 @@ def test_net_rejects_bad_batch(emu_lib):
 def test_net_gpu(hip_lib):
     _check_net(hip_lib)
     _check_net(hip_lib, dict(lt=40, ly=3, lx=3, channels=2, hidden=8, layers=3), batch=4)
+    # 4 layers x 32 channels: the forward kernel's widest register blocking
+    _check_net(hip_lib, dict(lt=6, ly=3, lx=4, channels=2, hidden=32, layers=4), batch=3)
 
 
 def _device_move_case(lib, pairing):
