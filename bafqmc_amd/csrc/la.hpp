@@ -568,24 +568,42 @@ DEV cd refl_entry(LDS_AS const cd* A, int n, int r, int k) {
 // x <- (I - t v v^H) x, v = reflector k stored below the diagonal of A.
 template <int R>
 DEV void refl_apply_col(const Ctx& c, cd (&x)[R], LDS_AS const cd* A, int n, int k, cd t, int gl) {
-    // slots whose four rows all lie above row k or beyond row n - 1 are skipped
-    // (k and n are uniform, so these are scalar branches)
-    // The reflector is read twice from LDS rather than held in registers.
+    // Slots whose four rows all lie above row k or beyond row n - 1 are skipped
+    // (k and n are uniform, so these are scalar branches).  The loads go in
+    // batches of kB slots issued together (one LDS round trip per batch, not
+    // per slot); a batch may read a slot that is then skipped.  The reflector
+    // is read twice from LDS rather than held in registers.
+    constexpr int kB = R % 6 == 0 ? 6 : (R % 4 == 0 ? 4 : 1);
+    auto needed = [&](int i) { return kColGroup * i + kColGroup - 1 >= k && kColGroup * i < n; };
     double wr = 0.0, wi = 0.0;
 #pragma unroll
-    for (int i = 0; i < R; ++i) {
-        if (kColGroup * i + kColGroup - 1 >= k && kColGroup * i < n) {
-            const cd v = refl_entry(A, n, gl + kColGroup * i, k);
-            wr += v.r * x[i].r + v.i * x[i].i;
-            wi += v.r * x[i].i - v.i * x[i].r;
+    for (int i0 = 0; i0 < R; i0 += kB) {
+        if (kColGroup * (i0 + kB) - 1 >= k && kColGroup * i0 < n) {   // the batch holds a needed slot
+            cd v[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) v[j] = refl_entry(A, n, gl + kColGroup * (i0 + j), k);
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                if (needed(i0 + j)) {
+                    wr += v[j].r * x[i0 + j].r + v[j].i * x[i0 + j].i;
+                    wi += v[j].r * x[i0 + j].i - v[j].i * x[i0 + j].r;
+                }
+            }
         }
     }
     const cd s = t * cd{c.quad_sum(wr), c.quad_sum(wi)};
     reload_fence();
 #pragma unroll
-    for (int i = 0; i < R; ++i)
-        if (kColGroup * i + kColGroup - 1 >= k && kColGroup * i < n)
-            x[i] = x[i] - refl_entry(A, n, gl + kColGroup * i, k) * s;
+    for (int i0 = 0; i0 < R; i0 += kB) {
+        if (kColGroup * (i0 + kB) - 1 >= k && kColGroup * i0 < n) {   // the batch holds a needed slot
+            cd v[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) v[j] = refl_entry(A, n, gl + kColGroup * (i0 + j), k);
+#pragma unroll
+            for (int j = 0; j < kB; ++j)
+                if (needed(i0 + j)) x[i0 + j] = x[i0 + j] - v[j] * s;
+        }
+    }
 }
 
 // ZLARFG scalars of the quad's register column as pivot of step k: alpha = x_k,
@@ -619,37 +637,32 @@ DEV double col_candidate(const Ctx& c, const cd (&x)[R], int n, int k, int gl, d
     return ar * ar + ai * ai + x2;
 }
 
-// Largest pivoting norm among the wavefront's active columns (ties to the
-// lower column), published in rec[2 w], rec[2 w + 1] for the next pivot search.
-DEV void publish_best(const Ctx& c, LDS_AS double* rec, bool active, double a2, int col, int n) {
-    // a NaN norm (a chain that has blown up) wins, so every step has an owner
-    double bv = active ? (a2 == a2 ? a2 : INFINITY) : -1.0;
-    int bi = active ? col : n;
-    c.wave_argmax(bv, bi);
-    if (c.lane() == 0) {
-        rec[2 * c.wave()] = bv;
-        rec[2 * c.wave() + 1] = (double)bi;
-    }
+// Pivoting norm of each original column for the next pivot search: -1 once
+// the column has been chosen; a NaN norm (a chain that has blown up) becomes
+// +inf and wins, so every step has an owner.
+DEV void publish_norm(LDS_AS double* nrm, bool active, double a2, int col, int gl, int n) {
+    if (gl == 0 && col < n) nrm[col] = active ? (a2 == a2 ? a2 : INFINITY) : -1.0;
 }
 
 // Householder QR with column pivoting (ZGEQP3 semantics, exact column norms)
 // with the same outputs as qrp().  Quad q owns original column q for the whole
 // factorization and keeps, besides the column, the ZLARFG scalars it would use
-// as the next pivot.  Step k: every wavefront picks the pivot from the
-// per-wavefront best norms of step k - 1 (ties to the lower original column);
-// the owner writes column k of the result (R above the diagonal, beta, the
-// reflector below) -- barrier -- the other quads apply H(k)^H to their columns,
-// compute their next norms and scalars, and each wavefront publishes its best
-// norm -- barrier.
+// as the next pivot.  Step k: every wavefront that holds columns picks the
+// pivot from the norms published in step k - 1 (the largest, ties to the lower
+// original column, as IDAMAX); the owner writes column k of the result (R above
+// the diagonal, beta, the reflector below) -- barrier -- the other quads apply
+// H(k)^H to their columns, compute their next norms and scalars, and publish
+// the norm -- barrier.  One LDS slot per column replaces a per-wavefront
+// reduction before the barrier, so each step runs one wavefront argmax.
 template <int R>
 DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, double* D_) {
     LDS_AS cd* tau = as_lds(qs.tau);
     LDS_AS int* perm = as_lds(qs.perm);
-    LDS_AS double* rec = as_lds(qs.bcast);      // 2 doubles per wavefront (<= 8)
+    LDS_AS double* nrm = as_lds(qs.nrm);        // pivoting norm per column (n <= 96)
     LDS_AS double* D = as_lds(D_);
     const int col = c.tid() / kColGroup, gl = c.tid() % kColGroup;
     const int colc = col < n ? col : n - 1;
-    const int nw = c.nwaves();
+    const bool holds = c.wave() * (64 / kColGroup) < n;   // the wavefront holds columns (uniform)
     bool active = col < n;
     cd x[R];
 #pragma unroll
@@ -662,15 +675,24 @@ DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, doub
     double beta;
     cd t, scale;
     double a2 = col_candidate<R>(c, x, n, 0, gl, beta, t, scale);
-    publish_best(c, rec, active, a2, col, n);
+    publish_norm(nrm, active, a2, col, gl, n);
     if (c.tid() < n) perm[c.tid()] = c.tid();   // defined even if a step had no owner
     c.sync();                                    // columns in registers: A may be overwritten
 #pragma unroll 1
     for (int k = 0; k < n; ++k) {
-        // ---- pivot from the per-wavefront records
-        double bv = c.lane() < nw ? rec[2 * c.lane()] : -1.0;
-        int bi = c.lane() < nw ? (int)rec[2 * c.lane() + 1] : n;
-        c.wave_argmax(bv, bi);
+        // ---- pivot from the published norms: lane l looks at columns l, l + 64
+        int bi = -1;                              // no owner in a wavefront without columns
+        if (holds) {
+            const int l0 = c.lane(), l1 = c.lane() + 64;
+            double bv = l0 < n ? nrm[l0] : -1.0;
+            bi = l0;
+            const double v1 = l1 < n ? nrm[l1] : -1.0;
+            if (v1 > bv) {
+                bv = v1;
+                bi = l1;
+            }
+            c.wave_argmax(bv, bi);
+        }
         if (col == bi) {
             // column k of the result: R above the diagonal (normalised at the
             // end), beta on it, the reflector below
@@ -695,7 +717,7 @@ DEV void qrp_cols_t(const Ctx& c, LDS_AS cd* A, int n, const QrScratch& qs, doub
                 refl_apply_col<R>(c, x, A, n, k, csel(active, conj(tk), cd{0.0, 0.0}), gl);
                 a2 = col_candidate<R>(c, x, n, k + 1, gl, beta, t, scale);
             }
-            publish_best(c, rec, active, a2, col, n);
+            publish_norm(nrm, active, a2, col, gl, n);
             c.sync();
         }
     }
