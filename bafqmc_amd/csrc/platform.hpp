@@ -97,6 +97,9 @@ struct HipCtx {
     }
     DEV double lane_bcast(double v, int l) const { return readlane_d(v, l); }
     DEV bool wave_any(bool p) const { return __any(p); }
+    // A value that is the same in every lane, moved to a scalar register so that
+    // branches on it are scalar branches.
+    DEV static int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
     // Wavefront argmax of (v, i), ties to the smaller i; the result is uniform.
     DEV void wave_argmax(double& v, int& i) const {
         argmax_step(v, i, dpp<0xB1>(v), dpp<0xB1>(i));
@@ -190,6 +193,7 @@ struct EmuCtx {
         wsync();
         return r;
     }
+    static int uniform(int v) { return v; }
     bool wave_any(bool p) const {
         double v = p ? 1.0 : 0.0;
         for (int m = 1; m < 64; m <<= 1) v += shfl_xor(v, m);

@@ -456,6 +456,8 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 // one wavefront); G in LDS or in device memory, U and W in LDS.
 template <class GP, class UP>
 DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols) {
+    n = c.uniform(n);                 // scalar tests of the k-steps taken
+    cols = c.uniform(cols);
     const int nt = (n + 15) >> 4;
     const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
     for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
