@@ -286,6 +286,8 @@ def run_learned(args, cfg, p, spec, nchains):
     stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf0), axis=1) & np.all(np.isfinite(sf1), axis=1)
     tau0, eff0, e0, se0 = _efficiency(sf0, wall0, stable)
     tau1, eff1, e1, se1 = _efficiency(sf1, wall1, stable)
+    if args.dump:
+        np.savez(args.dump, sf_plain=sf0, sf_learned=sf1, stable=stable, wall_plain=wall0, wall_learned=wall1)
     out = {
         "metric": METRIC,
         "value": eff1, "unit": "effective S_SF(K) samples/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
@@ -408,6 +410,8 @@ def main():
                          "own scale) or d log W / d phi (a Metropolis-adjusted Langevin move, step h)")
     ap.add_argument("--layers", type=int, default=3, help="convolution layers of the proposal network")
     ap.add_argument("--hidden", type=int, default=8, help="hidden channels of the proposal network")
+    ap.add_argument("--dump", default="",
+                    help="learned mode: also write the per-chain S_SF(K) series of both phases to this .npz")
     ap.add_argument("--mode", default="learned", choices=["plain", "learned"],
                     help="plain: the batched local sweep; learned: plain sweeps, then the network trained on "
                          "their displacements, then sweeps interleaved with learned global moves, compared "
