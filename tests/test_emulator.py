@@ -17,6 +17,7 @@ from tests.test_gpu_parity import CASES
 from bafqmc_amd.model import ModelSpec
 from bafqmc_amd.solver import BatchedSolver
 from tests.test_gpu_parity import (check_sweep_vs_oracle, check_golden, check_mfma_layout, check_lane_ops, check_qrp,
+                                   check_qrp_ties,
                                    check_uneven_batch, check_gaussian, GOLDEN)
 
 
@@ -39,6 +40,11 @@ def test_emu_gemm(emu_lib):
 @pytest.mark.parametrize("n,variant", [(9, 0), (18, 0), (33, 0), (150, 0), (9, 1), (18, 1), (32, 1)])
 def test_emu_qrp(emu_lib, n, variant):
     check_qrp(emu_lib, n, variant)
+
+
+@pytest.mark.parametrize("n,variant", [(16, 0), (16, 1), (72, 1)])
+def test_emu_qrp_ties(emu_lib, n, variant):
+    check_qrp_ties(emu_lib, n, variant)
 
 
 @pytest.mark.parametrize("case", range(8))

@@ -108,6 +108,51 @@ def test_device_qrp(hip_lib, n, variant):
     check_qrp(hip_lib, n, variant)
 
 
+def tied_matrices(n):
+    """Inputs whose column norms tie exactly (the identity, a diagonal with
+    repeated magnitudes in scrambled order) and a matrix of repeated columns
+    perturbed at 1e-9, numerically of rank 4.  (Exactly repeated columns make
+    |R_kk| = 0, and V = D^-1 R is not finite, in the reference as here.)"""
+    rng = np.random.default_rng(7 + n)
+    vals = np.array([1.0, 3.0, 3.0, 0.5, 2.0, 2.0, 3.0, 1.0])[np.arange(n) % 8]
+    diag = np.diag(vals * np.exp(2j * np.pi * rng.random(n)))[:, rng.permutation(n)]
+    base = rng.normal(size=(n, 4)) + 1j * rng.normal(size=(n, 4))
+    noise = 1e-9 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    return {"identity": np.eye(n, dtype=complex), "diagonal": diag, "repeated": np.tile(base, (1, n // 4)) + noise}
+
+
+def check_qrp_ties(lib, n, variant):
+    """Pivoted QR on exactly tied column norms: a valid factorization A P = Q R
+    with nonincreasing |R_kk| equal to LAPACK's (ZGEQP3 breaks a tie to the
+    lowest current position, these routines to the lowest original column, so
+    the pivot order itself may differ on ties)."""
+    import scipy.linalg as sla
+    for name, a in tied_matrices(n).items():
+        r, tau, perm, d = lib.debug_qrp(a, variant)
+        R = np.triu(r) * d[:, None]
+        Q = np.eye(n, dtype=complex)
+        for k in range(n - 1, -1, -1):
+            v = np.zeros(n, complex)
+            v[k] = 1.0
+            v[k + 1:] = r[k + 1:, k]
+            Q = (np.eye(n) - tau[k] * np.outer(v, v.conj())) @ Q
+        assert sorted(perm.tolist()) == list(range(n)), name
+        assert np.max(np.abs(Q @ R - a[:, perm])) < 1e-13 * max(1.0, np.abs(a).max()), name
+        assert np.max(np.abs(Q.conj().T @ Q - np.eye(n))) < 1e-13, name
+        if name != "repeated":           # full rank: |R_kk| are the sorted magnitudes
+            assert np.all(np.diff(d) <= 1e-13 * d[0]), name
+            _, rl, _ = sla.qr(a, pivoting=True)
+            np.testing.assert_allclose(d, np.abs(np.diag(rl)), rtol=1e-13, err_msg=name)
+        else:                            # numerically rank 4: the first four carry the matrix
+            assert np.all(np.diff(d) <= 1e-12 * d[0]), name
+            assert d[3] > 1e-3 * d[0] and np.max(d[4:]) < 1e-7 * d[0], name
+
+
+@pytest.mark.parametrize("n,variant", [(16, 0), (72, 0), (16, 1), (72, 1), (88, 1)])
+def test_device_qrp_ties(hip_lib, n, variant):
+    check_qrp_ties(hip_lib, n, variant)
+
+
 CASES = [
     dict(pairing=False, u1=0.7, u2=-0.1, mu=-5.0, nlx=3, nly=3, ltrot=6, beta=0.6, nwrap=2, shift_loc=0.0),
     dict(pairing=False, u1=0.7, u2=-0.1, mu=-5.0, nlx=3, nly=3, ltrot=6, beta=0.6, nwrap=2, shift_loc=1.5),
