@@ -250,11 +250,10 @@ constexpr int kGroup = 8;   // lanes cooperating on one column
 // once; v comes from LDS.  With NG = kGroup the arithmetic and its order are
 // those of the two-pass loop of qrp_apply_loop.
 template <int R, int NG>
-DEV void qrp_apply_regs(const Ctx& c, cd* A, int n, int k, const QrScratch& s, cd ctau) {
+DEV void qrp_apply_regs(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, LDS_AS double* nrm, cd ctau) {
     constexpr int kGroup = NG;
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
-    auto v = as_lds(s.v);
     for (int jb = k + 1; jb < n; jb += ngrp) {
         const int j = jb + grp;
         const bool ok = j < n;
@@ -290,14 +289,13 @@ DEV void qrp_apply_regs(const Ctx& c, cd* A, int n, int k, const QrScratch& s, c
             }
         }
         a = group_sum(c, a, kGroup);
-        if (ok && gl == 0) s.nrm[j] = a;
+        if (ok && gl == 0) nrm[j] = a;
     }
 }
 
-DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, const QrScratch& s, cd ctau) {
+DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, LDS_AS double* nrm, cd ctau) {
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
-    auto v = as_lds(s.v);
     for (int jb = k + 1; jb < n; jb += ngrp) {
         const int j = jb + grp;
         const bool ok = j < n;
@@ -321,7 +319,7 @@ DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, const QrScratch& s, c
             }
         }
         a = group_sum(c, a, kGroup);
-        if (ok && gl == 0) s.nrm[j] = a;
+        if (ok && gl == 0) nrm[j] = a;
     }
 }
 
@@ -332,6 +330,14 @@ DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, const QrScratch& s, c
 // factors with H(k) = I - tau_k v_k v_k^H, Q = H(0) H(1) ... H(n-1).
 // Column norms are recomputed exactly after every step (no downdating).
 DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
+    // the scratch arrays are in LDS: typed pointers held in registers, so that
+    // stores through A (which may alias any generic address) do not force
+    // their reloading
+    LDS_AS double* const nrm = as_lds(s.nrm);
+    LDS_AS int* const perm = as_lds(s.perm);
+    LDS_AS double* const bcast = as_lds(s.bcast);
+    LDS_AS cd* const tau = as_lds(s.tau);
+    LDS_AS cd* const vs = as_lds(s.v);
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     // initial squared column norms
@@ -342,8 +348,8 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
             for (int r = gl; r < n; r += kGroup) a += abs2(A[r * n + j]);
         a = group_sum(c, a, kGroup);
         if (j < n && gl == 0) {
-            s.nrm[j] = a;
-            s.perm[j] = j;
+            nrm[j] = a;
+            perm[j] = j;
         }
     }
     c.sync();
@@ -356,7 +362,7 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
                 const int j = jb + c.lane();
                 if (j < n) {
                     // a NaN norm (a chain that has blown up) wins, so the pivot is always a column
-                    const double v = s.nrm[j] == s.nrm[j] ? s.nrm[j] : INFINITY;
+                    const double v = nrm[j] == nrm[j] ? nrm[j] : INFINITY;
                     if (v > best) { best = v; bi = j; }
                 }
             }
@@ -365,10 +371,10 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
                 const int oi = c.shfl_xor_i(bi, m);
                 if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
             }
-            if (c.lane() == 0) s.bcast[0] = (double)bi;
+            if (c.lane() == 0) bcast[0] = (double)bi;
         }
         c.sync();
-        const int p = (int)s.bcast[0];
+        const int p = (int)bcast[0];
         if (p != k) {
             for (int r = tid; r < n; r += nth) {
                 const cd t = A[r * n + k];
@@ -376,8 +382,8 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
                 A[r * n + p] = t;
             }
             if (tid == 0) {
-                const int ti = s.perm[k]; s.perm[k] = s.perm[p]; s.perm[p] = ti;
-                const double tn = s.nrm[k]; s.nrm[k] = s.nrm[p]; s.nrm[p] = tn;
+                const int ti = perm[k]; perm[k] = perm[p]; perm[p] = ti;
+                const double tn = nrm[k]; nrm[k] = nrm[p]; nrm[p] = tn;
             }
         }
         c.sync();
@@ -402,32 +408,32 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
                     ti = -alpha.i / beta;
                     scale = cinv(alpha - cd{beta, 0.0});
                 }
-                s.tau[k] = cd{tr, ti};
-                s.bcast[1] = scale.r;
-                s.bcast[2] = scale.i;
-                s.bcast[3] = beta;
+                stc(tau, k, cd{tr, ti});
+                bcast[1] = scale.r;
+                bcast[2] = scale.i;
+                bcast[3] = beta;
             }
         }
         c.sync();
         {
             // the reflector, scaled, into A below the diagonal and into LDS
-            const cd scale = cd{s.bcast[1], s.bcast[2]};
+            const cd scale = cd{bcast[1], bcast[2]};
             for (int r = k + 1 + tid; r < n; r += nth) {
                 const cd x = A[r * n + k] * scale;
                 A[r * n + k] = x;
-                s.v[r] = x;
+                stc(vs, r, x);
             }
             if (tid == 0) {
-                A[k * n + k] = cd{s.bcast[3], 0.0};
-                s.v[k] = cd{1.0, 0.0};
+                A[k * n + k] = cd{bcast[3], 0.0};
+                stc(vs, k, cd{1.0, 0.0});
             }
         }
         c.sync();
         // ---- apply H(k)^H = I - conj(tau) v v^H to columns j > k, new norms
-        const cd ctau = conj(s.tau[k]);
-        if (n <= 18 * kGroup) qrp_apply_regs<18, kGroup>(c, A, n, k, s, ctau);
-        else if (n <= 18 * 2 * kGroup) qrp_apply_regs<18, 2 * kGroup>(c, A, n, k, s, ctau);
-        else qrp_apply_loop(c, A, n, k, s, ctau);
+        const cd ctau = conj(ldc(tau, k));
+        if (n <= 18 * kGroup) qrp_apply_regs<18, kGroup>(c, A, n, k, vs, nrm, ctau);
+        else if (n <= 18 * 2 * kGroup) qrp_apply_regs<18, 2 * kGroup>(c, A, n, k, vs, nrm, ctau);
+        else qrp_apply_loop(c, A, n, k, vs, nrm, ctau);
         c.sync();
     }
     // ---- D and row normalisation of R
