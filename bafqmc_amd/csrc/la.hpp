@@ -77,8 +77,12 @@ DEV double wave_sum(const Ctx& c, double v) {
     for (int m = 32; m >= 1; m >>= 1) v += c.shfl_xor(v, m);
     return v;
 }
-// Sum over an aligned group of `gs` lanes (gs a power of two <= 64).
+// Sum over an aligned group of `gs` lanes (gs a power of two <= 64); groups
+// of 8 and 16 reduce with DPP lane permutations (no LDS round trip), which
+// need every lane of the wavefront active.
 DEV double group_sum(const Ctx& c, double v, int gs) {
+    if (gs == 8) return c.oct_sum(v);
+    if (gs == 16) return c.row16_sum(v);
     for (int m = gs >> 1; m >= 1; m >>= 1) v += c.shfl_xor(v, m);
     return v;
 }
@@ -454,6 +458,54 @@ DEV cd refl(const cd* A, int n, int r, int k) {
     return r == k ? cd{1.0, 0.0} : A[r * n + k];
 }
 
+// X[:, j0:] <- (I - t v v^H) X[:, j0:], v = reflector k of A (unit at row k,
+// A[r][k] below it), X in device memory.  A group of NG lanes owns a column
+// and holds its rows k + gl + NG i (slot i, n - k <= NG R) in registers from
+// the dot product to the update; the reflector's slots and the column's are
+// each loaded in one batch (clamped rows, masked values).  The arithmetic and
+// its order are those of the row loops of form_q and apply_qh_left.
+template <int R, int NG>
+DEV void refl_apply_glb(const Ctx& c, cd* X, const cd* A, int n, int k, cd t, int j0) {
+    const int tid = c.tid(), nth = c.nthreads();
+    const int grp = tid / NG, gl = tid % NG, ngrp = nth / NG;
+    cd v[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = k + gl + NG * i, rc = r < n ? r : n - 1;
+        const cd a = A[rc * n + k];
+        v[i] = r == k ? cd{1.0, 0.0} : (r < n ? a : cd{0.0, 0.0});
+    }
+    for (int jb = j0; jb < n; jb += ngrp) {
+        const int j = jb + grp;
+        const bool ok = j < n;
+        const int jc = ok ? j : n - 1;
+        cd x[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = k + gl + NG * i, rc = r < n ? r : n - 1;
+            const cd a = X[rc * n + jc];
+            x[i] = r < n ? a : cd{0.0, 0.0};
+        }
+        double wr = 0.0, wi = 0.0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            if (k + gl + NG * i < n) {
+                const cd p2 = conj(v[i]) * x[i];
+                wr += p2.r;
+                wi += p2.i;
+            }
+        }
+        wr = group_sum(c, wr, NG);
+        wi = group_sum(c, wi, NG);
+        const cd s = t * cd{wr, wi};
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = k + gl + NG * i;
+            if (ok && r < n) X[r * n + j] = x[i] - v[i] * s;
+        }
+    }
+}
+
 // Q = H(0) ... H(n-1) formed explicitly (ZUNGQR).
 DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n) {
     mat_identity(c, Q, n);
@@ -461,6 +513,16 @@ DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n) {
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     for (int k = n - 1; k >= 0; --k) {
         const cd tk = tau[k];
+        if (n <= 18 * kGroup) {
+            refl_apply_glb<18, kGroup>(c, Q, A, n, k, tk, k);
+            c.sync();
+            continue;
+        }
+        if (n <= 18 * 2 * kGroup) {
+            refl_apply_glb<18, 2 * kGroup>(c, Q, A, n, k, tk, k);
+            c.sync();
+            continue;
+        }
         for (int jb = k; jb < n; jb += ngrp) {
             const int j = jb + grp;
             const bool ok = j < n;
@@ -487,6 +549,16 @@ DEVFN void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n)
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     for (int k = 0; k < n; ++k) {
         const cd tk = conj(tau[k]);
+        if (n <= 18 * kGroup) {
+            refl_apply_glb<18, kGroup>(c, X, A, n, k, tk, 0);
+            c.sync();
+            continue;
+        }
+        if (n <= 18 * 2 * kGroup) {
+            refl_apply_glb<18, 2 * kGroup>(c, X, A, n, k, tk, 0);
+            c.sync();
+            continue;
+        }
         for (int jb = 0; jb < n; jb += ngrp) {
             const int j = jb + grp;
             const bool ok = j < n;

@@ -80,6 +80,12 @@ struct HipCtx {
         v += dpp<0xB1>(v);
         return v + dpp<0x4E>(v);
     }
+    // Sum over the aligned group of 8 lanes; all eight lanes get the same bits.
+    DEV double oct_sum(double v) const {
+        v += dpp<0xB1>(v);
+        v += dpp<0x4E>(v);
+        return v + dpp<0x141>(v);
+    }
     // Sum over the aligned row of 16 lanes; all sixteen lanes get the same bits.
     DEV double row16_sum(double v) const {
         v += dpp<0xB1>(v);
@@ -183,6 +189,10 @@ struct EmuCtx {
     }
     double row16_sum(double v) const {
         for (int m = 1; m < 16; m <<= 1) v += shfl_xor(v, m);
+        return v;
+    }
+    double oct_sum(double v) const {
+        for (int m = 1; m < 8; m <<= 1) v += shfl_xor(v, m);
         return v;
     }
     double lane_bcast(double v, int l) const {
