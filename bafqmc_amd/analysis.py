@@ -58,6 +58,49 @@ def block_statistics(samples, block_size: int, skip: int = 0):
     return {"actual": float(means.mean()), "stderr": float(means.std(ddof=1) / math.sqrt(nb)), "blocks": nb}
 
 
+def tau_int_pooled(series, c: float = 6.0):
+    """Integrated autocorrelation time of many independent chains of one
+    stationary process: the autocovariance is averaged over chains about the
+    grand mean, Sokal's window stops at the first W >= c tau(W), and the
+    standard error is the Madras-Sokal estimate tau sqrt(2 (2 W + 1) / N) with
+    N the number of samples.  Returns a dict with tau, its error, the window,
+    the number of samples per chain and `short` (series shorter than 50 tau)."""
+    x = np.asarray(series, dtype=float)
+    if x.ndim == 1:
+        x = x[None, :]
+    m = float(np.mean(x))
+    x = x - m
+    nb = x.shape[1]
+    var = float(np.mean(x * x))
+    out = {"tau": 0.5, "err": 0.0, "window": 0, "length": int(nb), "short": True}
+    if var <= 0.0 or nb < 4:
+        return out
+    tau, w = 0.5, 0
+    for w in range(1, nb // 2):
+        tau += float(np.mean(x[:, :-w] * x[:, w:])) / var
+        if w >= c * tau:
+            break
+    tau = max(tau, 0.5)
+    out.update(tau=tau, window=int(w), err=tau * math.sqrt(2.0 * (2 * w + 1) / x.size),
+               short=bool(nb < 50 * tau))
+    return out
+
+
+def chain_mean_error(series):
+    """Standard error of the grand mean from the spread of the chains' time
+    averages (independent chains; valid when each series is much longer than
+    tau), and the blocked standard error of the pooled series
+    (campaign_analysis.py:215-232 blocking, blocks of 1/10 of a chain) as a
+    cross-check."""
+    x = np.asarray(series, dtype=float)
+    means = x.mean(axis=1)
+    se = float(np.std(means, ddof=1) / math.sqrt(len(means)))
+    bs = max(1, x.shape[1] // 10)
+    blocked = [block_statistics(row, bs) for row in x] if x.shape[1] >= 2 * bs else []
+    se_blk = float(np.sqrt(np.mean([b["stderr"] ** 2 for b in blocked])) / math.sqrt(len(x))) if blocked else float("nan")
+    return {"mean": float(means.mean()), "stderr": se, "stderr_blocked": se_blk}
+
+
 def tau_int(series, c: float = 5.0):
     """series: [chains, bins] real.  Returns (tau_int, window, acf)."""
     x = np.asarray(series, dtype=float)

@@ -14,10 +14,11 @@
 #include "sweep.hpp"
 #include "net.hpp"
 #include "gmove.hpp"
+#include "conv.hpp"
 
 namespace bafq {
 
-enum KernelId { K_PREPARE = 0, K_SWEEP = 1, K_WARMUP = 2, K_GRAD = 3 };
+enum KernelId { K_PREPARE = 0, K_SWEEP = 1, K_WARMUP = 2, K_GRAD = 3, K_MEASURE = 4 };
 enum PathId { P_GLOBAL = 0, P_LDS = 1 };
 
 // back-end hooks (0 on success)
@@ -53,6 +54,11 @@ size_t rt_lds_limit();         // LDS bytes available to one workgroup
 int rt_debug_gemm(int n, const double* a, int op_a, const double* b, int op_b, double* c);
 int rt_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int variant);
 int rt_debug_mfma(double* out);
+int rt_debug_mfma32(float* out);
+// device-time stopwatch on the default stream (HIP events); *t created on first use
+int rt_timer_start(void** t);
+int rt_timer_stop(void* t, double* ms);
+void rt_timer_free(void* t);
 int rt_debug_lanes(const double* in, const int32_t* idx, double* out);
 // periodic 3D convolution network (net.hpp); device pointers
 int rt_conv_fwd(const float* in, int cin, const float* W, const float* b, float* out, int cout, int batch,
@@ -68,6 +74,12 @@ int rt_gm_to_net(const double* phi, float* x, const GmArgs& a);
 int rt_gm_propose(double* phi, double* old, double* f0, const float* y, const GmArgs& a);
 int rt_gm_sums(const double* phi, const double* old, const double* f0, const float* y, const GmArgs& a, double* out);
 int rt_gm_restore(double* phi, const double* old, const int* accept, int per, size_t n);
+// Hamiltonian moves (gmove.hpp gm_hmc_*)
+int rt_gm_hmc_init(const double* phi, double* old, double* p0, double* p, const GmArgs& a);
+int rt_gm_hmc_kick(double* p, const float* y, const GmArgs& a);
+int rt_gm_hmc_rotate(double* phi, double* p, const GmArgs& a);
+int rt_gm_hmc_sums(const double* phi, const double* old, const double* p0, const double* p, const GmArgs& a,
+                   double* out);
 // words [c * per, (c + 1) * per) of dst <- src for chains c with accept[c] == 0
 int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n);
 
@@ -101,6 +113,7 @@ struct bafq_handle {
     void* launches = nullptr;                 // back-end launch cache
     // learned global moves: old fields, forward drift, per-chain sums, flags, subset ids
     double *gm_old = nullptr, *gm_f0 = nullptr, *gm_out = nullptr, *gm_stats = nullptr;
+    double* gm_p = nullptr;                   // Hamiltonian moves: the momenta
     uint64_t* gm_state = nullptr;             // copy of every chain's matrices before the move
     double* d_grad = nullptr;                 // d log W / d phi, laid out like the fields
     int* gm_accept = nullptr;
@@ -404,6 +417,7 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
 void bafq_destroy(bafq_handle* h) {
     if (!h) return;
     if (h->gm_old) bafq::rt_free(h->gm_old);
+    if (h->gm_p) bafq::rt_free(h->gm_p);
     if (h->gm_state) bafq::rt_free(h->gm_state);
     if (h->d_grad) bafq::rt_free(h->d_grad);
     bafq::rt_release_launches(h->launches);
@@ -475,7 +489,7 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
             l.a.ids = h->d_path_ids[p];
             l.a.count = h->path_count[p];
             l.a.queue = h->d_queue + p;
-            l.grid = h->grid[k == K_GRAD ? K_SWEEP : k][p];
+            l.grid = h->grid[(k == K_GRAD || k == K_MEASURE) ? K_SWEEP : k][p];
             l.block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
             l.smem = h->max_lds[p];
         }
@@ -501,6 +515,12 @@ int bafq_sweep(bafq_handle* h, int n_sweeps, int measure) {
     if (!h) return 1;
     if (n_sweeps <= 0) return 0;
     return launch_all(h, bafq::K_SWEEP, n_sweeps, measure);
+}
+
+int bafq_measure_pass(bafq_handle* h, int n_passes) {
+    if (!h) return 1;
+    if (n_passes <= 0) return 0;
+    return launch_all(h, bafq::K_MEASURE, n_passes, 1);
 }
 
 int bafq_get_stats(bafq_handle* h, double* stats) {
@@ -558,6 +578,7 @@ int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int 
 }
 
 int bafq_debug_mfma_probe(double* out) { return bafq::rt_debug_mfma(out); }
+int bafq_debug_mfma32_probe(float* out) { return bafq::rt_debug_mfma32(out); }
 int bafq_debug_lane_ops(const double* in, const int32_t* idx, double* out) { return bafq::rt_debug_lanes(in, idx, out); }
 
 }  // extern "C"
@@ -574,6 +595,7 @@ struct bafq_net {
     std::vector<float*> act;             // act[0] input ... act[L] output
     float *g0 = nullptr, *g1 = nullptr, *target = nullptr;
     int step = 0;
+    void* timer = nullptr;
     std::string err;
 };
 
@@ -624,6 +646,7 @@ int bafq_net_create(const bafq_net_desc* d, int max_batch, bafq_net** out) {
 
 void bafq_net_destroy(bafq_net* h) {
     if (!h) return;
+    bafq::rt_timer_free(h->timer);
     if (h->dev) bafq::rt_free(h->dev);
     delete h;
 }
@@ -668,6 +691,20 @@ int bafq_net_forward(bafq_net* h, int batch, const float* x, float* y) {
     if (rt_h2d(h->act[0], x, n * sizeof(float))) return net_fail(h, rt_error());
     if (net_forward_dev(h, batch)) return 1;
     if (rt_sync() || rt_d2h(y, h->act[h->d.layers], n * sizeof(float))) return net_fail(h, rt_error());
+    return 0;
+}
+
+int bafq_net_time_forward(bafq_net* h, int batch, int reps, double* ms) {
+    using namespace bafq;
+    if (!h || !ms) return 1;
+    if (batch < 1 || batch > h->max_batch || reps < 1) return net_fail(h, "batch or reps out of range");
+    if (net_forward_dev(h, batch)) return 1;                  // warm (kernel attributes, caches)
+    if (rt_sync() || rt_timer_start(&h->timer)) return net_fail(h, rt_error());
+    for (int r = 0; r < reps; ++r)
+        if (net_forward_dev(h, batch)) return 1;
+    double t = 0.0;
+    if (rt_timer_stop(h->timer, &t)) return net_fail(h, rt_error());
+    *ms = t / reps;
     return 0;
 }
 
@@ -735,6 +772,32 @@ int bafq_net_train(bafq_net* h, int batch, const float* x, const float* target, 
 // ------------------------------------------------------------------ learned global moves
 extern "C" {
 
+// Buffers of the device global moves and the copy of every chain's state
+// (G, UDV factors and checkpoint lists, work matrices: one block of the same
+// size per chain, consecutive) and counters taken before a move, so that
+// rejected chains get them back without a rebuild.
+static int gm_snapshot(bafq_handle* h, size_t total, int nc) {
+    using namespace bafq;
+    if (!h->gm_old) {
+        // old fields, forward drift (or initial momenta), 5 sums per chain (doubles); acceptance flags
+        const size_t bytes = sizeof(double) * (2 * total + 5 * (size_t)nc) + sizeof(int) * (size_t)nc;
+        if (rt_malloc((void**)&h->gm_old, bytes)) return fail(h, rt_error());
+        h->gm_f0 = h->gm_old + total;
+        h->gm_out = h->gm_f0 + total;
+        h->gm_accept = (int*)(h->gm_out + 5 * (size_t)nc);
+    }
+    const size_t block = h->chain_block_bytes;
+    if (!h->gm_state) {
+        if (rt_malloc((void**)&h->gm_state, block * nc + sizeof(double) * S_LEN * (size_t)nc))
+            return fail(h, rt_error());
+        h->gm_stats = (double*)((char*)h->gm_state + block * nc);
+    }
+    if (rt_d2d(h->gm_state, h->chains[0].g, block * nc) ||
+        rt_d2d(h->gm_stats, h->d_stats, sizeof(double) * S_LEN * (size_t)nc))
+        return fail(h, rt_error());
+    return 0;
+}
+
 int bafq_global_propose(bafq_handle* h, bafq_net* net, double lam, const double* sigma, uint64_t seed,
                         double* out) {
     using namespace bafq;
@@ -747,27 +810,8 @@ int bafq_global_propose(bafq_handle* h, bafq_net* net, double lam, const double*
     if (net && (net->d.lt != m.ltrot || net->d.ly != m.nly || net->d.lx != m.nlx || net->d.channels != 2))
         return fail(h, "the network's lattice is not the model's");
     const size_t per = (size_t)gm_per(s), total = per * nc;
-    if (!h->gm_old) {
-        // old fields, forward drift, 5 sums per chain (doubles); acceptance flags
-        const size_t bytes = sizeof(double) * (2 * total + 5 * (size_t)nc) + sizeof(int) * (size_t)nc;
-        if (rt_malloc((void**)&h->gm_old, bytes)) return fail(h, rt_error());
-        h->gm_f0 = h->gm_old + total;
-        h->gm_out = h->gm_f0 + total;
-        h->gm_accept = (int*)(h->gm_out + 5 * (size_t)nc);
-    }
-    // every chain's matrices (G, UDV factors and checkpoint lists, work matrices:
-    // one block of the same size per chain, consecutive) and counters, so that
-    // rejected chains get them back without a rebuild
-    const size_t block = h->chain_block_bytes;
-    if (!h->gm_state) {
-        if (rt_malloc((void**)&h->gm_state, block * nc + sizeof(double) * S_LEN * (size_t)nc))
-            return fail(h, rt_error());
-        h->gm_stats = (double*)((char*)h->gm_state + block * nc);
-    }
-    if (rt_d2d(h->gm_state, h->chains[0].g, block * nc) ||
-        rt_d2d(h->gm_stats, h->d_stats, sizeof(double) * S_LEN * (size_t)nc))
-        return fail(h, rt_error());
-    GmArgs a;
+    if (gm_snapshot(h, total, nc)) return 1;
+    GmArgs a{};
     a.s = s;
     a.lam = lam;
     a.seed = seed;
@@ -792,6 +836,57 @@ int bafq_global_propose(bafq_handle* h, bafq_net* net, double lam, const double*
             if (net_forward_dev(net, a.batch)) return fail(h, net->err);
         }
         if (rt_gm_sums(h->d_phi, h->gm_old, h->gm_f0, y, a, h->gm_out)) return fail(h, rt_error());
+    }
+    if (rt_sync() || rt_d2h(out, h->gm_out, sizeof(double) * 5 * (size_t)nc)) return fail(h, rt_error());
+    return launch_all(h, K_PREPARE, 0, 0);
+}
+
+int bafq_hmc_propose(bafq_handle* h, bafq_net* net, double angle, int steps, const double* center,
+                     const double* force_scale, uint64_t seed, double* out) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (h->models.size() != 1) return fail(h, "device global moves need a handle with one model");
+    if (!center || !force_scale || !out || !(angle > 0.0) || !(angle <= 3.2) || steps < 1 || steps > 64)
+        return fail(h, "invalid Hamiltonian move arguments (0 < angle <= 3.2, 1 <= steps <= 64)");
+    const ModelDev& m = h->models[0];
+    const int nc = (int)h->chains.size();
+    const NetShape s{m.ltrot, m.nly, m.nlx};
+    if (net && (net->d.lt != m.ltrot || net->d.ly != m.nly || net->d.lx != m.nlx || net->d.channels != 2))
+        return fail(h, "the network's lattice is not the model's");
+    const size_t per = (size_t)gm_per(s), total = per * nc;
+    if (gm_snapshot(h, total, nc)) return 1;
+    if (!h->gm_p && rt_malloc((void**)&h->gm_p, sizeof(double) * total)) return fail(h, rt_error());
+    GmArgs a{};
+    a.s = s;
+    a.lam = 0.5 * angle;                  // half kick
+    a.seed = seed;
+    a.cosh_ = std::cos(angle);
+    a.sinh_ = std::sin(angle);
+    for (int ch = 0; ch < 2; ++ch) {
+        a.mask[ch] = m.active[ch] ? 1.0 : 0.0;
+        a.center[ch] = center[ch];
+        a.fscale[ch] = force_scale[ch];
+    }
+    const int chunk = net ? net->max_batch : nc;
+    for (int c0 = 0; c0 < nc; c0 += chunk) {
+        a.c0 = c0;
+        a.batch = std::min(chunk, nc - c0);
+        if (rt_gm_hmc_init(h->d_phi, h->gm_old, h->gm_f0, h->gm_p, a)) return fail(h, rt_error());
+        const float* y = net ? net->act[net->d.layers] : nullptr;
+        if (net) {
+            if (rt_gm_to_net(h->d_phi, net->act[0], a)) return fail(h, rt_error());
+            if (net_forward_dev(net, a.batch)) return fail(h, net->err);
+        }
+        for (int k = 0; k < steps; ++k) {
+            if (net && rt_gm_hmc_kick(h->gm_p, y, a)) return fail(h, rt_error());
+            if (rt_gm_hmc_rotate(h->d_phi, h->gm_p, a)) return fail(h, rt_error());
+            if (net) {
+                if (rt_gm_to_net(h->d_phi, net->act[0], a)) return fail(h, rt_error());
+                if (net_forward_dev(net, a.batch)) return fail(h, net->err);
+                if (rt_gm_hmc_kick(h->gm_p, y, a)) return fail(h, rt_error());
+            }
+        }
+        if (rt_gm_hmc_sums(h->d_phi, h->gm_old, h->gm_f0, h->gm_p, a, h->gm_out)) return fail(h, rt_error());
     }
     if (rt_sync() || rt_d2h(out, h->gm_out, sizeof(double) * 5 * (size_t)nc)) return fail(h, rt_error());
     return launch_all(h, K_PREPARE, 0, 0);

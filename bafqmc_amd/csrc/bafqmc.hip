@@ -52,6 +52,18 @@ __global__ void __launch_bounds__(512) k_grad_lds(LaunchArgs a) {
     body_grad<true>(c, a, (int*)smem);
 }
 
+__global__ void __launch_bounds__(512) k_measure(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem + kSlotBytes};
+    body_measure<false>(c, a, (int*)smem);
+}
+
+__global__ void __launch_bounds__(512) k_measure_lds(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem + kSlotBytes};
+    body_measure<true>(c, a, (int*)smem);
+}
+
 __global__ void __launch_bounds__(64) k_reset_queue(int* q) {
     if (threadIdx.x == 0) *q = 0;
 }
@@ -230,11 +242,61 @@ __global__ void __launch_bounds__(256) k_gm_restore_words(uint64_t* dst, const u
         if (!accept[e / per]) dst[e] = src[e];
 }
 
+__global__ void __launch_bounds__(256) k_gm_hmc_init(const double* phi, double* old, double* p0, double* p, GmArgs a) {
+    const size_t n = (size_t)a.batch * gm_per(a.s);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        gm_hmc_init_elem(phi, old, p0, p, a, (int)e);
+}
+__global__ void __launch_bounds__(256) k_gm_hmc_kick(double* p, const float* y, GmArgs a) {
+    const size_t n = (size_t)a.batch * gm_per(a.s);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        gm_hmc_kick_elem(p, y, a, (int)e);
+}
+__global__ void __launch_bounds__(256) k_gm_hmc_rotate(double* phi, double* p, GmArgs a) {
+    const size_t n = (size_t)a.batch * gm_per(a.s);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        gm_hmc_rotate_elem(phi, p, a, (int)e);
+}
+__global__ void __launch_bounds__(256) k_gm_hmc_sums(const double* phi, const double* old, const double* p0,
+                                                     const double* p, GmArgs a, double* out) {
+    __shared__ double part[5][256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    gm_hmc_sums(phi, old, p0, p, a, b, tid, gm_per(a.s), 256, acc);
+    for (int k = 0; k < 5; ++k) part[k][tid] = acc[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int k = 0; k < 5; ++k) part[k][tid] += part[k][tid + h];
+        __syncthreads();
+    }
+    if (tid < 5) out[(size_t)(a.c0 + b) * 5 + tid] = part[tid][0];
+}
+
 static int grid_for(size_t n) { return (int)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535); }
 
 int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n) {
     hipLaunchKernelGGL(k_gm_restore_words, dim3(grid_for(n)), dim3(256), 0, 0, dst, src, accept, per, n);
     return check(hipGetLastError(), "k_gm_restore_words");
+}
+
+int rt_gm_hmc_init(const double* phi, double* old, double* p0, double* p, const GmArgs& a) {
+    hipLaunchKernelGGL(k_gm_hmc_init, dim3(grid_for((size_t)a.batch * gm_per(a.s))), dim3(256), 0, 0, phi, old, p0, p,
+                       a);
+    return check(hipGetLastError(), "k_gm_hmc_init");
+}
+int rt_gm_hmc_kick(double* p, const float* y, const GmArgs& a) {
+    hipLaunchKernelGGL(k_gm_hmc_kick, dim3(grid_for((size_t)a.batch * gm_per(a.s))), dim3(256), 0, 0, p, y, a);
+    return check(hipGetLastError(), "k_gm_hmc_kick");
+}
+int rt_gm_hmc_rotate(double* phi, double* p, const GmArgs& a) {
+    hipLaunchKernelGGL(k_gm_hmc_rotate, dim3(grid_for((size_t)a.batch * gm_per(a.s))), dim3(256), 0, 0, phi, p, a);
+    return check(hipGetLastError(), "k_gm_hmc_rotate");
+}
+int rt_gm_hmc_sums(const double* phi, const double* old, const double* p0, const double* p, const GmArgs& a,
+                   double* out) {
+    hipLaunchKernelGGL(k_gm_hmc_sums, dim3(a.batch), dim3(256), 0, 0, phi, old, p0, p, a, out);
+    return check(hipGetLastError(), "k_gm_hmc_sums");
 }
 
 int rt_gm_to_net(const double* phi, float* x, const GmArgs& a) {
@@ -255,14 +317,64 @@ int rt_gm_restore(double* phi, const double* old, const int* accept, int per, si
     return check(hipGetLastError(), "k_gm_restore");
 }
 
+template <int NB>
+__global__ void __launch_bounds__(512) k_conv_mfma(ConvArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem};
+    conv_mfma_body<NB>(c, a);
+}
+
+// The matrix-core convolution (conv.hpp) when the layer fits its plan, else
+// 1 (the caller runs the per-element kernel).  BAFQ_CONV=elem selects the
+// per-element kernels everywhere (for comparisons).
+static int conv_mfma_launch(ConvArgs a, int batch, int* used) {
+    *used = 0;
+    const char* env = std::getenv("BAFQ_CONV");
+    if (env && std::string(env) == "elem") return 0;
+    const int plane = a.s.ly * a.s.lx;
+    a.cinp = conv_pad4(a.cin);
+    a.coutp = conv_pad16(a.cout);
+    const int nb = a.coutp / 16;
+    if (nb > 4) return 0;
+    a.tt = conv_pick_tt(a.s.lt, plane, a.cinp, a.coutp, rt_lds_limit());
+    if (a.tt == 0) return 0;
+    a.tblocks = (a.s.lt + a.tt - 1) / a.tt;
+    const size_t lb = conv_lds_bytes(a.tt, a.cinp, a.coutp, plane);
+    const void* fn = nb == 1 ? (const void*)k_conv_mfma<1> : nb == 2 ? (const void*)k_conv_mfma<2>
+                   : nb == 3 ? (const void*)k_conv_mfma<3> : (const void*)k_conv_mfma<4>;
+    if (lb > 65536 && check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb),
+                            "hipFuncSetAttribute"))
+        return 1;
+    const dim3 grid(batch * a.tblocks), block(kConvWaves * 64);
+    if (nb == 1) hipLaunchKernelGGL(k_conv_mfma<1>, grid, block, lb, 0, a);
+    else if (nb == 2) hipLaunchKernelGGL(k_conv_mfma<2>, grid, block, lb, 0, a);
+    else if (nb == 3) hipLaunchKernelGGL(k_conv_mfma<3>, grid, block, lb, 0, a);
+    else hipLaunchKernelGGL(k_conv_mfma<4>, grid, block, lb, 0, a);
+    *used = 1;
+    return check(hipGetLastError(), "k_conv_mfma");
+}
+
 int rt_conv_fwd(const float* in, int cin, const float* W, const float* b, float* out, int cout, int batch,
                 NetShape s, int relu) {
+    ConvArgs ca{};
+    ca.in = in; ca.W = W; ca.bias = b; ca.mask = nullptr; ca.out = out;
+    ca.cin = cin; ca.cout = cout; ca.relu = relu; ca.flip = 0; ca.s = s;
+    int used = 0;
+    if (conv_mfma_launch(ca, batch, &used)) return 1;
+    if (used) return 0;
     const int n = batch * cout * s.vol();
     hipLaunchKernelGGL(k_conv_fwd, dim3(grid_for(n)), dim3(256), 0, 0, in, cin, W, b, out, cout, n, s, relu);
     return check(hipGetLastError(), "k_conv_fwd");
 }
 int rt_conv_bwd_data(const float* gout, int cout, const float* W, float* gin, int cin, const float* act, int mask,
                      int batch, NetShape s) {
+    // the forward layer's transposed, reflected stencil from cout to cin channels
+    ConvArgs ca{};
+    ca.in = gout; ca.W = W; ca.bias = nullptr; ca.mask = mask ? act : nullptr; ca.out = gin;
+    ca.cin = cout; ca.cout = cin; ca.relu = 0; ca.flip = 1; ca.s = s;
+    int used = 0;
+    if (conv_mfma_launch(ca, batch, &used)) return 1;
+    if (used) return 0;
     const int n = batch * cin * s.vol();
     hipLaunchKernelGGL(k_conv_bwd_data, dim3(grid_for(n)), dim3(256), 0, 0, gout, cout, W, gin, cin, act, mask, n, s);
     return check(hipGetLastError(), "k_conv_bwd_data");
@@ -283,6 +395,24 @@ int rt_adam(float* p, const float* g, float* m, float* v, float lr, int step, si
     return check(hipGetLastError(), "k_adam");
 }
 
+// Lane-layout probe of the FP32 MFMA 16x16x4: as k_debug_mfma.
+__global__ void __launch_bounds__(64) k_debug_mfma32(float* out) {
+    Ctx c{nullptr};
+    const int p = blockIdx.x, l = threadIdx.x;
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    acc = c.mfma32(l == p ? 1.0f : 0.0f, 1000.0f + l, acc);
+    for (int r = 0; r < 4; ++r) out[(p * 64 + l) * 4 + r] = acc[r];
+}
+
+int rt_debug_mfma32(float* out) {
+    float* d = nullptr;
+    if (rt_malloc((void**)&d, sizeof(float) * 64 * 64 * 4)) return 1;
+    hipLaunchKernelGGL(k_debug_mfma32, dim3(64), dim3(64), 0, 0, d);
+    int rc = check(hipGetLastError(), "k_debug_mfma32") || rt_sync() || rt_d2h(out, d, sizeof(float) * 64 * 64 * 4);
+    rt_free(d);
+    return rc;
+}
+
 int rt_debug_mfma(double* out) {
     double* d = nullptr;
     if (rt_malloc((void**)&d, sizeof(double) * 64 * 64 * 4)) return 1;
@@ -290,6 +420,36 @@ int rt_debug_mfma(double* out) {
     int rc = check(hipGetLastError(), "k_debug_mfma") || rt_sync() || rt_d2h(out, d, sizeof(double) * 64 * 64 * 4);
     rt_free(d);
     return rc;
+}
+
+struct Stopwatch {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+int rt_timer_start(void** t) {
+    if (!*t) {
+        Stopwatch* w = new Stopwatch();
+        *t = w;
+        if (check(hipEventCreate(&w->ev[0]), "hipEventCreate") || check(hipEventCreate(&w->ev[1]), "hipEventCreate"))
+            return 1;
+    }
+    return check(hipEventRecord(((Stopwatch*)*t)->ev[0], 0), "hipEventRecord");
+}
+int rt_timer_stop(void* t, double* ms) {
+    Stopwatch* w = (Stopwatch*)t;
+    if (check(hipEventRecord(w->ev[1], 0), "hipEventRecord") ||
+        check(hipEventSynchronize(w->ev[1]), "hipEventSynchronize"))
+        return 1;
+    float f = 0.0f;
+    if (check(hipEventElapsedTime(&f, w->ev[0], w->ev[1]), "hipEventElapsedTime")) return 1;
+    *ms = (double)f;
+    return 0;
+}
+void rt_timer_free(void* t) {
+    if (!t) return;
+    Stopwatch* w = (Stopwatch*)t;
+    for (hipEvent_t& e : w->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete w;
 }
 
 int rt_malloc(void** p, size_t bytes) { return check(hipMalloc(p, bytes), "hipMalloc"); }
@@ -334,6 +494,7 @@ size_t rt_lds_limit() {
 static const void* kernel_of(KernelId k, PathId path) {
     if (k == K_PREPARE) return path == P_LDS ? (const void*)k_prepare_lds : (const void*)k_prepare;
     if (k == K_GRAD) return path == P_LDS ? (const void*)k_grad_lds : (const void*)k_grad;
+    if (k == K_MEASURE) return path == P_LDS ? (const void*)k_measure_lds : (const void*)k_measure;
     return path == P_LDS ? (const void*)k_sweep_lds : (const void*)k_sweep;
 }
 

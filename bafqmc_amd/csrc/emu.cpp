@@ -8,6 +8,7 @@
 // a launch run in parallel on a few OS threads.  x86-64 only.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstdlib>
 #include <functional>
@@ -187,6 +188,18 @@ static void run_groups(int grid, int block, size_t smem, F&& fn) {
     for (auto& x : th) x.join();
 }
 
+int rt_timer_start(void** t) {
+    if (!*t) *t = new std::chrono::steady_clock::time_point();
+    *(std::chrono::steady_clock::time_point*)*t = std::chrono::steady_clock::now();
+    return 0;
+}
+int rt_timer_stop(void* t, double* ms) {
+    const auto d = std::chrono::steady_clock::now() - *(std::chrono::steady_clock::time_point*)t;
+    *ms = std::chrono::duration<double, std::milli>(d).count();
+    return 0;
+}
+void rt_timer_free(void* t) { delete (std::chrono::steady_clock::time_point*)t; }
+
 int rt_malloc(void** p, size_t bytes) {
     *p = std::calloc(1, bytes ? bytes : 1);
     if (!*p) { g_err = "out of memory"; return 1; }
@@ -261,6 +274,9 @@ int rt_launch_paths(void**, KernelId k, const PathLaunch* pl, int n, double* ms)
             } else if (k == K_GRAD) {
                 if (l.path == P_LDS) body_grad<true>(c, l.a, slot);
                 else body_grad<false>(c, l.a, slot);
+            } else if (k == K_MEASURE) {
+                if (l.path == P_LDS) body_measure<true>(c, l.a, slot);
+                else body_measure<false>(c, l.a, slot);
             } else {
                 if (l.path == P_LDS) body_sweep<true>(c, l.a, slot);
                 else body_sweep<false>(c, l.a, slot);
@@ -293,6 +309,30 @@ int rt_gm_sums(const double* phi, const double* old, const double* f0, const flo
     }
     return 0;
 }
+int rt_gm_hmc_init(const double* phi, double* old, double* p0, double* p, const GmArgs& a) {
+    const size_t n = (size_t)a.batch * gm_per(a.s);
+    for (size_t e = 0; e < n; ++e) gm_hmc_init_elem(phi, old, p0, p, a, (int)e);
+    return 0;
+}
+int rt_gm_hmc_kick(double* p, const float* y, const GmArgs& a) {
+    const size_t n = (size_t)a.batch * gm_per(a.s);
+    for (size_t e = 0; e < n; ++e) gm_hmc_kick_elem(p, y, a, (int)e);
+    return 0;
+}
+int rt_gm_hmc_rotate(double* phi, double* p, const GmArgs& a) {
+    const size_t n = (size_t)a.batch * gm_per(a.s);
+    for (size_t e = 0; e < n; ++e) gm_hmc_rotate_elem(phi, p, a, (int)e);
+    return 0;
+}
+int rt_gm_hmc_sums(const double* phi, const double* old, const double* p0, const double* p, const GmArgs& a,
+                   double* out) {
+    for (int b = 0; b < a.batch; ++b) {
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        gm_hmc_sums(phi, old, p0, p, a, b, 0, gm_per(a.s), 1, acc);
+        for (int k = 0; k < 5; ++k) out[(size_t)(a.c0 + b) * 5 + k] = acc[k];
+    }
+    return 0;
+}
 int rt_gm_restore(double* phi, const double* old, const int* accept, int per, size_t n) {
     for (size_t e = 0; e < n; ++e) gm_restore_elem(phi, old, accept, per, (int)e);
     return 0;
@@ -304,8 +344,34 @@ int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, s
 }
 
 // ------------------------------------------------------------------ network
+// The matrix-core convolution (conv.hpp) on the emulator when the layer fits
+// its plan; BAFQ_CONV=elem selects the per-element definition.
+static int conv_mfma_run(ConvArgs a, int batch) {
+    const char* env = std::getenv("BAFQ_CONV");
+    if (env && std::string(env) == "elem") return 0;
+    const int plane = a.s.ly * a.s.lx;
+    a.cinp = conv_pad4(a.cin);
+    a.coutp = conv_pad16(a.cout);
+    const int nb = a.coutp / 16;
+    if (nb > 4) return 0;
+    a.tt = conv_pick_tt(a.s.lt, plane, a.cinp, a.coutp, rt_lds_limit());
+    if (a.tt == 0) return 0;
+    a.tblocks = (a.s.lt + a.tt - 1) / a.tt;
+    run_groups(batch * a.tblocks, kConvWaves * 64, conv_lds_bytes(a.tt, a.cinp, a.coutp, plane), [&](EmuCtx& c) {
+        if (nb == 1) conv_mfma_body<1>(c, a);
+        else if (nb == 2) conv_mfma_body<2>(c, a);
+        else if (nb == 3) conv_mfma_body<3>(c, a);
+        else conv_mfma_body<4>(c, a);
+    });
+    return 1;
+}
+
 int rt_conv_fwd(const float* in, int cin, const float* W, const float* b, float* out, int cout, int batch,
                 NetShape s, int relu) {
+    ConvArgs ca{};
+    ca.in = in; ca.W = W; ca.bias = b; ca.mask = nullptr; ca.out = out;
+    ca.cin = cin; ca.cout = cout; ca.relu = relu; ca.flip = 0; ca.s = s;
+    if (conv_mfma_run(ca, batch)) return 0;
     const int n = batch * cout * s.vol();
     for (int e = 0; e < n; ++e) {
         const float y = conv_fwd_elem(in, cin, W, b, cout, s, e);
@@ -315,6 +381,10 @@ int rt_conv_fwd(const float* in, int cin, const float* W, const float* b, float*
 }
 int rt_conv_bwd_data(const float* gout, int cout, const float* W, float* gin, int cin, const float* act, int mask,
                      int batch, NetShape s) {
+    ConvArgs ca{};
+    ca.in = gout; ca.W = W; ca.bias = nullptr; ca.mask = mask ? act : nullptr; ca.out = gin;
+    ca.cin = cout; ca.cout = cin; ca.relu = 0; ca.flip = 1; ca.s = s;
+    if (conv_mfma_run(ca, batch)) return 0;
     const int n = batch * cin * s.vol();
     for (int e = 0; e < n; ++e) gin[e] = conv_bwd_data_elem(gout, cout, W, cin, act, mask != 0, s, e);
     return 0;
@@ -355,6 +425,16 @@ int rt_debug_mfma(double* out) {
         run_groups(1, 64, 0, [&](EmuCtx& c) {
             d4 acc = {0.0, 0.0, 0.0, 0.0};
             acc = c.mfma(c.lane() == p ? 1.0 : 0.0, 1000.0 + c.lane(), acc);
+            for (int r = 0; r < 4; ++r) out[(p * 64 + c.lane()) * 4 + r] = acc[r];
+        });
+    return 0;
+}
+
+int rt_debug_mfma32(float* out) {
+    for (int p = 0; p < 64; ++p)
+        run_groups(1, 64, 0, [&](EmuCtx& c) {
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = c.mfma32(c.lane() == p ? 1.0f : 0.0f, 1000.0f + c.lane(), acc);
             for (int r = 0; r < 4; ++r) out[(p * 64 + c.lane()) * 4 + r] = acc[r];
         });
     return 0;

@@ -27,6 +27,7 @@
 #define DEVFN __device__ __noinline__
 #define HOSTDEV __host__ __device__ inline
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 #else
 #include <atomic>
 #include <chrono>
@@ -40,6 +41,11 @@ struct d4 {
     double& operator[](int i) { return v[i]; }
     double operator[](int i) const { return v[i]; }
 };
+struct f4 {
+    float v[4];
+    float& operator[](int i) { return v[i]; }
+    float operator[](int i) const { return v[i]; }
+};
 #endif
 
 namespace bafq {
@@ -48,6 +54,11 @@ constexpr int kWave = 64;
 
 // Row of the 16x16 MFMA f64 result held by lane `lane` in accumulator register r.
 HOSTDEV int mfma_row(int lane, int r) { return (lane >> 4) + 4 * r; }
+// Row of the 16x16 MFMA f32 (16x16x4f32) result held by lane `lane` in
+// accumulator register r; operands as for f64: A[l % 16][l / 16],
+// B[l / 16][l % 16]; column l % 16 (checked on gfx950 by
+// bafq_debug_mfma32_probe / tests/test_gpu_parity.py).
+HOSTDEV int mfma32_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 
 #ifndef BAFQ_EMU
 struct HipCtx {
@@ -64,6 +75,9 @@ struct HipCtx {
     DEV int shfl_xor_i(int v, int mask) const { return __shfl_xor(v, mask, 64); }
     DEV d4 mfma(double a, double b, d4 c) const {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    DEV f4 mfma32(float a, float b, f4 c) const {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
     DEV int atomic_add(int* p, int v) const { return atomicAdd(p, v); }
     // constant-rate device clock (hipDeviceAttributeWallClockRate kHz)
@@ -174,6 +188,24 @@ struct EmuCtx {
             const int i = mfma_row(l, r);
             double s = c[r];
             for (int k = 0; k < 4; ++k) s += A[i + 16 * k] * B[j + 16 * k];
+            c[r] = s;
+        }
+        wsync();
+        return c;
+    }
+    // f32 16x16x4: products accumulated in k order in FP32 (the hardware's
+    // order of the four products may differ; tests compare with tolerances)
+    f4 mfma32(float a, float b, f4 c) const {
+        double* A = wb(0);
+        double* B = wb(1);
+        A[lane()] = a;
+        B[lane()] = b;
+        wsync();
+        const int l = lane(), j = l & 15;
+        for (int r = 0; r < 4; ++r) {
+            const int i = mfma32_row(l, r);
+            float s = c[r];
+            for (int k = 0; k < 4; ++k) s += (float)A[i + 16 * k] * (float)B[j + 16 * k];
             c[r] = s;
         }
         wsync();

@@ -859,6 +859,9 @@ DEV void store_drift_and_logdet(const Ctx& c, const ModelDev& m, const ChainDev&
     if (c.tid() == 0) {
         double s = 0.0;
         for (int w = 0; w < c.nwaves(); ++w) s += L.red[w];
+        // the reference sums over its 4Ns Nambu G, whose {c, b+} block is the
+        // complex conjugate of the reduced {b, c+} block held here
+        if (m.kind == 1) s *= 2.0;
         if (s > ch.stats[S_XMAX]) ch.stats[S_XMAX] = s;
         ch.stats[S_XMEAN_SUM] += s;
         ch.stats[S_NWRAP] += 1.0;
@@ -1173,7 +1176,7 @@ DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
 }
 
 DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
-                           bool meas, double* grad = nullptr) {
+                           bool meas, double* grad = nullptr, bool frozen = false) {
     const int n = m.n;
     Timer tm;
     live_identity(c, n, ch.ul, ch.dl, ch.vl);     // Wrap_L(Ltrot): L list slot Nst is the identity
@@ -1198,6 +1201,7 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
             if (grad) grad_extract(c, m, phi_t, as_global(ch.g), t, nf, grad);
+            else if (frozen) {}
             else if (m.dk > 0) site_loop_delayed_glb(c, m, ch, L, ch.g, t, nf, false, seed);
             else site_loop(c, m, ch, L, ch.g, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
@@ -1341,7 +1345,7 @@ DEV void prepare_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const 
 }
 
 DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
-                        bool meas, double* grad = nullptr) {
+                        bool meas, double* grad = nullptr, bool frozen = false) {
     const int n = m.n;
     cd* G = L.mat;
     Timer tm;
@@ -1365,6 +1369,7 @@ DEV void sweep_left_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         for (int nf = 2; nf >= 1; --nf) {
             if (!m.active[nf - 1]) continue;
             if (grad) grad_extract(c, m, phi_t, as_lds(G), t, nf, grad);
+            else if (frozen) {}
             else if (m.dk > 0) site_loop_delayed(c, m, ch, L, G, t, nf, false, seed);
             else site_loop_lds(c, m, ch, L, G, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
@@ -1475,6 +1480,31 @@ DEV void body_grad(const Ctx& c, const LaunchArgs& a, int* slot) {
             global_from_lds(c, ch.g, L.mat, m.n);
         } else {
             sweep_left_global(c, m, ch, L, seed, false, grad);
+        }
+        c.sync();
+    }
+}
+
+// Measurement pass: one left sweep with the fields frozen (no proposals, the
+// RNG stream untouched) measuring the equal-time estimators at every slice,
+// as sweep_L of Local_sweep measures them (local_sweep.f90:96-115) -- the
+// measurement of a configuration reached by a global move.  G, the UDV factors
+// and the checkpoint lists end in the state sweep_L leaves them in.
+template <bool kLds>
+DEV void body_measure(const Ctx& c, const LaunchArgs& a, int* slot) {
+    for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
+        const ChainDev& ch = a.chains[a.ids[q]];
+        const ModelDev& m = a.models[ch.model];
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
+        int seed = *ch.seed;
+        for (int s = 0; s < a.nsweep; ++s) {
+            if (kLds) {
+                lds_from_global(c, L.mat, ch.g, m.n);
+                sweep_left_lds(c, m, ch, L, seed, true, nullptr, true);
+                global_from_lds(c, ch.g, L.mat, m.n);
+            } else {
+                sweep_left_global(c, m, ch, L, seed, true, nullptr, true);
+            }
         }
         c.sync();
     }

@@ -104,24 +104,10 @@ def uniform_shift_proposals(solver, rng, width):
     return out
 
 
-def metropolis_global_device(solver, net, lam, sigma, rng, seed=None):
-    """One exact global Metropolis-Hastings step for every chain with the
-    proposal of bafqmc_amd.proposal.LearnedProposal (phi' = phi + lam f(phi) +
-    sigma xi, sigma per channel, already scaled), drawn, evaluated and undone
-    on the device (bafq_global_propose / bafq_global_finish): no fields cross
-    to the host.  net: a proposal.ConvNet or None (no drift).  Returns the
-    boolean acceptance per chain; G is current afterwards."""
+def _accept_device(solver, out, st0, rng):
+    """Exact acceptance of a device move from its per-chain sums (the out
+    layout of bafq_global_propose / bafq_hmc_propose), then bafq_global_finish."""
     import ctypes as C
-    lib = solver.lib
-    n = solver.nchains
-    st0 = solver.stats()
-    out = np.zeros((n, 5))
-    sig = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, dtype=np.float64), (2,)))
-    if seed is None:
-        seed = int(rng.integers(0, 2 ** 63))
-    solver._check(lib.bafq_global_propose(solver.h, net.h if net is not None else None, float(lam),
-                                          sig.ctypes.data_as(C.POINTER(C.c_double)), C.c_uint64(seed),
-                                          out.ctypes.data_as(C.POINTER(C.c_double))), "bafq_global_propose")
     st1 = solver.stats()
     p = solver.params(0)
     if p.pairing:
@@ -132,12 +118,66 @@ def metropolis_global_device(solver, net, lam, sigma, rng, seed=None):
     else:
         lw0 = -0.5 * out[:, 1] + 2.0 * st0[:, LOGDET]
         lw1 = -0.5 * out[:, 2] + 2.0 * st1[:, LOGDET]
-    u = rng.random(n)
-    accept = np.log(np.maximum(u, 1e-300)) < lw1 - lw0 + out[:, 0]
+    u = rng.random(solver.nchains)
+    log_a = lw1 - lw0 + out[:, 0]
+    accept = np.log(np.maximum(u, 1e-300)) < log_a
     flags = np.ascontiguousarray(accept.astype(np.int32))
-    solver._check(lib.bafq_global_finish(solver.h, flags.ctypes.data_as(C.POINTER(C.c_int32))),
+    solver._check(solver.lib.bafq_global_finish(solver.h, flags.ctypes.data_as(C.POINTER(C.c_int32))),
                   "bafq_global_finish")
-    return accept
+    return accept, log_a
+
+
+def metropolis_global_device(solver, net, lam, sigma, rng, seed=None):
+    """One exact global Metropolis-Hastings step for every chain with the
+    proposal of bafqmc_amd.proposal.LearnedProposal (phi' = phi + lam f(phi) +
+    sigma xi, sigma per channel, already scaled), drawn, evaluated and undone
+    on the device (bafq_global_propose / bafq_global_finish): no fields cross
+    to the host.  net: a proposal.ConvNet or None (no drift).  Returns the
+    boolean acceptance per chain; G is current afterwards."""
+    import ctypes as C
+    lib = solver.lib
+    st0 = solver.stats()
+    out = np.zeros((solver.nchains, 5))
+    sig = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, dtype=np.float64), (2,)))
+    if seed is None:
+        seed = int(rng.integers(0, 2 ** 63))
+    solver._check(lib.bafq_global_propose(solver.h, net.h if net is not None else None, float(lam),
+                                          sig.ctypes.data_as(C.POINTER(C.c_double)), C.c_uint64(seed),
+                                          out.ctypes.data_as(C.POINTER(C.c_double))), "bafq_global_propose")
+    return _accept_device(solver, out, st0, rng)[0]
+
+
+def gaussian_centers(params):
+    """Centre of the Gaussian part of log W per channel: the pairing weight's
+    normal-ordering factor exp(-alpha_2 sum phi_2) shifts the U2 channel's
+    Gaussian to -alpha_2 (src/pairing/src/operator_Hubbard.f90:100-137); every
+    other centre is 0."""
+    if params.pairing and params.u2 < -1e-10:
+        return np.array([0.0, -math.sqrt(-2.0 * params.u2 * params.beta / params.ltrot)])
+    return np.zeros(2)
+
+
+def hmc_global_device(solver, net, angle, steps, rng, seed=None, return_log_a=False, force_scale=1.0):
+    """One exact global Metropolis-Hastings step for every chain with the split
+    Hamiltonian move of bafq_hmc_propose: momenta p ~ N(0, 1), `steps` steps of
+    a half kick by the network's force (force_scale per channel times its
+    output), a rotation of (phi - c, p) by `angle` and a half kick, then the
+    exact acceptance with the true weight and the kinetic energies.  Returns
+    the boolean acceptance per chain."""
+    import ctypes as C
+    lib = solver.lib
+    st0 = solver.stats()
+    out = np.zeros((solver.nchains, 5))
+    cen = np.ascontiguousarray(gaussian_centers(solver.params(0)), dtype=np.float64)
+    fsc = np.ascontiguousarray(np.broadcast_to(np.asarray(force_scale, dtype=np.float64), (2,)))
+    if seed is None:
+        seed = int(rng.integers(0, 2 ** 63))
+    solver._check(lib.bafq_hmc_propose(solver.h, net.h if net is not None else None, float(angle), int(steps),
+                                       cen.ctypes.data_as(C.POINTER(C.c_double)),
+                                       fsc.ctypes.data_as(C.POINTER(C.c_double)), C.c_uint64(seed),
+                                       out.ctypes.data_as(C.POINTER(C.c_double))), "bafq_hmc_propose")
+    accept, log_a = _accept_device(solver, out, st0, rng)
+    return (accept, log_a) if return_log_a else accept
 
 
 def grad_log_weights(solver):

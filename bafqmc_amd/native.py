@@ -64,15 +64,20 @@ SYMBOLS = {
     "bafq_net_get_params": (C.c_int, [C.c_void_p, c_float_p]),
     "bafq_net_forward": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p]),
     "bafq_net_gradient": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, c_double_p, c_float_p]),
+    "bafq_net_time_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p]),
     "bafq_net_train": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, C.c_int, C.c_float, c_double_p]),
     "bafq_global_propose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, c_double_p, C.c_uint64, c_double_p]),
     "bafq_global_finish": (C.c_int, [C.c_void_p, c_int32_p]),
+    "bafq_hmc_propose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, c_double_p, c_double_p, C.c_uint64,
+                                   c_double_p]),
+    "bafq_measure_pass": (C.c_int, [C.c_void_p, C.c_int]),
     "bafq_grad_log_weight": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_get_green": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_debug_gemm": (C.c_int, [C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, c_double_p]),
     "bafq_debug_qrp": (C.c_int, [C.c_int, c_double_p, c_double_p, c_int32_p, c_double_p, C.c_int]),
     "bafq_debug_mfma_probe": (C.c_int, [c_double_p]),
+    "bafq_debug_mfma32_probe": (C.c_int, [C.POINTER(C.c_float)]),
     "bafq_debug_lane_ops": (C.c_int, [c_double_p, c_int32_p, c_double_p]),
 }
 
@@ -117,6 +122,12 @@ class NativeLibrary:
         out = np.zeros((64, 64, 4))
         if self.lib.bafq_debug_mfma_probe(dptr(out)):
             raise RuntimeError("bafq_debug_mfma_probe failed")
+        return out
+
+    def debug_mfma32_probe(self):
+        out = np.zeros((64, 64, 4), np.float32)
+        if self.lib.bafq_debug_mfma32_probe(out.ctypes.data_as(C.POINTER(C.c_float))):
+            raise RuntimeError("bafq_debug_mfma32_probe failed")
         return out
 
     def debug_lane_ops(self, v, idx):

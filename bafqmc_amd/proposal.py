@@ -98,6 +98,13 @@ class ConvNet:
         self._check(self.lib.bafq_net_forward(self.h, x.shape[0], fptr(x), fptr(y)), "bafq_net_forward")
         return y
 
+    def time_forward(self, batch, reps=5):
+        """Device milliseconds of one forward pass over `batch` resident inputs."""
+        ms = C.c_double()
+        self._check(self.lib.bafq_net_time_forward(self.h, int(batch), int(reps), C.byref(ms)),
+                    "bafq_net_time_forward")
+        return ms.value
+
     def gradient(self, x, target):
         x = self._batch(x)
         t = np.ascontiguousarray(target, dtype=np.float32)
@@ -133,18 +140,23 @@ def active_channels(params):
     return np.array([abs(params.u1) > 1e-10, abs(params.u2) > 1e-10], dtype=np.float64)
 
 
-def train_on_sweeps(net, x, t, epochs, lr, rng, holdout=0.25, batch=None):
-    """Fit the network to the one-sweep displacement t of fields x (both in the
-    network layout) by minibatch Adam; returns the per-channel residual
-    variance on the held-out part and the held-out R^2 (variance explained)."""
+def train_on_sweeps(net, x, t, epochs, lr, rng, holdout=0.25, batch=None, budget_s=None):
+    """Fit the network to targets t of fields x (both in the network layout)
+    by minibatch Adam (at most `epochs` epochs, or until `budget_s` seconds
+    have passed); returns the per-channel residual variance on the held-out
+    part and the held-out R^2 (variance explained)."""
+    import time
     batch = min(batch or net.max_batch, net.max_batch)
     n = len(x)
     ntr = n - max(batch, int(n * holdout))
+    t0 = time.perf_counter()
     for ep in range(epochs):
         idx = rng.permutation(ntr)
         for b in range(0, ntr - batch + 1, batch):
             j = idx[b:b + batch]
             net.train(x[j], t[j], 1, lr if ep < (3 * epochs) // 4 else lr / 3)
+        if budget_s is not None and time.perf_counter() - t0 > budget_s:
+            break
     res = []
     for i in range(ntr, n, batch):
         res.append(net.forward(x[i:i + batch]) - t[i:i + batch])
@@ -190,3 +202,25 @@ class LearnedProposal:
         w = np.where(on, 1.0 / (2.0 * np.where(on, self.sigma, 1.0) ** 2), 0.0)
         lq = np.sum(w * ((P - F - f0) ** 2 - (F - P - f1) ** 2), axis=(1, 2, 3))
         return list(P), lq
+
+
+def collect_forces(solver, params, sweeps, max_grad=50.0):
+    """Training pairs of the Hamiltonian moves' network from the plain sampler:
+    the fields phi of every chain after each of `sweeps` local sweeps and the
+    residual force d log W / d phi + (phi - c) (the part of the gradient beyond
+    the Gaussian centred at c, bafqmc_amd.global_moves.gaussian_centers) on the
+    channels that couple.  Chains whose gradient exceeds max_grad (near a pole
+    of (1 - B)^-1) are left out.  Returns (x, target) in the network layout."""
+    from .global_moves import gaussian_centers, grad_log_weights
+    cen = gaussian_centers(params)
+    mask = active_channels(params)
+    xs, ts = [], []
+    for _ in range(sweeps):
+        fields, _ = solver.get_fields()
+        for f, g in zip(fields, grad_log_weights(solver)):
+            if np.all(np.isfinite(g)) and np.max(np.abs(g)) < max_grad:
+                xs.append(f)
+                ts.append((g + (f - cen)) * mask)
+        solver.sweep(1, measure=False)
+    solver.read_bin()
+    return to_net(xs, params), to_net(ts, params)

@@ -159,6 +159,10 @@ class BatchedSolver:
     def sweep(self, n=1, measure=True):
         self._check(self.lib.bafq_sweep(self.h, int(n), 1 if measure else 0), "bafq_sweep")
 
+    def measure_pass(self, n=1):
+        """n left passes with frozen fields, measuring every slice (bafq_measure_pass)."""
+        self._check(self.lib.bafq_measure_pass(self.h, int(n)), "bafq_measure_pass")
+
     def last_kernel_ms(self):
         v = C.c_double()
         self._check(self.lib.bafq_last_kernel_ms(self.h, C.byref(v)), "bafq_last_kernel_ms")

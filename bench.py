@@ -1,28 +1,38 @@
 #!/usr/bin/env python3
-"""Throughput of the batched BAFQMC sweep on one MI355X.
+"""Effective S_SF(K) samples/s of the batched BAFQMC sampler on one MI355X,
+learned-proposal form vs the plain batched sweep at equal error bars.
 
-Workload (default): 512 independent Nambu-pairing chains on a 6x6 triangular
-lattice (reduced Green matrices N = 2 Ns = 72), t=1, U1=1, U2=-0.5, Delta=0.5,
-beta=8, mu=-5, dtau=0.1 (Ltrot=80), Nwrap=10, shiftLoc=1.0, fields i.i.d. N(0,1)
-from per-chain seeds.  One step = one batched sweep (sweep_L + sweep_R of every
-chain, 2 Ltrot measurements each) followed by reading the bin.
+Workload (default, `--config c4`): 256 independent number-conserving chains
+on a 9x9 triangular lattice (N = 81), t = 1, U1 = 1, U2 = 0, mu = -5,
+beta = 8, dtau = 0.1 (Ltrot = 80), Nwrap = 10, shiftLoc = 1.5, fields i.i.d.
+N(0,1) from per-chain seeds; proposal network: 4 periodic 3x3x3 convolution
+layers, 32 hidden channels, FP32, He-normal initialisation from a seed.
+`--config c5` runs the 12x12 pairing workload (N = 288, Ltrot = 100,
+128 chains) the same way.
 
-Metric: effective independent S_SF(K) samples/s = sum over chains of
-(bins/s) / (2 tau_int), tau_int measured on the per-chain Re S_SF(K) bin
-series (each standardised, then pooled over chains).
-Default (--mode learned): the chains run K plain measured sweeps; the
-proposal network is trained on d log W / d phi of the fields of 12 more (the
-device's gradient pass); then the chains run K steps of 8 learned global
-moves (Metropolis-adjusted Langevin with the network's gradient, drawn and
-accepted on the device) + one measured sweep.  `value` is the learned form's
-rate; the plain rate, both error bars and their ratio at equal error bars are
-in `config`.  --mode plain reports the plain sweep alone.
-Chains whose stabilisation drift reaches the reference's "ortho unstable"
-threshold (5.5e-5, src/number_conserving/src/stabilization.f90:320,358) -- the
-bosonic weight drives some chains towards poles of (1 - B)^-1 -- are run but
-not counted.
+* plain step: one batched sweep (sweep_L + sweep_R of every chain, 2 Ltrot
+  measurements each), then the bin is read;
+* learned step: one Hamiltonian global move of every chain (momenta drawn on
+  the device, the Gaussian part of log W integrated exactly, the determinant
+  part by the network's learned force, exact Metropolis-Hastings with the
+  fresh FP64 determinant; bafq_hmc_propose / bafq_global_finish), then one
+  measurement pass (a left sweep with frozen fields measuring every slice,
+  bafq_measure_pass), then the bin is read.
 
-Prints one JSON line.  ``--gpus N`` with N > 1 is not supported (one GPU).
+The network is trained before the learned phase on the residual force
+d log W / d phi + phi of fields from the plain sampler (the device's gradient
+pass).  Each form runs `--steps` K timed steps (the timed region: throughput)
+inside a longer run of `--stat-steps` S steps of the same form (the
+statistics: tau_int with its window and error, and the standard error of the
+mean from the spread of the chains' time averages).  Every chain is counted.
+
+value: plain-sweep-equivalent effective independent S_SF(K) samples/s of the
+learned form at equal error bars, i.e. sigma^2 / (SE_learned^2 T_learned)
+with sigma^2 the per-step variance of the plain form's S_SF(K) and T the time
+the S statistics steps take at the timed rate; config.gain_at_equal_error_bars
+is the learned/plain ratio of 1 / (SE^2 T).
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: one GPU.
 """
 from __future__ import annotations
 
@@ -40,32 +50,46 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 UNSTABLE_DRIFT = 5.5e-5
 PEAK_HBM_GBS = 8000.0
 
 CONFIGS = {
-    "c2": dict(name="pairing 6x6 beta=8 Ltrot=80 U1=1 U2=-0.5 Delta=0.5, 512 chains",
-               p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=80, beta=8.0,
-                      nwrap=10, shift_loc=1.0), chains=512),
-    "c4": dict(name="number-conserving 9x9 beta=8 Ltrot=80 U1=1 U2=0, 256 chains",
+    "c4": dict(name="number-conserving 9x9 beta=8 Ltrot=80 U1=1 U2=0 mu=-5, 256 chains, "
+                    "4-layer 32-channel FP32 proposal network",
                p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=80, beta=8.0, nwrap=10,
-                      shift_loc=1.5), chains=256),
+                      shift_loc=1.5), chains=256, net=dict(layers=4, hidden=32)),
+    "c5": dict(name="pairing 12x12 beta=10 Ltrot=100 U1=1 U2=-0.5 Delta=0.3 mu=-5, 128 chains, "
+                    "4-layer 32-channel FP32 proposal network",
+               p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12, ltrot=100, beta=10.0,
+                      nwrap=10, shift_loc=1.0), chains=128, net=dict(layers=4, hidden=32)),
+    "c2": dict(name="pairing 6x6 beta=8 Ltrot=80 U1=1 U2=-0.5 Delta=0.5 mu=-5, 512 chains",
+               p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=80, beta=8.0,
+                      nwrap=10, shift_loc=1.0), chains=512, net=dict(layers=4, hidden=32)),
     "uneven": dict(name="uneven batch: number-conserving 3x3/6x6/9x9/12x12 x beta 4/8 x U2 0/-0.5, "
                         "U1=1 mu=-5 dtau=0.1, 16 chains each, 256 chains",
                    models=[dict(pairing=False, u1=1.0, u2=u2, mu=-5.0, nlx=L, nly=L, ltrot=int(round(b / 0.1)),
                                 beta=b, nwrap=10, shift_loc=1.5)
                            for L in (3, 6, 9, 12) for b in (4.0, 8.0) for u2 in (0.0, -0.5)],
                    chains=256),
-    "c5": dict(name="pairing 12x12 beta=10 Ltrot=100 U1=1 U2=-0.5 Delta=0.3, 128 chains",
-               p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12, ltrot=100, beta=10.0,
-                      nwrap=10, shift_loc=1.0), chains=128),
-    "nc12": dict(name="number-conserving 12x12 beta=8 Ltrot=80 U1=1 U2=-0.5, 96 chains",
+    "nc12": dict(name="number-conserving 12x12 beta=8 Ltrot=80 U1=1 U2=-0.5 mu=-5, 96 chains",
                  p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=12, nly=12, ltrot=80, beta=8.0, nwrap=10,
-                        shift_loc=1.5), chains=96),
-    "c1": dict(name="number-conserving 3x3 beta=4 Ltrot=40 U1=1 U2=0, 512 chains",
+                        shift_loc=1.5), chains=96, net=dict(layers=4, hidden=32)),
+    "c1": dict(name="number-conserving 3x3 beta=4 Ltrot=40 U1=1 U2=0 mu=-5, 512 chains",
                p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=3, nly=3, ltrot=40, beta=4.0, nwrap=10,
-                      shift_loc=1.5), chains=512),
+                      shift_loc=1.5), chains=512, net=dict(layers=3, hidden=8)),
 }
+
+METRIC = "effective S_SF(K) samples/sec, learned vs plain sweep, 1 MI355X at equal σ"
+
+
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def nch_of(p):
+    return int(abs(p.u1) > 1e-10) + int(abs(p.u2) > 1e-10)
 
 
 def flops_per_chain_sweep(n, ns, ltrot, nwrap, nch, c_meas=40.0):
@@ -76,11 +100,34 @@ def flops_per_chain_sweep(n, ns, ltrot, nwrap, nch, c_meas=40.0):
             + 2 * ltrot * c_meas * ns ** 2)
 
 
-def measured_traffic(config, nchains):
-    """HBM bytes per launch of the sweep kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when it covers this workload."""
-    f = ROOT / "profiles" / "k_sweep_lds_pairing6x6_512chains_hbm.json"
-    if config != "c2" or nchains != 512 or not f.exists():
+def flops_per_measure_pass(n, ns, ltrot, nwrap, c_meas=40.0):
+    """One left pass with frozen fields: the kinetic GEMMs and stabilisations of
+    sweep_L, no updates, one measurement per slice."""
+    return ltrot * 24 * n ** 3 + (ltrot / nwrap) * 56 * n ** 3 + ltrot * c_meas * ns ** 2
+
+
+def flops_per_prepare(n, ltrot, nwrap):
+    """Local_sweep_pre: one B product per slice on the U factor (8 N^3), stab_UR per
+    checkpoint (QR 16/3 + form Q 16/3 + triangular product 4 N^3), one stab_green (41.3 N^3)."""
+    return ltrot * 8 * n ** 3 + (ltrot / nwrap) * 14.7 * n ** 3 + 41.3 * n ** 3
+
+
+def net_flops(desc, batch):
+    """FP32 flops of one forward pass of the convolution network."""
+    vol = desc["lt"] * desc["ly"] * desc["lx"]
+    tot = 0
+    for l in range(desc["layers"]):
+        cin = desc["channels"] if l == 0 else desc["hidden"]
+        cout = desc["channels"] if l == desc["layers"] - 1 else desc["hidden"]
+        tot += 2 * 27 * cin * cout * vol
+    return tot * batch
+
+
+def measured_traffic(config, kernel):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated) of this workload, when there is one."""
+    f = ROOT / "profiles" / f"{kernel}_{config}_hbm.json"
+    if not f.exists():
         return None
     return json.loads(f.read_text())["traffic_bytes_per_launch"]
 
@@ -116,7 +163,7 @@ def cpu_baseline(cfg, tau, budget_s=12.0):
                 "cores": cores, "kind": "reference",
                 "sample": (f"reference Fortran (amdflang+MKL sequential) mpirun -np {cores}, one chain per rank, "
                            f"{nbin} bins x 1 sweep of the same workload in {wall:.1f} s = {rate:.3f} chain-sweeps/s; "
-                           f"divided by 2*tau_int={2 * tau:.2f} from the GPU run (same Markov chain)")}
+                           f"divided by 2*tau_int={2 * tau:.2f} of the plain GPU run (same Markov chain)")}
     except Exception as e:  # noqa: BLE001 - fall back to the NumPy port of the oracle
         from oracle import bafq_oracle as O
         p = O.Params(**{k: d[k] for k in ("pairing", "u1", "u2", "mu", "nlx", "nly", "ltrot", "beta", "nwrap",
@@ -136,185 +183,254 @@ def cpu_baseline(cfg, tau, budget_s=12.0):
                           f"reference binary unavailable: {e}"}
 
 
-METRIC = "effective S_SF(K) samples/sec, learned vs plain sweep, 1 MI355X at equal \u03c3"
-
-
-def roofline(p, nchains, kernel_ms, config):
-    """FP64 roofline of the sweep kernel from its HIP-event launch times."""
-    nch = int(abs(p.u1) > 1e-10) + int(abs(p.u2) > 1e-10)
-    f_chain = flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch)
-    avg_ms = float(np.mean(kernel_ms))
-    achieved = f_chain * nchains / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else float("nan")
-    return {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(config, nchains),
-            "kernel_ms": avg_ms, "flops_per_launch": f_chain * nchains}
+def roofline(name, flops, kernel_ms, bound="mfma", peak=PEAK_FP64_TFLOPS, traffic=None):
+    avg = float(np.mean(kernel_ms))
+    achieved = flops / (avg * 1e-3) / 1e12 if avg > 0 else float("nan")
+    return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic, "kernel_ms": avg, "flops_per_launch": flops}
 
 
 def choose_schedule(args, p, nchains):
     if args.schedule == "auto":
         from bafqmc_amd import autotune
         from bafqmc_amd.native import load_library
-        tuned = autotune.tune_model(load_library(), p, chains=nchains, sweeps=2)
+        tuned = autotune.tune_model(load_library(), p, chains=nchains, sweeps=2, tune_nwrap=args.tune_nwrap)
         return [tuned.schedule], tuned
     if args.schedule == "default":
         return None, None
     return [tuple(int(v) for v in args.schedule.split(","))], None
 
 
-def progress(msg):
-    """A progress line on stderr (the JSON line alone goes to stdout)."""
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+def run_phase(s, step, K, S, label):
+    """max(K, S) steps of one form; the first K are the timed region.  Returns
+    (per-chain S_SF(K) series [chains, max(K, S)], wall seconds of the K timed
+    steps, per-step kernel milliseconds reported by `step`)."""
+    n = max(K, S)
+    sf = np.zeros((s.nchains, n))
+    kms = []
+    wall_k = float("nan")
+    progress(f"{label}: {K} timed steps, {n} steps for the statistics")
+    t0 = time.perf_counter()
+    for k in range(n):
+        kms.append(step())
+        obs, _ = s.read_bin_raw()
+        sf[:, k] = obs[:, 20] / obs[:, 32]
+        if k + 1 == K:
+            wall_k = time.perf_counter() - t0
+        if (k + 1) % 50 == 0:
+            progress(f"{label} step {k + 1}/{n}")
+    return sf, wall_k, kms
 
 
-def pooled_tau(series):
-    """tau_int of per-chain bin series pooled over chains, each chain's series
-    standardised first: chains near a pole of (1 - B)^-1 have S_SF(K)
-    fluctuations orders of magnitude above the rest and would otherwise set
-    the pooled autocorrelation alone."""
-    from bafqmc_amd.analysis import tau_int
-    x = series - series.mean(axis=1, keepdims=True)
-    sd = x.std(axis=1, keepdims=True)
-    return tau_int(x / np.where(sd > 0, sd, 1.0))
-
-
-def _efficiency(series, wall, stable):
-    """Per-chain bin series -> (tau_int, effective samples/s, 1 / (SE^2 wall)).
-    SE is the standard error of the mean over the stable chains' time
-    averages (independent chains), so 1 / (SE^2 wall) compares methods at
-    equal error bars: its ratio is the ratio of the times to reach one."""
-    x = series[stable]
-    tau, _, _ = pooled_tau(x)
-    steps = x.shape[1]
-    eff = len(x) * (steps / wall) / (2.0 * tau)
-    means = x.mean(axis=1)
-    se = float(np.std(means, ddof=1) / math.sqrt(len(means)))
-    return tau, eff, 1.0 / (se * se * wall), se
+def statistics(sf, wall_k, K, label):
+    """Throughput of the timed steps and statistics of the whole series."""
+    from bafqmc_amd.analysis import chain_mean_error, tau_int_pooled
+    finite = np.all(np.isfinite(sf), axis=1)
+    x = sf if finite.all() else sf[finite]
+    tau = tau_int_pooled(x)
+    err = chain_mean_error(x)
+    steps_per_s = K / wall_k
+    t_stat = x.shape[1] / steps_per_s                   # time the statistics steps take at the timed rate
+    var = float(np.var(x))
+    return {"ms_per_step": 1000.0 * wall_k / K, "steps_per_s": steps_per_s,
+            "tau_int_sf_k": tau["tau"], "tau_err": tau["err"], "tau_window": tau["window"],
+            "stat_steps": tau["length"], "series_shorter_than_50_tau": tau["short"],
+            "eff_samples_per_s": len(x) * steps_per_s / (2.0 * tau["tau"]),
+            "mean_sf_k": err["mean"], "stderr_sf_k": err["stderr"], "stderr_sf_k_blocked": err["stderr_blocked"],
+            "variance_per_step": var, "inverse_variance_per_s": 1.0 / (err["stderr"] ** 2 * t_stat),
+            "nonfinite_chains": int((~finite).sum())}
 
 
 def run_learned(args, cfg, p, spec, nchains):
-    """Plain sweeps vs sweeps interleaved with learned global moves on the same
-    chains (the second phase continues from the first)."""
+    """Plain sweeps, then the learned form on the same chains (it continues
+    from the plain phase's fields), compared at equal error bars."""
     from bafqmc_amd import global_moves as GM
     from bafqmc_amd.analysis import chain_seeds
     from bafqmc_amd.model import conf_set_batch
-    from bafqmc_amd.proposal import ConvNet, LearnedProposal, to_net, train_on_sweeps
+    from bafqmc_amd.proposal import ConvNet, collect_forces, train_on_sweeps
     from bafqmc_amd.solver import BatchedSolver
     phis, seeds = conf_set_batch(p, chain_seeds(13579113, nchains))
     schedule, tuned = choose_schedule(args, p, nchains)
+    if tuned is not None and tuned.nwrap != p.nwrap:
+        import dataclasses
+        from bafqmc_amd.model import ModelSpec
+        p = dataclasses.replace(p, nwrap=tuned.nwrap)
+        spec = ModelSpec(p)
     s = BatchedSolver([spec], [0] * nchains, schedule=schedule)
     s.set_fields(list(phis), seeds)
     s.prepare()
     for _ in range(args.warmup):
         s.sweep(1)
         s.read_bin()
-    # ---- plain phase (also the training data: fields before/after each sweep)
-    K = args.steps
-    sf0 = np.zeros((nchains, K))
-    xs, ts = [], []
-    kernel_ms = []
-    progress(f"plain phase: {K} steps")
-    t0 = time.perf_counter()
-    for k in range(K):
+    K, S = args.steps, args.stat_steps
+
+    def plain_step():
         s.sweep(1)
-        kernel_ms.append(s.last_kernel_ms())
-        obs, st = s.read_bin_raw()
-        sf0[:, k] = obs[:, 20] / obs[:, 32]
-        if (k + 1) % 10 == 0:
-            progress(f"plain step {k + 1}/{K}")
-    wall0 = time.perf_counter() - t0
-    ntrain = max(1, min(12, 8192 // nchains + 1))
-    for _ in range(ntrain):
-        f0, _ = s.get_fields()
-        if args.target == "grad":
-            # chains near a pole of (1 - B)^-1 have gradients orders of magnitude
-            # above the rest: they are left out of the fit
-            for f, g in zip(f0, GM.grad_log_weights(s)):
-                if np.all(np.isfinite(g)) and np.max(np.abs(g)) < 50.0:
-                    xs.append(f)
-                    ts.append(g)
-            s.sweep(1, measure=False)
-            continue
-        s.sweep(1, measure=False)
-        f1, _ = s.get_fields()
-        xs += f0
-        ts += [b - a for a, b in zip(f0, f1)]
-    s.read_bin()
-    progress(f"training the network on {len(xs)} " + ("gradients" if args.target == "grad" else "sweep displacements"))
+        return s.last_kernel_ms()
+
+    sf0, wall0, sweep_ms = run_phase(s, plain_step, K, S, "plain")
+    st_plain = s.stats()
+    # ---- network: residual force of fields from the plain sampler
+    hidden = args.hidden or cfg["net"]["hidden"]
+    layers = args.layers or cfg["net"]["layers"]
+    ntrain = max(2, min(12, 4096 // nchains + 1))
+    progress(f"collecting the force of {ntrain} sweeps of fields")
     t1 = time.perf_counter()
-    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, args.hidden, args.layers, max_batch=min(512, nchains), seed=1)
-    var_c, r2 = train_on_sweeps(net, to_net(xs, p), to_net(ts, p), 20, 3e-3, np.random.default_rng(0), batch=64)
+    x, t = collect_forces(s, p, ntrain)
+    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, hidden, layers, max_batch=min(256, nchains), seed=1)
+    progress(f"training the network ({net.nparam} parameters) on {len(x)} fields")
+    # targets normalised per channel; the move multiplies the output back
+    t64 = t.astype(np.float64)
+    force_rms = np.sqrt(np.mean(t64 ** 2, axis=(0, 2, 3, 4)))
+    fscale = np.where(force_rms > 0, force_rms, 1.0)
+    tn = (t64 / fscale[None, :, None, None, None]).astype(np.float32)
+    _, r2 = train_on_sweeps(net, x, tn, args.epochs, 1e-3, np.random.default_rng(0), batch=32,
+                            budget_s=args.train_s)
     t_train = time.perf_counter() - t1
-    sigma = np.sqrt(var_c)
-    progress(f"trained in {t_train:.1f} s, held-out R2 {r2:.4f}")
+    force_rms = float(np.sqrt(np.mean(t64 ** 2)))
+    progress(f"trained in {t_train:.1f} s, held-out R2 {r2:.4f}, residual force rms {force_rms:.3g}")
     rng = np.random.default_rng(2468)
-    # step size: the largest mean squared jump over a few trial moves.  Sweep
-    # target: phi' = phi + lam f + sqrt(lam) sigma xi; gradient target (MALA
-    # with step h = 2 lam): phi' = phi + (h / 2) grad + sqrt(h) xi
-    if args.target == "grad":
-        sigma = np.array([math.sqrt(2.0), math.sqrt(2.0)])
-        lams = (0.005, 0.015, 0.05, 0.15)
-    else:
-        lams = (0.1, 0.25, 0.5, 1.0)
+    # ---- integrator: steps of a quarter rotation in total, the fastest by trial
     fields, sd = s.get_fields()
     trials = {}
-    for lam in lams:
+    for steps in args.hmc_steps:
         s.set_fields(fields, sd)
         s.prepare()
-        prop = LearnedProposal(net, p, sigma, lam)
-        f, _ = s.get_fields()
-        acc = GM.metropolis_global_device(s, net, prop.lam, prop.sigma, rng)
-        g, _ = s.get_fields()
-        trials[lam] = (float(acc.mean()), sum(float(np.sum((a - b) ** 2)) for a, b in zip(g, f)) / (nchains * f[0].size))
-    lam = max(trials, key=lambda l: trials[l][1])
+        acc, tm = 0.0, time.perf_counter()
+        for _ in range(3):
+            acc += float(GM.hmc_global_device(s, net, 0.5 * math.pi / steps, steps, rng, force_scale=fscale).mean())
+            s.measure_pass(1)
+        trials[steps] = (acc / 3, (time.perf_counter() - tm) / 3)
+    s.read_bin()
+    hmc_steps = max(trials, key=lambda k: trials[k][0] / trials[k][1])
+    angle = 0.5 * math.pi / hmc_steps
     s.set_fields(fields, sd)
     s.prepare()
-    prop = LearnedProposal(net, p, sigma, lam)
-    # ---- hybrid phase: one learned global move + one measured sweep per step
-    sf1 = np.zeros((nchains, K))
-    accepted = 0
-    progress(f"learned phase: {K} steps, lambda {lam}")
-    t0 = time.perf_counter()
-    for k in range(K):
-        for _ in range(args.moves):
-            accepted += int(GM.metropolis_global_device(s, net, prop.lam, prop.sigma, rng).sum())
-        s.sweep(1)
-        obs, st = s.read_bin_raw()
-        sf1[:, k] = obs[:, 20] / obs[:, 32]
-        if (k + 1) % 10 == 0:
-            progress(f"learned step {k + 1}/{K}")
-    wall1 = time.perf_counter() - t0
-    stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf0), axis=1) & np.all(np.isfinite(sf1), axis=1)
-    tau0, eff0, e0, se0 = _efficiency(sf0, wall0, stable)
-    tau1, eff1, e1, se1 = _efficiency(sf1, wall1, stable)
+    s.read_bin()
+    accepted = []
+    phase_ms = {"move_prepare": [], "measure": []}
+
+    def learned_step():
+        a = GM.hmc_global_device(s, net, angle, hmc_steps, rng, force_scale=fscale)
+        phase_ms["move_prepare"].append(s.last_kernel_ms())
+        accepted.append(float(a.mean()))
+        s.measure_pass(1)
+        ms = s.last_kernel_ms()
+        phase_ms["measure"].append(ms)
+        return ms
+
+    sf1, wall1, _ = run_phase(s, learned_step, K, S, f"learned ({hmc_steps} step(s) of angle {angle:.3f})")
+    st_learned = s.stats()
+    conv_ms = net.time_forward(min(256, nchains), 5)
+    desc = dict(lt=p.ltrot, ly=p.nly, lx=p.nlx, channels=2, hidden=hidden, layers=layers)
+    a0 = statistics(sf0, wall0, K, "plain")
+    a1 = statistics(sf1, wall1, K, "learned")
+    gain = a1["inverse_variance_per_s"] / a0["inverse_variance_per_s"]
+    value = a1["inverse_variance_per_s"] * a0["variance_per_step"]
+    nch = nch_of(p)
+    kernels = {
+        "k_sweep_lds": roofline("k_sweep_lds", flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch) * nchains,
+                                sweep_ms),
+        "k_measure_lds": roofline("k_measure_lds", flops_per_measure_pass(p.n, p.ns, p.ltrot, p.nwrap) * nchains,
+                                  phase_ms["measure"]),
+        "k_prepare_lds": roofline("k_prepare_lds", flops_per_prepare(p.n, p.ltrot, p.nwrap) * nchains,
+                                  phase_ms["move_prepare"]),
+        "k_conv_mfma (one forward pass)": roofline("k_conv_mfma", net_flops(desc, min(256, nchains)), [conv_ms],
+                                                   peak=PEAK_FP32_TFLOPS),
+    }
+    if s.schedule()[0][0] != 1:
+        for k0, k1 in (("k_sweep_lds", "k_sweep"), ("k_measure_lds", "k_measure"), ("k_prepare_lds", "k_prepare")):
+            kernels[k1] = dict(kernels.pop(k0), kernel=k1)
+    # per learned step: network passes (hmc_steps + 1 per chunk), prepare, measurement
+    share = {"measure": float(np.mean(phase_ms["measure"])), "prepare": float(np.mean(phase_ms["move_prepare"])),
+             "conv": conv_ms * (hmc_steps + 1) * math.ceil(nchains / min(256, nchains))}
+    dominant = max(share, key=share.get)
+    top = {"measure": kernels.get("k_measure_lds", kernels.get("k_measure")),
+           "prepare": kernels.get("k_prepare_lds", kernels.get("k_prepare")),
+           "conv": kernels["k_conv_mfma (one forward pass)"]}[dominant]
+    top = dict(top)
+    top["traffic"] = measured_traffic(args.config, top["kernel"])
+    unstable = int(np.sum(np.maximum(st_plain[:, 4], st_learned[:, 4]) >= UNSTABLE_DRIFT))
     if args.dump:
-        np.savez(args.dump, sf_plain=sf0, sf_learned=sf1, stable=stable, wall_plain=wall0, wall_learned=wall1)
+        np.savez(args.dump, sf_plain=sf0, sf_learned=sf1, wall_plain=wall0, wall_learned=wall1)
     out = {
         "metric": METRIC,
-        "value": eff1, "unit": "effective S_SF(K) samples/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "value": value, "unit": "plain-sweep-equivalent effective S_SF(K) samples/s at equal error bars",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": 1000.0 * wall1 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (HS fields i.i.d. N(0,1) from per-chain seeds)",
-        "config": {"workload": cfg["name"], "chains": nchains, "stable_chains": int(stable.sum()),
-                   "value_is": "learned-proposal form (one learned global move, drawn and accepted on the device, "
-                               "+ one measured sweep per step)",
-                   "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1],
+        "config": {"workload": cfg["name"], "chains": nchains, "counted_chains": nchains,
+                   "unstable_chains": unstable,
+                   "value_is": (f"learned form: one Hamiltonian global move ({hmc_steps} step(s), exact MH with the "
+                                "FP64 determinant) + one measurement pass per step; value = sigma^2_plain / "
+                                "(SE_learned^2 T_learned)"),
+                   "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1], "nwrap": p.nwrap,
                                 "autotune_ms_per_sweep": tuned.table if tuned else None},
-                   "plain": {"ms_per_step": 1000.0 * wall0 / K, "tau_int_sf_k": tau0, "eff_samples_per_s": eff0,
-                             "stderr_sf_k": se0, "inverse_variance_per_s": e0},
-                   "learned": {"ms_per_step": 1000.0 * wall1 / K, "tau_int_sf_k": tau1, "eff_samples_per_s": eff1,
-                               "stderr_sf_k": se1, "inverse_variance_per_s": e1,
-                               "global_acceptance": accepted / (nchains * K * args.moves), "lambda": lam,
-                               "moves_per_sweep": args.moves,
-                               "lambda_trials": {str(k): v for k, v in trials.items()},
-                               "network": {"layers": args.layers, "hidden": args.hidden, "params": net.nparam,
-                                           "target": args.target},
-                               "train_s": t_train, "train_samples": len(xs), "heldout_r2": r2},
-                   "gain_eff_samples": eff1 / eff0, "gain_at_equal_error_bars": e1 / e0},
-        "roofline": roofline(p, nchains, kernel_ms, args.config),
+                   "plain": a0,
+                   "learned": dict(a1, global_acceptance=float(np.mean(accepted)), hmc_steps=hmc_steps,
+                                   angle=angle, hmc_trials={str(k): v for k, v in trials.items()},
+                                   step_ms={k: float(np.mean(v)) for k, v in phase_ms.items()},
+                                   network={"layers": layers, "hidden": hidden, "params": net.nparam,
+                                            "target": "residual force d log W/d phi + (phi - c)",
+                                            "heldout_r2": r2, "target_rms": force_rms,
+                                            "force_scale": list(map(float, fscale)), "train_s": t_train,
+                                            "train_samples": len(x)}),
+                   "gain_eff_samples": a1["eff_samples_per_s"] / a0["eff_samples_per_s"],
+                   "gain_at_equal_error_bars": gain, "target_gain": 10.0,
+                   "kernels": kernels, "learned_step_time_share_ms": share},
+        "roofline": top,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, tau0)
+        out["cpu_baseline"] = cpu_baseline(cfg, a0["tau_int_sf_k"])
     s.close()
     net.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_plain(args, cfg, p, spec, nchains):
+    from bafqmc_amd.analysis import chain_seeds
+    from bafqmc_amd.model import conf_set_batch
+    from bafqmc_amd.solver import BatchedSolver
+    phis, seeds = conf_set_batch(p, chain_seeds(13579113, nchains))
+    schedule, tuned = choose_schedule(args, p, nchains)
+    if tuned is not None and tuned.nwrap != p.nwrap:
+        import dataclasses
+        from bafqmc_amd.model import ModelSpec
+        p = dataclasses.replace(p, nwrap=tuned.nwrap)
+        spec = ModelSpec(p)
+    s = BatchedSolver([spec], [0] * nchains, schedule=schedule)
+    s.set_fields(list(phis), seeds)
+    s.prepare()
+    for _ in range(args.warmup):
+        s.sweep(1)
+        s.read_bin()
+    phase = np.zeros(8)
+
+    def step():
+        s.sweep(1)
+        return s.last_kernel_ms()
+
+    sf, wall, kms = run_phase(s, step, args.steps, args.stat_steps, "plain")
+    st = s.stats()
+    a = statistics(sf, wall, args.steps, "plain")
+    kern = "k_sweep_lds" if s.schedule()[0][0] == 1 else "k_sweep"
+    rf = roofline(kern, flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch_of(p)) * nchains, kms)
+    rf["traffic"] = measured_traffic(args.config, kern)
+    out = {
+        "metric": METRIC, "value": a["eff_samples_per_s"], "unit": "effective S_SF(K) samples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": a["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (HS fields i.i.d. N(0,1) from per-chain seeds)",
+        "config": {"workload": cfg["name"], "chains": nchains, "matrix_n": p.n, "ltrot": p.ltrot,
+                   "nwrap": p.nwrap, "value_is": "plain batched sweep (no global moves)", "plain": a,
+                   "unstable_chains": int(np.sum(st[:, 4] >= UNSTABLE_DRIFT)), "kernel": kern,
+                   "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1],
+                                "autotune_ms_per_sweep": tuned.table if tuned else None}},
+        "roofline": rf,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, a["tau_int_sf_k"])
+    s.close()
     print(json.dumps(out), flush=True)
 
 
@@ -323,7 +439,7 @@ def run_uneven(args, cfg, nchains):
     LDS path (N <= 96) and the device-memory path (12x12, N = 144) run as two
     persistent kernels in one HIP graph; schedules from the autotuner."""
     from bafqmc_amd import autotune
-    from bafqmc_amd.analysis import chain_seeds, tau_int
+    from bafqmc_amd.analysis import chain_seeds
     from bafqmc_amd.model import ModelSpec, conf_set_batch
     from bafqmc_amd.native import load_library
     from bafqmc_amd.solver import BatchedSolver
@@ -353,42 +469,29 @@ def run_uneven(args, cfg, nchains):
     for _ in range(args.warmup):
         s.sweep(1)
         s.read_bin()
-    sf = np.zeros((nchains, args.steps))
-    kernel_ms = []
-    progress(f"uneven batch: {args.steps} steps")
-    t0 = time.perf_counter()
-    for k in range(args.steps):
+
+    def step():
         s.sweep(1)
-        kernel_ms.append(s.last_kernel_ms())
-        obs, st = s.read_bin_raw()
-        sf[:, k] = obs[:, 20] / obs[:, 32]
-        progress(f"step {k + 1}/{args.steps}")
-    wall = time.perf_counter() - t0
-    stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf), axis=1)
-    tau, window, _ = pooled_tau(sf[stable])
-    value = int(stable.sum()) * (args.steps / wall) / (2.0 * tau)
-    flops = 0.0
-    for c in range(nchains):
-        p = params[chain_model[c]]
-        nch = int(abs(p.u1) > 1e-10) + int(abs(p.u2) > 1e-10)
-        flops += flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch)
-    avg_ms = float(np.mean(kernel_ms))
-    achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else float("nan")
+        return s.last_kernel_ms()
+
+    sf, wall, kms = run_phase(s, step, args.steps, args.stat_steps, "uneven batch")
+    st = s.stats()
+    a = statistics(sf, wall, args.steps, "uneven")
+    flops = sum(flops_per_chain_sweep(params[m].n, params[m].ns, params[m].ltrot, params[m].nwrap,
+                                      nch_of(params[m])) for m in chain_model)
     out = {
-        "metric": METRIC, "value": value, "unit": "effective S_SF(K) samples/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000.0 * wall / args.steps,
+        "metric": METRIC, "value": a["eff_samples_per_s"], "unit": "effective S_SF(K) samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": a["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (HS fields i.i.d. N(0,1) from per-chain seeds)",
         "config": {"workload": cfg["name"], "chains": nchains, "value_is": "plain batched sweep, uneven batch",
-                   "tau_int_sf_k": tau, "tau_window": window, "unstable_chains": int((~stable).sum()),
-                   "sweeps_per_s": nchains * args.steps / wall, "autotune_s": t_tune,
+                   "plain": a, "unstable_chains": int(np.sum(st[:, 4] >= UNSTABLE_DRIFT)),
+                   "sweeps_per_s": nchains * a["steps_per_s"], "autotune_s": t_tune,
                    "schedules": {f"{p.nlx}x{p.nly} b{p.beta:g} U2{p.u2:g}": list(sc)
                                  for p, sc in zip(params, s.schedule())},
                    "paths": {"lds_chains": int(sum(1 for c in range(nchains) if s.schedule()[chain_model[c]][0] == 1)),
                              "hbm_chains": int(sum(1 for c in range(nchains) if s.schedule()[chain_model[c]][0] == 0))}},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel_ms": avg_ms,
-                     "flops_per_launch": flops},
+        "roofline": roofline("k_sweep_lds + k_sweep (one graph)", flops, kms),
     }
     s.close()
     print(json.dumps(out), flush=True)
@@ -397,25 +500,26 @@ def run_uneven(args, cfg, nchains):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--steps", type=int, default=100, help="timed steps of each form")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed plain sweeps before the plain phase")
+    ap.add_argument("--stat-steps", type=int, default=400,
+                    help="steps of each form for the statistics (tau_int, error bars); the timed steps are "
+                         "the first --steps of them")
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", default="auto",
                     help="'auto' (autotuner, before the timed region), 'default' (library choice) or 'PATH,DELAYED'")
-    ap.add_argument("--moves", type=int, default=8, help="learned global moves per sweep (learned mode)")
-    ap.add_argument("--target", default="grad", choices=["sweep", "grad"],
-                    help="what the network learns: the one-sweep displacement (a Langevin move of the sweep's "
-                         "own scale) or d log W / d phi (a Metropolis-adjusted Langevin move, step h)")
-    ap.add_argument("--layers", type=int, default=3, help="convolution layers of the proposal network")
-    ap.add_argument("--hidden", type=int, default=8, help="hidden channels of the proposal network")
-    ap.add_argument("--dump", default="",
-                    help="learned mode: also write the per-chain S_SF(K) series of both phases to this .npz")
-    ap.add_argument("--mode", default="learned", choices=["plain", "learned"],
-                    help="plain: the batched local sweep; learned: plain sweeps, then the network trained on "
-                         "their displacements, then sweeps interleaved with learned global moves, compared "
-                         "at equal error bars")
+    ap.add_argument("--tune-nwrap", action="store_true",
+                    help="let the autotuner also choose Nwrap under its drift bound")
+    ap.add_argument("--hmc-steps", type=lambda v: [int(x) for x in v.split(",")], default=[1, 2],
+                    help="integrator step counts tried (each a quarter rotation in total)")
+    ap.add_argument("--layers", type=int, default=0, help="convolution layers (default: the workload's)")
+    ap.add_argument("--hidden", type=int, default=0, help="hidden channels (default: the workload's)")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--train-s", type=float, default=20.0, help="training time budget in seconds")
+    ap.add_argument("--dump", default="", help="learned mode: write both phases' per-chain S_SF(K) series")
+    ap.add_argument("--mode", default="learned", choices=["plain", "learned"])
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -432,11 +536,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4}))
         return
-
-    from bafqmc_amd.model import ModelSpec, conf_set_batch
-    from bafqmc_amd.solver import BatchedSolver
-    from bafqmc_amd.analysis import chain_seeds, tau_int
-
+    from bafqmc_amd.model import ModelSpec
     cfg = CONFIGS[args.config]
     nchains = args.chains or cfg["chains"]
     if "models" in cfg:
@@ -446,64 +546,8 @@ def main():
     spec = ModelSpec(p)
     if args.mode == "learned":
         run_learned(args, cfg, p, spec, nchains)
-        return
-    phis, seeds = conf_set_batch(p, chain_seeds(13579113, nchains))
-    schedule, tuned = choose_schedule(args, p, nchains)
-    s = BatchedSolver([spec], [0] * nchains, schedule=schedule)
-    s.set_fields(list(phis), seeds)
-    s.prepare()
-    for _ in range(args.warmup):
-        s.sweep(1)
-        s.read_bin()
-    sf = np.zeros((nchains, args.steps))
-    kernel_ms = []
-    phase = np.zeros(8)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        s.sweep(1)
-        kernel_ms.append(s.last_kernel_ms())
-        obs, st = s.read_bin_raw()
-        sf[:, k] = obs[:, 20] / obs[:, 32]
-        phase += st[:, 12:20].sum(axis=0)
-        if (k + 1) % 20 == 0 or args.steps < 20:
-            progress(f"step {k + 1}/{args.steps}")
-    wall = time.perf_counter() - t0
-    ms_per_step = 1000.0 * wall / args.steps
-    # the drift maximum S_XMAX is kept over the whole run
-    stable = (st[:, 4] < UNSTABLE_DRIFT) & np.all(np.isfinite(sf), axis=1)
-    n_stable = int(stable.sum())
-    tau, window, _ = pooled_tau(sf[stable]) if n_stable else (float("nan"), 0, None)
-    sweeps_per_s = nchains * args.steps / wall
-    value = n_stable * (args.steps / wall) / (2.0 * tau)
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "effective S_SF(K) samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (HS fields i.i.d. N(0,1) from per-chain seeds)",
-        "config": {"workload": cfg["name"], "chains": nchains, "matrix_n": p.n, "ltrot": p.ltrot,
-                   "nwrap": p.nwrap, "sweeps_per_s": sweeps_per_s, "tau_int_sf_k": tau,
-                   "unstable_chains": nchains - n_stable,
-                   "tau_window": window,
-                   "value_is": "plain batched sweep (no global moves)",
-                   "kernel": "k_sweep_lds" if s.schedule()[0][0] == 1 else "k_sweep",
-                   "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1],
-                                "autotune_ms_per_sweep": tuned.table if tuned else None},
-                   "phase_ms_per_chain_sweep": {k: 1e3 * v / (nchains * args.steps) for k, v in
-                                                zip(("site", "kinetic", "measure", "wrap", "hs", "wrap_products",
-                                                     "wrap_stab_u", "wrap_green"), phase)}},
-        "roofline": roofline(p, nchains, kernel_ms, args.config),
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, tau)
-    print(json.dumps(out), flush=True)
+    else:
+        run_plain(args, cfg, p, spec, nchains)
 
 
 if __name__ == "__main__":

@@ -98,6 +98,16 @@ int bafq_prepare(bafq_handle* h);
  * pair are captured once in a HIP graph and replayed). */
 int bafq_sweep(bafq_handle* h, int n_sweeps, int measure);
 
+/* n_passes measurement passes: one left sweep each with the fields frozen (no
+ * proposals; the RNG streams are untouched) that measures the equal-time
+ * estimators at every slice into the same accumulators as bafq_sweep, as
+ * sweep_L measures them (src/number_conserving/src/local_sweep.f90:96-111).
+ * No counterpart in the reference, whose only measurement is inside
+ * Local_sweep: this measures a configuration reached by a global move
+ * (bafq_global_propose / bafq_hmc_propose) without a local sweep.  G must be
+ * current (after bafq_prepare, a sweep or a move). */
+int bafq_measure_pass(bafq_handle* h, int n_passes);
+
 /* Per-chain observable sums, sample counts and counters accumulated since the
  * previous call, then reset (replaces Obs_equal%ave + Fourier%preq's per-rank
  * values, src/number_conserving/src/fourier_trans.f90:350-457; acceptance
@@ -133,6 +143,8 @@ int bafq_debug_qrp(int n, double* a, double* tau, int32_t* perm, double* d, int 
  * accumulator r of `lane` for A = one-hot at lane p, B = 1000 + lane
  * (64 x 64 x 4 doubles). */
 int bafq_debug_mfma_probe(double* out);
+/* The same for the FP32 MFMA 16x16x4 used by the network's convolutions. */
+int bafq_debug_mfma32_probe(float* out);
 /* Cross-lane operations of one 64-lane wavefront on in[64] / idx[64]:
  * out[0:64] sums over aligned 16-lane rows, out[64:128] sums over aligned
  * quads, out[128:192] lane 37's value in every lane, out[192:256] and
@@ -166,6 +178,10 @@ int bafq_net_get_params(bafq_net* h, float* params);
 int bafq_net_forward(bafq_net* h, int batch, const float* x, float* y);
 /* loss = |f(x) - target|^2 / size and its gradient (grad may be NULL) */
 int bafq_net_gradient(bafq_net* h, int batch, const float* x, const float* target, double* loss, float* grad);
+/* Device time in milliseconds of one forward pass over the first `batch`
+ * inputs already on the device (the last forward or training input), the
+ * average of `reps` passes between HIP events on the stream they run on. */
+int bafq_net_time_forward(bafq_net* h, int batch, int reps, double* ms);
 /* `steps` Adam steps (learning rate lr) on one batch; loss of the last step */
 int bafq_net_train(bafq_net* h, int batch, const float* x, const float* target, int steps, float lr,
                    double* loss);
@@ -185,6 +201,22 @@ int bafq_net_train(bafq_net* h, int batch, const float* x, const float* target, 
 int bafq_global_propose(bafq_handle* h, bafq_net* net, double lam, const double* sigma, uint64_t seed,
                         double* out);
 int bafq_global_finish(bafq_handle* h, const int32_t* accept);
+
+/* A Hamiltonian global move of every chain with a learned force (no
+ * counterpart in the reference).  With log W(phi) = -|phi - c|^2 / 2 + R(phi)
+ * (c = center[ch] per channel), momenta p ~ N(0, 1) are drawn on the channels
+ * whose field couples (counter-based generator keyed by seed) and `steps`
+ * steps of split Hamiltonian dynamics are run: p += (angle/2) f(phi), (phi - c, p)
+ * rotated by `angle`, p += (angle/2) f(phi'), with f = force_scale[ch] times
+ * the network's output on phi (net may be NULL: f = 0, an exact draw from the
+ * Gaussian part when steps * angle = pi/2).  G is rebuilt for phi' (as
+ * bafq_prepare).
+ * out[c][0..4]: (|p|^2 - |p'|^2) / 2, sum phi^2, sum phi'^2, sum phi_2 and
+ * sum phi'_2 -- the layout of bafq_global_propose, so the exact acceptance is
+ * the same formula; bafq_global_finish keeps or undoes the move.
+ * 0 < angle <= 3.2, 1 <= steps <= 64. */
+int bafq_hmc_propose(bafq_handle* h, bafq_net* net, double angle, int steps, const double* center,
+                     const double* force_scale, uint64_t seed, double* out);
 
 /* d log W / d phi for every field of every chain (laid out like the fields
  * of bafq_set_fields; W the weight of the global moves), from one left sweep

@@ -82,6 +82,17 @@ def read_confout(path: Path, nchains: int, nvals: int):
     return out_seeds, out_fields
 
 
+def _big_stack():
+    """The reference keeps N x N work matrices as automatic (stack) arrays: at
+    N = 576 (12x12 pairing) they exceed the usual 8 MiB stack, so the ranks run
+    with an unlimited stack, as `ulimit -s unlimited` would give them."""
+    import resource
+    try:
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    except (ValueError, OSError):
+        pass
+
+
 def run(p: dict, seeds, fields=None, workdir=None, nranks=None, timeout=600):
     """Run the reference solver; return (directory, wall seconds)."""
     nranks = nranks or len(seeds)
@@ -93,7 +104,7 @@ def run(p: dict, seeds, fields=None, workdir=None, nranks=None, timeout=600):
     cmd = [MPIRUN, "-np", str(nranks), str(executable(p.get("pairing", False)))]
     t0 = time.perf_counter()
     res = subprocess.run(cmd, cwd=d, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                         text=True, timeout=timeout)
+                         text=True, timeout=timeout, preexec_fn=_big_stack)
     wall = time.perf_counter() - t0
     if res.returncode:
         raise RuntimeError(f"reference run failed ({res.returncode}):\n{res.stdout[-4000:]}")

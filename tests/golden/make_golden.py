@@ -74,6 +74,21 @@ CASES = {
     # config-4 lattice: NC 9x9, beta=8, Ltrot=80 (one chain, one bin)
     "nc_traj_9x9_c4": dict(p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=80, beta=8.0,
                                   nwrap=10, nbin=1, shift_loc=1.5, ini_type=2, ini_ampl=1.0), seeds=[24681357]),
+    # thermalisation lattice smaller than the lattice: fields generated on 3x3 x 20
+    # slices and tiled onto 6x6 x 40 by conf_transfer (fields.f90:156-234)
+    "nc_transfer_6x6": dict(p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=6, nly=6, ltrot=40, beta=4.0,
+                                   nlx_therm=3, nly_therm=3, ltrot_therm=20, nwrap=10, nbin=2, shift_loc=1.5,
+                                   ini_type=2, ini_ampl=1.0), seeds=[424243, 8642]),
+    "pairing_transfer_6x6": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=40,
+                                        beta=4.0, nlx_therm=3, nly_therm=3, ltrot_therm=20, nwrap=10, nbin=2,
+                                        shift_loc=1.0, ini_type=2, ini_ampl=1.0), seeds=[97531]),
+    # 12x12 lattices on the device-memory path: NC (N = 144) and the config-5 pairing
+    # chain (reference N = 576, reduced N = 288), one chain, one bin
+    "nc_traj_12x12": dict(p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=12, nly=12, ltrot=80, beta=8.0,
+                                 nwrap=10, nbin=1, shift_loc=1.5, ini_type=2, ini_ampl=1.0), seeds=[13579113]),
+    "pairing_traj_12x12_c5": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12,
+                                         ltrot=100, beta=10.0, nwrap=10, nbin=1, shift_loc=1.0, ini_type=2,
+                                         ini_ampl=1.0), seeds=[24681357]),
 }
 
 

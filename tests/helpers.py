@@ -31,7 +31,9 @@ def solver_params(d: dict) -> Params:
                nlx=d["nlx"], nly=d["nly"], ltrot=d["ltrot"], beta=d["beta"], nwrap=d["nwrap"],
                shift_loc=d["shift_loc"], nbin=d.get("nbin", 1), ini_type=d.get("ini_type", 2),
                ini_ampl=d.get("ini_ampl", 1.0), is_warm=bool(d.get("nwarm", 0)), nwarm=d.get("nwarm", 0))
-    p.nlx_therm, p.nly_therm, p.ltrot_therm = p.nlx, p.nly, p.ltrot
+    p.nlx_therm = d.get("nlx_therm", p.nlx)
+    p.nly_therm = d.get("nly_therm", p.nly)
+    p.ltrot_therm = d.get("ltrot_therm", p.ltrot)
     return p.validate()
 
 

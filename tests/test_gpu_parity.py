@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import bafq_oracle as O
-from bafqmc_amd.model import ModelSpec, conf_set_batch
+from bafqmc_amd.model import ModelSpec, conf_set_batch, conf_transfer
 from bafqmc_amd.solver import BatchedSolver
 from tests.helpers import (fixed_fields, load_golden, golden_row, golden_info_value, oracle_params, solver_params,
                            reduced, rel_err, assert_obs_close, obs_atol)
@@ -206,7 +206,7 @@ def check_sweep_vs_oracle(lib, case):
 
 GOLDEN = ["nc_fixed_3x3_w1", "nc_fixed_3x3_w6", "pairing_fixed_3x3_w2", "nc_traj_3x3_c1",
           "nc_traj_3x3_u2", "pairing_traj_3x3", "nc_warm_3x3", "pairing_warm_3x3", "pairing_traj_6x6_c2",
-          "nc_traj_9x9_c4"]
+          "nc_traj_9x9_c4", "nc_transfer_6x6", "pairing_transfer_6x6", "nc_traj_12x12", "pairing_traj_12x12_c5"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -224,8 +224,9 @@ def check_golden(lib, name):
     if "fields_in" in z.files:
         phis, seeds_run = list(z["fields_in"]), seeds
     else:
-        phis, seeds_run = conf_set_batch(p, seeds)
-        phis = list(phis)
+        # conf_set on the thermalisation lattice, then conf_transfer (fields.f90:35-45)
+        phis_t, seeds_run = conf_set_batch(p, seeds)
+        phis = list(conf_transfer(p, phis_t))
     s = BatchedSolver([ModelSpec(p)], [0] * len(seeds), lib=hip_lib)
     s.set_fields(phis, [int(x) for x in seeds_run])
     if pd.get("nwarm", 0):
@@ -253,6 +254,16 @@ def check_golden(lib, name):
     for c in range(len(seeds)):
         np.testing.assert_allclose(f[c], z["fields_out"][c], rtol=0, atol=1e-12)
         assert sd[c] == int(z["seeds_out"][c])
+    # compare_mat drift (stabilization.f90:256-266) against the reference's
+    # "Max diff Matrix" (main.f90:59-73): rounding of two different codes, so an
+    # order of magnitude; below 1e-8 both are rounding noise of G
+    drift = float(np.max(s.stats()[:, 4]))
+    ref_drift = golden_info_value(z, "Max diff Matrix")
+    print(f"golden {name}: drift {drift:.3e} reference {ref_drift:.3e}", flush=True)
+    if ref_drift >= 1e-8:
+        assert ref_drift / 10 <= drift <= 10 * ref_drift, (name, drift, ref_drift)
+    else:
+        assert drift < 1e-8, (name, drift, ref_drift)
     if not pd["pairing"]:
         # log_weight = -1/2 sum phi^2 + 2 log|det G| (pole_diagnostics.f90:130-131)
         for c in range(len(seeds)):

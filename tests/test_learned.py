@@ -23,9 +23,18 @@ def _case(seed, batch, desc=DESC):
     return rng.normal(size=shape).astype(np.float32), rng.normal(size=shape).astype(np.float32)
 
 
-def _params(seed, desc=DESC):
-    n = sum(co * ci * 27 + co for co, ci in NO.layer_shapes(desc["channels"], desc["hidden"], desc["layers"]))
-    return (np.random.default_rng(seed).normal(size=n) * 0.3).astype(np.float32)
+def _params(seed, desc=DESC, scale=None):
+    """Random parameters; scale None: 0.3 everywhere, else `scale` times the
+    He-normal width sqrt(2 / fan_in) of each layer (activations of order one
+    at any width)."""
+    rng = np.random.default_rng(seed)
+    if scale is None:
+        n = sum(co * ci * 27 + co for co, ci in NO.layer_shapes(desc["channels"], desc["hidden"], desc["layers"]))
+        return (rng.normal(size=n) * 0.3).astype(np.float32)
+    out = []
+    for co, ci in NO.layer_shapes(desc["channels"], desc["hidden"], desc["layers"]):
+        out += [rng.normal(size=co * ci * 27) * scale * np.sqrt(2.0 / (ci * 27)), rng.normal(size=co) * 0.1]
+    return np.concatenate(out).astype(np.float32)
 
 
 def test_oracle_conv_is_periodic_correlation():
@@ -57,9 +66,9 @@ def test_oracle_gradient_central_differences():
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
 
 
-def _check_net(lib, desc=DESC, batch=3):
+def _check_net(lib, desc=DESC, batch=3, scale=None):
     x, t = _case(5, batch, desc)
-    p = _params(6, desc)
+    p = _params(6, desc, scale)
     c, h, L = desc["channels"], desc["hidden"], desc["layers"]
     net = ConvNet(desc["lt"], desc["ly"], desc["lx"], c, h, L, max_batch=4, lib=lib)
     net.set_params(p)
@@ -93,6 +102,13 @@ def test_net_emulator(emu_lib):
     _check_net(emu_lib)
 
 
+def test_net_emulator_matrix_core_blocking(emu_lib):
+    """The matrix-core convolution at the 9x9 network's width: two time blocks
+    (the second a tail of 3 slices), six position tiles per wavefront, two
+    16-channel output blocks, zero-padded input channels in the first layer."""
+    _check_net(emu_lib, dict(lt=12, ly=9, lx=9, channels=2, hidden=32, layers=4), batch=2, scale=1.0)
+
+
 def test_net_rejects_bad_batch(emu_lib):
     net = ConvNet(2, 2, 2, 2, 2, 2, max_batch=2, lib=emu_lib)
     with pytest.raises(AssertionError):
@@ -106,6 +122,10 @@ def test_net_gpu(hip_lib):
     _check_net(hip_lib, dict(lt=40, ly=3, lx=3, channels=2, hidden=8, layers=3), batch=4)
     # 4 layers x 32 channels: the forward kernel's widest register blocking
     _check_net(hip_lib, dict(lt=6, ly=3, lx=4, channels=2, hidden=32, layers=4), batch=3)
+    # the 9x9 network: several time blocks and a tail block per chain
+    _check_net(hip_lib, dict(lt=80, ly=9, lx=9, channels=2, hidden=32, layers=4), batch=4, scale=1.0)
+    # 12x12: one time block holds fewer slices
+    _check_net(hip_lib, dict(lt=20, ly=12, lx=12, channels=2, hidden=32, layers=4), batch=2, scale=1.0)
 
 
 def _device_move_case(lib, pairing):

@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 from oracle import bafq_oracle as O
-from tests.helpers import fixed_fields, load_golden, golden_row, oracle_params, assert_obs_close, obs_atol
+from tests.helpers import (fixed_fields, load_golden, golden_row, golden_info_value, oracle_params,
+                           assert_obs_close, obs_atol)
 
 ABS_TOL, REL_TOL = 2.0e-9, 2.0e-8
 
@@ -163,7 +164,8 @@ def test_pairing_ratio_bruteforce_and_block_structure():
 
 
 GOLDEN_CASES = ["nc_fixed_3x3_w1", "nc_fixed_3x3_w2", "nc_fixed_3x3_w6", "pairing_fixed_3x3_w2",
-                "nc_traj_3x3_c1", "nc_traj_3x3_u2", "pairing_traj_3x3", "nc_warm_3x3", "pairing_warm_3x3"]
+                "nc_traj_3x3_c1", "nc_traj_3x3_u2", "pairing_traj_3x3", "nc_warm_3x3", "pairing_warm_3x3",
+                "nc_transfer_6x6", "pairing_transfer_6x6"]
 
 
 def oracle_run_golden(name):
@@ -177,8 +179,12 @@ def oracle_run_golden(name):
         if "fields_in" in z:
             phi, seed = z["fields_in"][c], s
         else:
+            # conf_set on the thermalisation lattice, then conf_transfer (fields.f90:35-45)
             r = O.Ranf(s)
-            phi = O.conf_set(r, pd.get("ini_type", 2), pd.get("ini_ampl", 1.0), (0.0, 0.0), ns, p.ltrot)
+            lxt, lyt = pd.get("nlx_therm", p.nlx), pd.get("nly_therm", p.nly)
+            ltt = pd.get("ltrot_therm", p.ltrot)
+            phi_t = O.conf_set(r, pd.get("ini_type", 2), pd.get("ini_ampl", 1.0), (0.0, 0.0), lxt * lyt, ltt)
+            phi = O.conf_transfer(phi_t, lxt, lyt, p.nlx, p.nly, p.ltrot)
             seed = r.seed
         ch = O.Chain(p, phi, seed)
         for _ in range(pd.get("nwarm", 0)):
@@ -204,5 +210,38 @@ def test_oracle_matches_reference_fortran(name):
     for c, ch in enumerate(chains):
         assert ch.rng.seed == int(z["seeds_out"][c])
         np.testing.assert_allclose(ch.phi, z["fields_out"][c], rtol=0, atol=1e-13)
+    # compare_mat (stabilization.f90:256-266): the largest drift of the chains
+    # against the reference's "Max diff Matrix" (main.f90:59-73) -- rounding of
+    # two LAPACK builds, so an order of magnitude; below 1e-8 both are noise
+    drift = max(ch.prop.xmax for ch in chains)
+    ref_drift = golden_info_value(z, "Max diff Matrix")
+    if ref_drift >= 1e-8:
+        assert ref_drift / 10 <= drift <= 10 * ref_drift, (name, drift, ref_drift)
+    else:
+        assert drift < 1e-8, (name, drift, ref_drift)
     if not pd["pairing"] and len(chains) == 1:
         np.testing.assert_allclose(chains[0].log_weight(), z["out_log_weight"][-1][0], rtol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["nc_transfer_6x6", "pairing_transfer_6x6"])
+def test_conf_transfer_matches_reference_fields(name):
+    """conf_transfer pinned directly: the tiled initial fields reproduce the
+    reference's after its sweeps only if every tile lands where fields.f90:156-234
+    puts it; here the tiling itself is checked against its definition on the
+    golden's thermalisation lattice (x and y shifted by the thermalisation
+    extent, slices repeated with period LtrotTherm), and the fields the
+    oracle generates from the golden's seeds give the reference's final fields."""
+    z, pd = load_golden(name)
+    lxt, lyt, ltt = pd["nlx_therm"], pd["nly_therm"], pd["ltrot_therm"]
+    nlx, nly, lt = pd["nlx"], pd["nly"], pd["ltrot"]
+    rng = np.random.default_rng(0)
+    phi_t = rng.normal(size=(ltt, lxt * lyt, 2))
+    phi = O.conf_transfer(phi_t, lxt, lyt, nlx, nly, lt)
+    for nt in range(lt):
+        for y in range(nly):
+            for x in range(nlx):
+                np.testing.assert_array_equal(phi[nt, y * nlx + x], phi_t[nt % ltt, (y % lyt) * lxt + x % lxt])
+    # the product's host copy of the tiling agrees
+    from bafqmc_amd.model import conf_transfer
+    from tests.helpers import solver_params
+    np.testing.assert_array_equal(conf_transfer(solver_params(pd), phi_t[None])[0], phi)
