@@ -96,8 +96,12 @@ def tune_model(lib, params, chains=64, sweeps=2, tune_nwrap=False, drift_tol=1e-
 
 
 def _key(params, chains, device):
+    """Cache key: every parameter that changes the sweep's cost or its drift
+    (the Hamiltonian, the discretisation, the update width and the initial
+    fields), the sample size and the device."""
     p = params
-    return json.dumps([device, int(p.pairing), p.nlx, p.nly, p.ltrot, p.nwrap, chains])
+    return json.dumps([device, int(p.pairing), p.nlx, p.nly, p.ltrot, p.nwrap, p.beta, p.u1, p.u2, p.mu, p.rdelta,
+                       p.shift_loc, p.ini_type, p.ini_ampl, p.ini_bias1, p.ini_bias2, p.nwarm, chains])
 
 
 def tune(lib, models, chain_model, sweeps=2, tune_nwrap=False, drift_tol=1e-7, cache: Path | None = None,

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -82,6 +83,8 @@ int rt_gm_hmc_sums(const double* phi, const double* old, const double* p0, const
                    double* out);
 // words [c * per, (c + 1) * per) of dst <- src for chains c with accept[c] == 0
 int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n);
+// words [c * per, (c + 1) * per) of dst <- src for chains c with accept[c] != 0
+int rt_gm_copy_accepted(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n);
 
 constexpr double kZero = 1.0e-10;   // src/number_conserving/src/calc_basic.f90:5
 
@@ -114,6 +117,14 @@ struct bafq_handle {
     // learned global moves: old fields, forward drift, per-chain sums, flags, subset ids
     double *gm_old = nullptr, *gm_f0 = nullptr, *gm_out = nullptr, *gm_stats = nullptr;
     double* gm_p = nullptr;                   // Hamiltonian moves: the momenta
+    // Hamiltonian moves: network output at each chain's current fields and at
+    // the proposal ([chain][2][Ltrot Ly Lx] floats); valid for one network
+    // version while no sweep or other move has changed the fields
+    float *gm_fcur = nullptr, *gm_fnew = nullptr;
+    const void* gm_fnet = nullptr;
+    uint64_t gm_fver = 0;
+    int gm_fvalid = 0;
+    int gm_last = 0;                          // last move: 1 learned drift move, 2 Hamiltonian move
     uint64_t* gm_state = nullptr;             // copy of every chain's matrices before the move
     double* d_grad = nullptr;                 // d log W / d phi, laid out like the fields
     int* gm_accept = nullptr;
@@ -418,6 +429,7 @@ void bafq_destroy(bafq_handle* h) {
     if (!h) return;
     if (h->gm_old) bafq::rt_free(h->gm_old);
     if (h->gm_p) bafq::rt_free(h->gm_p);
+    if (h->gm_fcur) bafq::rt_free(h->gm_fcur);
     if (h->gm_state) bafq::rt_free(h->gm_state);
     if (h->d_grad) bafq::rt_free(h->d_grad);
     bafq::rt_release_launches(h->launches);
@@ -446,6 +458,7 @@ int bafq_dims(const bafq_handle* h, int chain, int32_t* n, int32_t* obs_stride, 
 int bafq_set_fields(bafq_handle* h, const double* phi, const int32_t* seeds) {
     using namespace bafq;
     if (!h) return 1;
+    h->gm_fvalid = 0;
     if (rt_h2d(h->d_phi, phi, sizeof(double) * h->phi_total)) return fail(h, rt_error());
     for (size_t c = 0; c < h->chains.size(); ++c)
         if (seeds[c] <= 0) return fail(h, "RNG seeds must be positive");
@@ -502,6 +515,7 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
 
 int bafq_warmup(bafq_handle* h, int n) {
     if (!h) return 1;
+    h->gm_fvalid = 0;
     if (n <= 0) return 0;
     return launch_all(h, bafq::K_WARMUP, n, 0);
 }
@@ -513,14 +527,16 @@ int bafq_prepare(bafq_handle* h) {
 
 int bafq_sweep(bafq_handle* h, int n_sweeps, int measure) {
     if (!h) return 1;
+    h->gm_fvalid = 0;
     if (n_sweeps <= 0) return 0;
     return launch_all(h, bafq::K_SWEEP, n_sweeps, measure);
 }
 
-int bafq_measure_pass(bafq_handle* h, int n_passes) {
+int bafq_measure_pass(bafq_handle* h, int n_passes, int mode) {
     if (!h) return 1;
+    if (mode < 0 || mode > 2) return bafq::fail(h, "measurement mode must be 0, 1 or 2");
     if (n_passes <= 0) return 0;
-    return launch_all(h, bafq::K_MEASURE, n_passes, 1);
+    return launch_all(h, bafq::K_MEASURE, mode == 2 ? 1 : n_passes, mode + 1);
 }
 
 int bafq_get_stats(bafq_handle* h, double* stats) {
@@ -595,9 +611,19 @@ struct bafq_net {
     std::vector<float*> act;             // act[0] input ... act[L] output
     float *g0 = nullptr, *g1 = nullptr, *target = nullptr;
     int step = 0;
+    uint64_t version = 0;                // unique over all networks, renewed whenever the parameters change
     void* timer = nullptr;
     std::string err;
 };
+
+namespace bafq {
+// Parameter versions, unique over every network of the process: a cached
+// network output is valid only for the version that produced it.
+inline uint64_t next_net_version() {
+    static std::atomic<uint64_t> v{0};
+    return ++v;
+}
+}  // namespace bafq
 
 static int net_fail(bafq_net* h, const std::string& msg) {
     if (h) h->err = msg;
@@ -659,6 +685,7 @@ int bafq_net_set_params(bafq_net* h, const float* params) {
     using namespace bafq;
     if (!h) return 1;
     h->step = 0;
+    h->version = bafq::next_net_version();
     if (rt_h2d(h->params, params, h->nparam * sizeof(float)) || rt_memset(h->m, 0, h->nparam * sizeof(float)) ||
         rt_memset(h->v, 0, h->nparam * sizeof(float)) || rt_sync())
         return net_fail(h, rt_error());
@@ -761,6 +788,7 @@ int bafq_net_train(bafq_net* h, int batch, const float* x, const float* target, 
     for (int k = 0; k < steps; ++k) {
         if (net_gradient_dev(h, batch, loss)) return 1;
         ++h->step;
+        h->version = next_net_version();
         if (rt_adam(h->params, h->grad, h->m, h->v, lr, h->step, h->nparam)) return net_fail(h, rt_error());
         // the input activation is overwritten by nothing: act[0] stays x
     }
@@ -811,6 +839,8 @@ int bafq_global_propose(bafq_handle* h, bafq_net* net, double lam, const double*
         return fail(h, "the network's lattice is not the model's");
     const size_t per = (size_t)gm_per(s), total = per * nc;
     if (gm_snapshot(h, total, nc)) return 1;
+    h->gm_fvalid = 0;
+    h->gm_last = 1;
     GmArgs a{};
     a.s = s;
     a.lam = lam;
@@ -867,15 +897,30 @@ int bafq_hmc_propose(bafq_handle* h, bafq_net* net, double angle, int steps, con
         a.center[ch] = center[ch];
         a.fscale[ch] = force_scale[ch];
     }
+    // network output at the current fields: from the previous move when it is
+    // still valid (same network version, fields unchanged since), else computed
+    const size_t fper = 2 * (size_t)s.vol();
+    if (net && !h->gm_fcur) {
+        if (rt_malloc((void**)&h->gm_fcur, 2 * sizeof(float) * fper * nc)) return fail(h, rt_error());
+        h->gm_fnew = h->gm_fcur + fper * nc;
+        h->gm_fvalid = 0;
+    }
+    const bool cached = net && h->gm_fvalid && h->gm_fver == net->version;
     const int chunk = net ? net->max_batch : nc;
     for (int c0 = 0; c0 < nc; c0 += chunk) {
         a.c0 = c0;
         a.batch = std::min(chunk, nc - c0);
         if (rt_gm_hmc_init(h->d_phi, h->gm_old, h->gm_f0, h->gm_p, a)) return fail(h, rt_error());
-        const float* y = net ? net->act[net->d.layers] : nullptr;
+        const float* yout = net ? net->act[net->d.layers] : nullptr;
+        const float* y = nullptr;
         if (net) {
-            if (rt_gm_to_net(h->d_phi, net->act[0], a)) return fail(h, rt_error());
-            if (net_forward_dev(net, a.batch)) return fail(h, net->err);
+            float* cur = h->gm_fcur + fper * c0;
+            if (!cached) {
+                if (rt_gm_to_net(h->d_phi, net->act[0], a)) return fail(h, rt_error());
+                if (net_forward_dev(net, a.batch)) return fail(h, net->err);
+                if (rt_d2d(cur, yout, sizeof(float) * fper * a.batch)) return fail(h, rt_error());
+            }
+            y = cur;
         }
         for (int k = 0; k < steps; ++k) {
             if (net && rt_gm_hmc_kick(h->gm_p, y, a)) return fail(h, rt_error());
@@ -883,11 +928,17 @@ int bafq_hmc_propose(bafq_handle* h, bafq_net* net, double angle, int steps, con
             if (net) {
                 if (rt_gm_to_net(h->d_phi, net->act[0], a)) return fail(h, rt_error());
                 if (net_forward_dev(net, a.batch)) return fail(h, net->err);
+                y = yout;
                 if (rt_gm_hmc_kick(h->gm_p, y, a)) return fail(h, rt_error());
             }
         }
+        if (net && rt_d2d(h->gm_fnew + fper * c0, yout, sizeof(float) * fper * a.batch)) return fail(h, rt_error());
         if (rt_gm_hmc_sums(h->d_phi, h->gm_old, h->gm_f0, h->gm_p, a, h->gm_out)) return fail(h, rt_error());
     }
+    h->gm_last = 2;
+    h->gm_fnet = net;
+    h->gm_fver = net ? net->version : 0;
+    h->gm_fvalid = 0;                 // until bafq_global_finish settles which output is current
     if (rt_sync() || rt_d2h(out, h->gm_out, sizeof(double) * 5 * (size_t)nc)) return fail(h, rt_error());
     return launch_all(h, K_PREPARE, 0, 0);
 }
@@ -907,6 +958,18 @@ int bafq_global_finish(bafq_handle* h, const int32_t* accept) {
         rt_gm_restore_words((uint64_t*)h->d_stats, (const uint64_t*)h->gm_stats, h->gm_accept, S_LEN,
                             (size_t)S_LEN * nc))
         return fail(h, rt_error());
+    // Hamiltonian move with a network: the output at the proposal becomes the
+    // current one where the move was accepted (2 Ltrot Ly Lx floats = Ltrot Ly Lx words)
+    if (h->gm_last == 2 && h->gm_fnet) {
+        const size_t fwords = (size_t)m.ltrot * m.nly * m.nlx;
+        if (rt_gm_copy_accepted((uint64_t*)h->gm_fcur, (const uint64_t*)h->gm_fnew, h->gm_accept, fwords,
+                                fwords * nc))
+            return fail(h, rt_error());
+        h->gm_fvalid = 1;
+    } else {
+        h->gm_fvalid = 0;
+    }
+    h->gm_last = 0;
     return rt_sync() ? fail(h, rt_error()) : 0;
 }
 

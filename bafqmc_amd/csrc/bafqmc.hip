@@ -273,6 +273,12 @@ __global__ void __launch_bounds__(256) k_gm_hmc_sums(const double* phi, const do
     if (tid < 5) out[(size_t)(a.c0 + b) * 5 + tid] = part[tid][0];
 }
 
+__global__ void __launch_bounds__(256) k_gm_copy_accepted(uint64_t* dst, const uint64_t* src, const int* accept,
+                                                          size_t per, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        if (accept[e / per]) dst[e] = src[e];
+}
+
 static int grid_for(size_t n) { return (int)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535); }
 
 int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n) {
@@ -297,6 +303,11 @@ int rt_gm_hmc_sums(const double* phi, const double* old, const double* p0, const
                    double* out) {
     hipLaunchKernelGGL(k_gm_hmc_sums, dim3(a.batch), dim3(256), 0, 0, phi, old, p0, p, a, out);
     return check(hipGetLastError(), "k_gm_hmc_sums");
+}
+
+int rt_gm_copy_accepted(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n) {
+    hipLaunchKernelGGL(k_gm_copy_accepted, dim3(grid_for(n)), dim3(256), 0, 0, dst, src, accept, per, n);
+    return check(hipGetLastError(), "k_gm_copy_accepted");
 }
 
 int rt_gm_to_net(const double* phi, float* x, const GmArgs& a) {

@@ -337,6 +337,11 @@ int rt_gm_restore(double* phi, const double* old, const int* accept, int per, si
     for (size_t e = 0; e < n; ++e) gm_restore_elem(phi, old, accept, per, (int)e);
     return 0;
 }
+int rt_gm_copy_accepted(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n) {
+    for (size_t e = 0; e < n; ++e)
+        if (accept[e / per]) dst[e] = src[e];
+    return 0;
+}
 int rt_gm_restore_words(uint64_t* dst, const uint64_t* src, const int* accept, size_t per, size_t n) {
     for (size_t e = 0; e < n; ++e)
         if (!accept[e / per]) dst[e] = src[e];

@@ -1485,11 +1485,67 @@ DEV void body_grad(const Ctx& c, const LaunchArgs& a, int* slot) {
     }
 }
 
+// Measurement at the stabilisation points only: the left pass with frozen
+// fields keeps only what sweep_L needs at its checkpoints -- the product
+// UL <- UL B(nt + Nwrap, nt] and Wrap_L's stab_UL and stab_green
+// (stabilization.f90:292-328) -- and measures the Green's function stab_green
+// returns there, G(nt) for nt = Ltrot (the current G), Ltrot - Nwrap, ...,
+// Nwrap, the slices at which sweep_L measures a freshly stabilised G
+// (local_sweep.f90:101-103).  The G wraps between checkpoints are not done.
+// Ends, like sweep_L, with G(0), the left checkpoint lists and log|det G|.
+template <bool kLds>
+DEV void measure_checkpoints(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
+    const int n = m.n;
+    cd* G = kLds ? L.mat : ch.g;
+    if (kLds) measure_lds(c, m, ch, L, G);
+    else measure(c, m, ch, L, G);
+    live_identity(c, n, ch.ul, ch.dl, ch.vl);
+    for (int st = m.nst - 1; st >= 0; --st) {
+        const int nt = st * m.nwrap;
+        double ld;
+        if (kLds) {
+            lds_from_global(c, L.mat, ch.ul, n);
+            apply_slices_left(c, m, ch, L, L.mat, nt + 1, nt + m.nwrap);
+            global_from_lds(c, ch.ul, L.mat, n);
+            load_ud(c, L.mat, ch.ul, ch.dl, n, true);
+            stab_ul_from(c, m, ch, L, L.mat);
+            ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
+                            list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, L.mat, ch.wa);
+            lds_from_global(c, L.mat, ch.wa, n);
+        } else {
+            for (int t = nt + m.nwrap; t > nt; --t) {
+                const double* phi_t = ch.phi + (size_t)(t - 1) * m.ns * 2;
+                gemm_rc(c, ch.w1, m.ez[2], ch.ul, 1, n);          // UL <- UL e^{-dtau K}
+                mat_copy(c, ch.ul, ch.w1, n, false);
+                for (int nf = 2; nf >= 1; --nf) {
+                    if (!m.active[nf - 1]) continue;
+                    hs_factors(c, m, L, phi_t, nf);
+                    scale_mat(c, ch.ul, n, nullptr, L.f);         // UL <- UL e^{V}
+                }
+            }
+            load_ud(c, ch.wa, ch.ul, ch.dl, n, true);
+            stab_ul_from(c, m, ch, L, ch.wa);
+            ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
+                            list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, ch.wa, ch.g);
+        }
+        store_left(c, m, ch, st);
+        if (c.tid() == 0) ch.stats[S_LOGDET] = ld;
+        c.sync();
+        if (st > 0) {
+            if (kLds) measure_lds(c, m, ch, L, G);
+            else measure(c, m, ch, L, G);
+        }
+    }
+}
+
 // Measurement pass: one left sweep with the fields frozen (no proposals, the
 // RNG stream untouched) measuring the equal-time estimators at every slice,
 // as sweep_L of Local_sweep measures them (local_sweep.f90:96-115) -- the
-// measurement of a configuration reached by a global move.  G, the UDV factors
-// and the checkpoint lists end in the state sweep_L leaves them in.
+// measurement of a configuration reached by a global move; a.measure == 2
+// measures at the stabilisation points only (measure_checkpoints), and
+// a.measure == 3 measures the current Green's function once (no pass: the
+// state is left as it is).  Otherwise G, the UDV factors and the checkpoint
+// lists end in the state sweep_L leaves them in.
 template <bool kLds>
 DEV void body_measure(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
@@ -1497,14 +1553,27 @@ DEV void body_measure(const Ctx& c, const LaunchArgs& a, int* slot) {
         const ModelDev& m = a.models[ch.model];
         const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
         int seed = *ch.seed;
-        for (int s = 0; s < a.nsweep; ++s) {
+        if (a.measure == 3) {
+            // the Green's function as it is (G(0) = G(beta) after a rebuild or a sweep)
             if (kLds) {
                 lds_from_global(c, L.mat, ch.g, m.n);
+                measure_lds(c, m, ch, L, L.mat);
+            } else {
+                measure(c, m, ch, L, ch.g);
+            }
+            c.sync();
+            continue;
+        }
+        for (int s = 0; s < a.nsweep; ++s) {
+            if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
+            if (a.measure == 2) {
+                measure_checkpoints<kLds>(c, m, ch, L);
+            } else if (kLds) {
                 sweep_left_lds(c, m, ch, L, seed, true, nullptr, true);
-                global_from_lds(c, ch.g, L.mat, m.n);
             } else {
                 sweep_left_global(c, m, ch, L, seed, true, nullptr, true);
             }
+            if (kLds) global_from_lds(c, ch.g, L.mat, m.n);
         }
         c.sync();
     }

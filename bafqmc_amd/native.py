@@ -70,7 +70,7 @@ SYMBOLS = {
     "bafq_global_finish": (C.c_int, [C.c_void_p, c_int32_p]),
     "bafq_hmc_propose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, c_double_p, c_double_p, C.c_uint64,
                                    c_double_p]),
-    "bafq_measure_pass": (C.c_int, [C.c_void_p, C.c_int]),
+    "bafq_measure_pass": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "bafq_grad_log_weight": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_get_green": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "bafq_last_kernel_ms": (C.c_int, [C.c_void_p, c_double_p]),

@@ -159,9 +159,13 @@ class BatchedSolver:
     def sweep(self, n=1, measure=True):
         self._check(self.lib.bafq_sweep(self.h, int(n), 1 if measure else 0), "bafq_sweep")
 
-    def measure_pass(self, n=1):
-        """n left passes with frozen fields, measuring every slice (bafq_measure_pass)."""
-        self._check(self.lib.bafq_measure_pass(self.h, int(n)), "bafq_measure_pass")
+    MEASURE_MODES = {"all": 0, "checkpoints": 1, "current": 2}
+
+    def measure_pass(self, n=1, mode="all"):
+        """Measure with frozen fields (bafq_measure_pass): n left passes measuring
+        every slice ("all") or only the stabilisation points ("checkpoints"),
+        or the current Green's function once ("current")."""
+        self._check(self.lib.bafq_measure_pass(self.h, int(n), self.MEASURE_MODES[mode]), "bafq_measure_pass")
 
     def last_kernel_ms(self):
         v = C.c_double()

@@ -15,9 +15,13 @@ layers, 32 hidden channels, FP32, He-normal initialisation from a seed.
 * learned step: one Hamiltonian global move of every chain (momenta drawn on
   the device, the Gaussian part of log W integrated exactly, the determinant
   part by the network's learned force, exact Metropolis-Hastings with the
-  fresh FP64 determinant; bafq_hmc_propose / bafq_global_finish), then one
-  measurement pass (a left sweep with frozen fields measuring every slice,
-  bafq_measure_pass), then the bin is read.
+  fresh FP64 determinant; bafq_hmc_propose / bafq_global_finish), then the
+  estimators on the move's rebuilt Green's function G(beta) (`--measure
+  current`; `all` runs a left pass with frozen fields measuring every slice,
+  `checkpoints` measures at the stabilisation points; bafq_measure_pass), then
+  the bin is read.  Every slice's measurement estimates the same equal-time
+  averages; the comparison at equal error bars charges each form its own
+  variance per step.
 
 The network is trained before the learned phase on the residual force
 d log W / d phi + phi of fields from the plain sampler (the device's gradient
@@ -104,6 +108,13 @@ def flops_per_measure_pass(n, ns, ltrot, nwrap, c_meas=40.0):
     """One left pass with frozen fields: the kinetic GEMMs and stabilisations of
     sweep_L, no updates, one measurement per slice."""
     return ltrot * 24 * n ** 3 + (ltrot / nwrap) * 56 * n ** 3 + ltrot * c_meas * ns ** 2
+
+
+def flops_per_checkpoint_pass(n, ns, ltrot, nwrap, c_meas=40.0):
+    """Measurement at the stabilisation points only: one B product per slice on
+    the U factor (8 N^3) and one Wrap_L stabilisation (56 N^3) per checkpoint,
+    one measurement per checkpoint."""
+    return ltrot * 8 * n ** 3 + (ltrot / nwrap) * (56 * n ** 3 + c_meas * ns ** 2)
 
 
 def flops_per_prepare(n, ltrot, nwrap):
@@ -299,7 +310,7 @@ def run_learned(args, cfg, p, spec, nchains):
         acc, tm = 0.0, time.perf_counter()
         for _ in range(3):
             acc += float(GM.hmc_global_device(s, net, 0.5 * math.pi / steps, steps, rng, force_scale=fscale).mean())
-            s.measure_pass(1)
+            s.measure_pass(1, mode=args.measure)
         trials[steps] = (acc / 3, (time.perf_counter() - tm) / 3)
     s.read_bin()
     hmc_steps = max(trials, key=lambda k: trials[k][0] / trials[k][1])
@@ -314,7 +325,7 @@ def run_learned(args, cfg, p, spec, nchains):
         a = GM.hmc_global_device(s, net, angle, hmc_steps, rng, force_scale=fscale)
         phase_ms["move_prepare"].append(s.last_kernel_ms())
         accepted.append(float(a.mean()))
-        s.measure_pass(1)
+        s.measure_pass(1, mode=args.measure)
         ms = s.last_kernel_ms()
         phase_ms["measure"].append(ms)
         return ms
@@ -331,7 +342,9 @@ def run_learned(args, cfg, p, spec, nchains):
     kernels = {
         "k_sweep_lds": roofline("k_sweep_lds", flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch) * nchains,
                                 sweep_ms),
-        "k_measure_lds": roofline("k_measure_lds", flops_per_measure_pass(p.n, p.ns, p.ltrot, p.nwrap) * nchains,
+        "k_measure_lds": roofline("k_measure_lds", {"all": flops_per_measure_pass(p.n, p.ns, p.ltrot, p.nwrap),
+                                                 "checkpoints": flops_per_checkpoint_pass(p.n, p.ns, p.ltrot, p.nwrap),
+                                                 "current": 40.0 * p.ns ** 2}[args.measure] * nchains,
                                   phase_ms["measure"]),
         "k_prepare_lds": roofline("k_prepare_lds", flops_per_prepare(p.n, p.ltrot, p.nwrap) * nchains,
                                   phase_ms["move_prepare"]),
@@ -362,8 +375,8 @@ def run_learned(args, cfg, p, spec, nchains):
         "config": {"workload": cfg["name"], "chains": nchains, "counted_chains": nchains,
                    "unstable_chains": unstable,
                    "value_is": (f"learned form: one Hamiltonian global move ({hmc_steps} step(s), exact MH with the "
-                                "FP64 determinant) + one measurement pass per step; value = sigma^2_plain / "
-                                "(SE_learned^2 T_learned)"),
+                                f"FP64 determinant) + one measurement pass ({args.measure}) per step; value = "
+                                "sigma^2_plain / (SE_learned^2 T_learned)"),
                    "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1], "nwrap": p.nwrap,
                                 "autotune_ms_per_sweep": tuned.table if tuned else None},
                    "plain": a0,
@@ -514,6 +527,9 @@ def main():
                     help="let the autotuner also choose Nwrap under its drift bound")
     ap.add_argument("--hmc-steps", type=lambda v: [int(x) for x in v.split(",")], default=[1, 2],
                     help="integrator step counts tried (each a quarter rotation in total)")
+    ap.add_argument("--measure", default="current", choices=["all", "checkpoints", "current"],
+                    help="learned step's measurement: every slice, the stabilisation points only, or the "
+                         "rebuilt G(beta) of the move once")
     ap.add_argument("--layers", type=int, default=0, help="convolution layers (default: the workload's)")
     ap.add_argument("--hidden", type=int, default=0, help="hidden channels (default: the workload's)")
     ap.add_argument("--epochs", type=int, default=20)

@@ -102,11 +102,17 @@ int bafq_sweep(bafq_handle* h, int n_sweeps, int measure);
  * proposals; the RNG streams are untouched) that measures the equal-time
  * estimators at every slice into the same accumulators as bafq_sweep, as
  * sweep_L measures them (src/number_conserving/src/local_sweep.f90:96-111).
- * No counterpart in the reference, whose only measurement is inside
- * Local_sweep: this measures a configuration reached by a global move
+ * mode 0: every slice; mode 1: only at the Nst stabilisation points nt =
+ * Ltrot, Ltrot - Nwrap, ..., Nwrap, where Wrap_L computes G afresh, without
+ * the G wraps between them; mode 2: the current Green's function once
+ * (G(beta) after bafq_prepare or a move; no pass, n_passes ignored, nothing
+ * but the accumulators changes).  Every slice's measurement estimates the same
+ * equal-time averages (the trace is cyclic in imaginary time).  No
+ * counterpart in the reference, whose only measurement is inside Local_sweep:
+ * this measures a configuration reached by a global move
  * (bafq_global_propose / bafq_hmc_propose) without a local sweep.  G must be
  * current (after bafq_prepare, a sweep or a move). */
-int bafq_measure_pass(bafq_handle* h, int n_passes);
+int bafq_measure_pass(bafq_handle* h, int n_passes, int mode);
 
 /* Per-chain observable sums, sample counts and counters accumulated since the
  * previous call, then reset (replaces Obs_equal%ave + Fourier%preq's per-rank

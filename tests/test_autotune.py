@@ -85,3 +85,18 @@ def test_tuner_picks_the_fastest_gpu(hip_lib):
     print("autotune", ch, flush=True)
     times = [v for v in ch.table.values() if v is not None]
     assert ch.ms_per_sweep == min(times)
+
+
+def test_cache_key_separates_hamiltonians():
+    """Two models that differ only in U2 (or beta, mu, Delta) get their own
+    tuning entries: a schedule or Nwrap validated for one is never reused for
+    the other."""
+    import dataclasses
+    from bafqmc_amd.autotune import _key
+    from tests.helpers import solver_params
+    base = solver_params(dict(pairing=True, u1=1.0, u2=0.0, mu=-5.0, rdelta=0.5, nlx=3, nly=3, ltrot=40,
+                              beta=4.0, nwrap=10, shift_loc=1.0))
+    keys = {_key(base, 16, "gfx950")}
+    for change in (dict(u2=-0.5), dict(beta=4.5), dict(mu=-4.0), dict(rdelta=0.3), dict(u1=0.5)):
+        keys.add(_key(dataclasses.replace(base, **change), 16, "gfx950"))
+    assert len(keys) == 6

@@ -83,6 +83,97 @@ def _measure_pass_case(lib, name, schedule=None):
     s.close()
 
 
+def _checkpoint_pass_case(lib, name, schedule=None):
+    """Measurement at the stabilisation points only: the average of the
+    oracle's estimators on the dense Green's function (1 - B(tau) ... B(tau+1))^-1
+    at tau = Nwrap, 2 Nwrap, ..., Ltrot (no stabilisation, explicit products)."""
+    d = MEASURE_CASES[name]
+    p = solver_params(d)
+    phi = fixed_fields(d["ltrot"], 9)
+    ch = O.Chain(oracle_params(d), phi, 777)
+    samples = []
+    for tau in range(d["nwrap"], d["ltrot"] + 1, d["nwrap"]):
+        ch.prop.g = O.dense_green(ch, tau % d["ltrot"])[0]
+        samples.append(O.measure_pairing(ch) if d["pairing"] else O.measure_nc(ch))
+    ref = {k: sum(smp[k] for smp in samples) / len(samples) for k in samples[0]}
+    s = BatchedSolver([ModelSpec(p)], [0], lib=lib, schedule=schedule)
+    s.set_fields([phi], [777])
+    s.prepare()
+    s.measure_pass(1, mode="checkpoints")
+    got = s.read_bin()[0]
+    assert got["_nobs"] == d["ltrot"] // d["nwrap"]
+    for k in ("density_total", "energy_density", "sf_K", "dw_K", "kinetic", "doubleOcc", "pair_equal"):
+        if k not in ref:
+            continue
+        a, b = complex(got[k]), complex(ref[k])
+        assert abs(a - b) <= 1e-9 * abs(b) + 1e-11, (k, a, b)
+    # the pass ends where sweep_L ends: a full measurement pass from here agrees
+    # with one from a fresh rebuild
+    s.measure_pass(1)
+    after = s.read_bin()[0]
+    s2 = BatchedSolver([ModelSpec(p)], [0], lib=lib, schedule=schedule)
+    s2.set_fields([phi], [777])
+    s2.prepare()
+    s2.measure_pass(1)
+    fresh = s2.read_bin()[0]
+    for k in ("density_total", "sf_K", "dw_K"):
+        a, b = complex(after[k]), complex(fresh[k])
+        assert abs(a - b) <= 1e-9 * abs(b) + 1e-11, (k, a, b)
+    s.close()
+    s2.close()
+
+
+def _current_measurement_case(lib, name, schedule=None):
+    """mode "current": the oracle's estimators on G(beta) of the stabilised
+    rebuild, once; fields, G and the RNG stream unchanged."""
+    d = MEASURE_CASES[name]
+    p = solver_params(d)
+    phi = fixed_fields(d["ltrot"], 9)
+    ch = O.Chain(oracle_params(d), phi, 777)
+    ch.pre()
+    ref = O.measure_pairing(ch) if d["pairing"] else O.measure_nc(ch)
+    s = BatchedSolver([ModelSpec(p)], [0], lib=lib, schedule=schedule)
+    s.set_fields([phi], [777])
+    s.prepare()
+    g0 = s.green(0)
+    s.measure_pass(5, mode="current")
+    got = s.read_bin()[0]
+    assert got["_nobs"] == 1
+    for k in ("density_total", "energy_density", "sf_K", "dw_K", "kinetic", "pair_equal"):
+        if k not in ref:
+            continue
+        a, b = complex(got[k]), complex(ref[k])
+        assert abs(a - b) <= 1e-9 * abs(b) + 1e-11, (k, a, b)
+    np.testing.assert_array_equal(s.green(0), g0)
+    s.close()
+
+
+@pytest.mark.parametrize("name", sorted(MEASURE_CASES))
+def test_current_measurement_emulator(emu_lib, name):
+    _current_measurement_case(emu_lib, name)
+    _current_measurement_case(emu_lib, name, (0, 0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(MEASURE_CASES))
+def test_current_measurement_gpu(hip_lib, name):
+    _current_measurement_case(hip_lib, name)
+    _current_measurement_case(hip_lib, name, (0, 0))
+
+
+@pytest.mark.parametrize("name", sorted(MEASURE_CASES))
+def test_checkpoint_pass_emulator(emu_lib, name):
+    _checkpoint_pass_case(emu_lib, name)
+    _checkpoint_pass_case(emu_lib, name, (0, 0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(MEASURE_CASES))
+@pytest.mark.parametrize("schedule", [None, (0, 0)])
+def test_checkpoint_pass_gpu(hip_lib, name, schedule):
+    _checkpoint_pass_case(hip_lib, name, schedule)
+
+
 @pytest.mark.parametrize("name", sorted(MEASURE_CASES))
 def test_measure_pass_emulator(emu_lib, name):
     _measure_pass_case(emu_lib, name)
@@ -157,6 +248,24 @@ def _hmc_case(lib, pairing, steps, use_net=True, nl=3, lt=8, hidden=3, layers=3,
     for c in range(nchains):
         np.testing.assert_array_equal(fin[c], P[c] if accept[c] else F[c])
     assert list(sd) == seeds
+    if steps == 1:
+        # a second move starts from the network output the first one left
+        # current (the proposal's where accepted, the old fields' elsewhere)
+        F2 = np.stack(fin)
+        out2 = np.zeros((nchains, 5))
+        s._check(lib.bafq_hmc_propose(s.h, net.h if net else None, angle, steps,
+                                      np.ascontiguousarray(cen).ctypes.data_as(C.POINTER(C.c_double)),
+                                      fsc.ctypes.data_as(C.POINTER(C.c_double)), C.c_uint64(7),
+                                      out2.ctypes.data_as(C.POINTER(C.c_double))), "hmc 2")
+        P2 = np.stack(s.get_fields()[0])
+        h = angle
+        ph = (P2 - cen - (F2 - cen) * math.cos(h)) / math.sin(h)
+        p0 = ph - 0.5 * h * force(F2) * mask
+        p1 = ph * math.cos(h) - (F2 - cen) * math.sin(h) + 0.5 * h * force(P2) * mask
+        kin = 0.5 * np.sum(p0 ** 2 - p1 ** 2, axis=(1, 2, 3))
+        np.testing.assert_allclose(out2[:, 0], kin, rtol=1e-8, atol=1e-8 * np.abs(kin).max())
+        flags = np.ones(nchains, dtype=np.int32)
+        s._check(lib.bafq_global_finish(s.h, flags.ctypes.data_as(C.POINTER(C.c_int32))), "finish 2")
     # the exact acceptance helper runs end to end
     acc = GM.hmc_global_device(s, net, angle, steps, np.random.default_rng(3))
     assert acc.dtype == bool and acc.shape == (nchains,)
