@@ -24,9 +24,17 @@ def main(argv=None):
     r.add_argument("--chains", type=int, default=int(os.environ.get("BAFQMC_CHAINS", "1")),
                    help="independent Markov chains (the reference's MPI ranks)")
     r.add_argument("--no-poles", action="store_true", help="skip the per-bin pole diagnostics")
+    r.add_argument("--autotune", action="store_true",
+                   help="time the execution schedules on a sample of the chains first and run the fastest "
+                        "(the same Markov chain)")
+    r.add_argument("--tune-nwrap", action="store_true",
+                   help="with --autotune, also choose N_Ortho among the divisors of Ltrot near the input's, "
+                        "admitting only intervals whose measured drift stays below 1e-7 (results change at "
+                        "that level)")
     a = ap.parse_args(argv)
     from .driver import run
-    run(a.directory, a.solver, a.chains, poles=False if a.no_poles else None)
+    run(a.directory, a.solver, a.chains, poles=False if a.no_poles else None, autotune=a.autotune,
+        tune_nwrap=a.tune_nwrap)
     return 0
 
 

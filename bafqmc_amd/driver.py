@@ -18,6 +18,8 @@ The time-displaced path (``is_tau``) writes no estimators in the reference
 """
 from __future__ import annotations
 
+import dataclasses
+
 import math
 import sys
 import time
@@ -82,27 +84,35 @@ def write_poles(run_dir: Path, zs: list, scalars: list):
 
 
 def info_header(p: Params, nchains: int, spectrum) -> str:
-    """write_info (calc_basic.f90:138-174) plus the single-particle spectrum (model.f90:27-29)."""
+    """write_info plus the single-particle spectrum: number conserving
+    src/number_conserving/src/calc_basic.f90:138-174 and model.f90:27-29, pairing
+    (its own labels, the Delta row) src/pairing/src/calc_basic.f90:140-174 and
+    src/pairing/src/model.f90:31-33."""
+    if p.pairing:
+        title = " DQMC for pairing Bose Hubbard on triangular lattice"
+        u_rows = [f" Relative-density U1 (>=0)                    : {p.u1}",
+                  f" Total-density U2 (<=0)                       : {p.u2}",
+                  f" pairing Delta                                  : {p.rdelta}"]
+        warm_rows = [f" Thermalize auxiliary field magnitude for U1    : {p.shift_warm1}",
+                     f" Thermalize auxiliary field magnitude for U2    : {p.shift_warm2}"]
+    else:
+        title = " DQMC for boson Hubbard on triangular lattice"
+        u_rows = [f" Hubbard U1 (relative-density, repulsive)       : {p.u1}",
+                  f" Hubbard U2 (total-density, attractive)         : {p.u2}"]
+        warm_rows = [f" Thermalize U1 relative-field magnitude          : {p.shift_warm1}",
+                     f" Thermalize U2 total-field magnitude             : {p.shift_warm2}"]
     rows = [
         " =========================",
-        " DQMC for pairing Bose Hubbard on triangular lattice" if p.pairing
-        else " DQMC for boson Hubbard on triangular lattice",
+        title,
         f" Linear lengh Lx                                : {p.nlx}",
         f" Linear lengh Ly                                : {p.nly}",
         f" Hopping t                                      : {1.0}",
-        f" Hubbard U1 (relative-density, repulsive)       : {p.u1}",
-        f" Hubbard U2 (total-density, attractive)         : {p.u2}",
-    ]
-    if p.pairing:
-        rows.append(f" pairing Delta                                  : {p.rdelta}")
-    rows += [
+        *u_rows,
         f" chemical potential                             : {p.mu}",
         f" Local update auxiliary field magnitude Shift   : {p.shift_loc}",
     ]
     if p.is_warm:
-        rows += [f" # Warm                                         : {p.nwarm}",
-                 f" Thermalize U1 relative-field magnitude          : {p.shift_warm1}",
-                 f" Thermalize U2 total-field magnitude             : {p.shift_warm2}"]
+        rows += [f" # Warm                                         : {p.nwarm}", *warm_rows]
     rows += [
         f" Choosing initial distribution type             : {p.ini_type}",
         f" Magnitude of Gaussian distribution             : {p.ini_ampl}",
@@ -121,8 +131,11 @@ def info_header(p: Params, nchains: int, spectrum) -> str:
 
 
 def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, poles: bool | None = None,
-        log=sys.stdout):
-    """One reference run in ``run_dir``; returns a dict of run statistics."""
+        log=sys.stdout, autotune: bool = False, tune_nwrap: bool = False):
+    """One reference run in ``run_dir``; returns a dict of run statistics.
+    autotune: run the fastest execution schedule (bafqmc_amd.autotune; the same
+    Markov chain); tune_nwrap: let it also choose N_Ortho under its drift bound
+    (info.txt then reports the N_Ortho used)."""
     t0 = time.perf_counter()
     run_dir = Path(run_dir)
     pairing = solver in ("pairing", "pa", "nambu")
@@ -130,9 +143,20 @@ def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, 
     if poles is None:
         poles = not pairing          # the pairing executable writes no pole files
     phis, seeds = read_fields(run_dir, p, nchains)
+    schedule = None
+    if autotune:
+        from . import autotune as tuner
+        from .native import load_library
+        choice = tuner.tune_model(lib or load_library(), p, chains=min(nchains, 64), sweeps=2,
+                                  tune_nwrap=tune_nwrap)
+        schedule = [choice.schedule]
+        if choice.nwrap != p.nwrap:
+            p = dataclasses.replace(p, nwrap=choice.nwrap)
+        print(f"bafqmc_amd: schedule {choice.schedule}, N_Ortho {p.nwrap} "
+              f"({choice.ms_per_sweep:.2f} ms per sweep of {min(nchains, 64)} chains)", file=log)
     spec = ModelSpec(p)
     (run_dir / "info.txt").write_text(info_header(p, nchains, spec.spectrum))
-    s = BatchedSolver([spec], [0] * nchains, lib=lib)
+    s = BatchedSolver([spec], [0] * nchains, lib=lib, schedule=schedule)
     try:
         s.set_fields(list(phis), seeds)
         therm = None
@@ -145,6 +169,7 @@ def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, 
         s.prepare()
         s.read_bin()           # clear prepare's counters
         acc = np.zeros(2)
+        proposals = np.zeros(2)
         xmax, xmean, nwrap_total = 0.0, 0.0, 0.0
         for b in range(p.nbin):
             s.sweep(p.nsweep, measure=True)
@@ -156,6 +181,7 @@ def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, 
                     acc[0] += st["acc_u1"] / st["prop_u1"]
                 if st["prop_u2"] > 0:
                     acc[1] += st["acc_u2"] / st["prop_u2"]
+                proposals += (st["prop_u1"], st["prop_u2"])
                 xmax = max(xmax, st["xmax"])
                 xmean += st["xmean_sum"]
                 nwrap_total += st["nwrap"]
@@ -180,10 +206,19 @@ def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, 
                 accept = sum(d["_stats"][f"therm_acc_u{k + 1}"] for d in therm) / prop
                 tail.append(f" Thermalize {name} Accept Ratio                     : {accept}")
     denom = max(nchains * p.nbin, 1)
-    if abs(p.u1) > 1e-10:
-        tail.append(f" Accept_U1_shift                                : {acc[0] / denom}")
-    if abs(p.u2) > 1e-10:
-        tail.append(f" Accept_U2_shift                                : {acc[1] / denom}")
+    for k, name in ((0, "U1"), (1, "U2")):
+        if abs((p.u1, p.u2)[k]) <= 1e-10:
+            continue
+        tail.append(f" Accept_{name}_shift                                : {acc[k] / denom}")
+        if pairing:
+            # Local_control_print (src/pairing/src/local_sweep.f90:163-216): the
+            # phase of the local ratio det(P)^(-1/2).  In the reduced 2Ns form the
+            # ratio is |det P_2Ns|^-1 times positive Gaussian factors, so its
+            # argument is zero and no proposal lands on the pi branch.
+            tail += [f" Phase_{name}_mean_abs_arg_proposed                : {0.0}",
+                     f" Phase_{name}_mean_abs_arg_accepted                : {0.0}",
+                     f" Phase_{name}_branch_pi_fraction                   : {0.0}",
+                     f" Phase_{name}_local_proposals                      : {float(proposals[k])}"]
     tail += [f" Max diff Matrix                                : {xmax}",
              f" Mean diff Matrix                               : {xmean / max(nwrap_total, 1.0)}",
              f" Tot CPU time                                   : {wall}"]

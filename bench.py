@@ -332,10 +332,20 @@ def run_learned(args, cfg, p, spec, nchains):
 
     sf1, wall1, _ = run_phase(s, learned_step, K, S, f"learned ({hmc_steps} step(s) of angle {angle:.3f})")
     st_learned = s.stats()
+    # ---- the same move without the network (force 0): what the network adds
+    accepted_nn = []
+
+    def gaussian_step():
+        accepted_nn.append(float(GM.hmc_global_device(s, None, angle, hmc_steps, rng).mean()))
+        s.measure_pass(1, mode=args.measure)
+        return s.last_kernel_ms()
+
+    sf2, wall2, _ = run_phase(s, gaussian_step, K, S, "same move without the network")
     conv_ms = net.time_forward(min(256, nchains), 5)
     desc = dict(lt=p.ltrot, ly=p.nly, lx=p.nlx, channels=2, hidden=hidden, layers=layers)
     a0 = statistics(sf0, wall0, K, "plain")
     a1 = statistics(sf1, wall1, K, "learned")
+    a2 = statistics(sf2, wall2, K, "no network")
     gain = a1["inverse_variance_per_s"] / a0["inverse_variance_per_s"]
     value = a1["inverse_variance_per_s"] * a0["variance_per_step"]
     nch = nch_of(p)
@@ -388,6 +398,9 @@ def run_learned(args, cfg, p, spec, nchains):
                                             "heldout_r2": r2, "target_rms": force_rms,
                                             "force_scale": list(map(float, fscale)), "train_s": t_train,
                                             "train_samples": len(x)}),
+                   "without_network": dict(a2, global_acceptance=float(np.mean(accepted_nn))),
+                   "gain_of_network_at_equal_error_bars": a1["inverse_variance_per_s"] / a2["inverse_variance_per_s"],
+                   "residual_force_rms": force_rms,    # the Gaussian part's force -(phi - c) has rms 1
                    "gain_eff_samples": a1["eff_samples_per_s"] / a0["eff_samples_per_s"],
                    "gain_at_equal_error_bars": gain, "target_gain": 10.0,
                    "kernels": kernels, "learned_step_time_share_ms": share},
@@ -520,11 +533,14 @@ def main():
                          "the first --steps of them")
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--chains", type=int, default=0)
+    ap.add_argument("--mu", type=float, default=None, help="chemical potential (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", default="auto",
                     help="'auto' (autotuner, before the timed region), 'default' (library choice) or 'PATH,DELAYED'")
-    ap.add_argument("--tune-nwrap", action="store_true",
-                    help="let the autotuner also choose Nwrap under its drift bound")
+    ap.add_argument("--no-tune-nwrap", dest="tune_nwrap", action="store_false",
+                    help="keep the workload's Nwrap (default: the autotuner also chooses Nwrap among the "
+                         "divisors of Ltrot near it, admitting only intervals whose measured drift stays "
+                         "below 1e-7)")
     ap.add_argument("--hmc-steps", type=lambda v: [int(x) for x in v.split(",")], default=[1, 2],
                     help="integrator step counts tried (each a quarter rotation in total)")
     ap.add_argument("--measure", default="current", choices=["all", "checkpoints", "current"],
@@ -558,6 +574,8 @@ def main():
     if "models" in cfg:
         run_uneven(args, cfg, nchains)
         return
+    if args.mu is not None:
+        cfg = dict(cfg, p=dict(cfg["p"], mu=args.mu), name=cfg["name"].replace(f"mu={cfg['p']['mu']:g}", f"mu={args.mu:g}"))
     p = make_params(cfg["p"])
     spec = ModelSpec(p)
     if args.mode == "learned":

@@ -89,6 +89,20 @@ CASES = {
     "pairing_traj_12x12_c5": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12,
                                          ltrot=100, beta=10.0, nwrap=10, nbin=1, shift_loc=1.0, ini_type=2,
                                          ini_ampl=1.0), seeds=[24681357]),
+    # pairing solver with the U1 channel alone (U2 = 0): the inputs of the
+    # reference's 3x3 paper points at Delta = 0 and 0.3 (dtau = 0.01, Ltrot = 400,
+    # small initial fields, 500 Gaussian warm-up sweeps), two chains, three bins
+    "pairing_traj_3x3_d0": dict(p=dict(pairing=True, u1=1.0, u2=0.0, mu=-5.0, rdelta=0.0, nlx=3, nly=3, ltrot=400,
+                                       beta=4.0, nwrap=10, nbin=3, shift_loc=1.0, ini_type=2, ini_ampl=0.1,
+                                       nwarm=500), seeds=[31415926, 27182818]),
+    "pairing_traj_3x3_d03": dict(p=dict(pairing=True, u1=1.0, u2=0.0, mu=-5.0, rdelta=0.3, nlx=3, nly=3, ltrot=400,
+                                        beta=4.0, nwrap=10, nbin=3, shift_loc=1.0, ini_type=2, ini_ampl=0.1,
+                                        nwarm=500), seeds=[16180339]),
+    # pairing with both channels at U2 < 0 has no finite trace (docs/algorithm.md:152-156):
+    # the chain runs to a pole of (1 - B)^-1 within a few dozen sweeps, 3x3, beta = 1
+    "pairing_blowup_3x3": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=3, nly=3, ltrot=10,
+                                      beta=1.0, nwrap=10, nbin=20, shift_loc=1.0, ini_type=2, ini_ampl=1.0),
+                               seeds=[13579113]),
 }
 
 

@@ -134,3 +134,54 @@ def assert_obs_close(v, ref, label, rtol=OBS_RTOL, atol=OBS_ATOL):
     if not np.all(err <= bound):
         raise AssertionError(f"{label}: |diff| {np.max(err):.3e} > bound {np.min(bound):.3e}; "
                              f"got {v!r} want {ref!r}")
+
+
+# ------------------------------------------------------------- statistics
+STAT_OBS = {"density_total": ("density_total", False), "energy_density": ("energy_density", False),
+            "sf_K": ("sf_K", True), "dw_K": ("dw_K", True)}
+
+
+def run_statistics(lib, params: Params, chains: int, therm: int, meas: int, seed: int, read_every: int = 20):
+    """`chains` independent chains of one model, seeded from chain_seeds(seed):
+    the Gaussian warm-up of the input (Local_sweep_therm), Local_sweep_pre,
+    `therm` unmeasured sweeps, then `meas` measured sweeps.  Returns the
+    per-chain time averages {observable: [chains]} (real parts), every chain
+    counted, and per-chain diagnostics (largest compare_mat drift, whether every
+    value stayed finite)."""
+    from bafqmc_amd.analysis import chain_seeds
+    from bafqmc_amd.model import conf_set_batch
+    from bafqmc_amd.solver import COMPLEX_SLOTS, OBS_SLOTS, O_NOBS, STATS
+    phis, seeds = conf_set_batch(params, chain_seeds(seed, chains))
+    s = BatchedSolver([ModelSpec(params)], [0] * chains, lib=lib)
+    try:
+        s.set_fields(list(phis), seeds)
+        if params.nwarm:
+            s.warmup(params.nwarm)
+        s.prepare()
+        if therm:
+            s.sweep(therm, measure=False)
+        s.read_bin_raw()
+        sums = np.zeros((chains, len(STAT_OBS)))
+        nobs = np.zeros(chains)
+        drift = np.zeros(chains)
+        done = 0
+        while done < meas:
+            k = min(read_every, meas - done)
+            s.sweep(k)
+            obs, st = s.read_bin_raw()
+            for j, (slot_name, cplx) in enumerate(STAT_OBS.values()):
+                sums[:, j] += obs[:, COMPLEX_SLOTS[slot_name] if cplx else OBS_SLOTS[slot_name]]
+            nobs += obs[:, O_NOBS]
+            drift = np.maximum(drift, st[:, STATS["xmax"]])
+            done += k
+    finally:
+        s.close()
+    means = sums / nobs[:, None]
+    return ({k: means[:, j] for j, k in enumerate(STAT_OBS)},
+            {"drift": drift, "finite": np.all(np.isfinite(means), axis=1)})
+
+
+def mean_and_error(x):
+    """Grand mean and its standard error from the spread of independent chains' means."""
+    x = np.asarray(x, dtype=float)
+    return float(np.mean(x)), float(np.std(x, ddof=1) / np.sqrt(len(x)))

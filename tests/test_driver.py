@@ -5,6 +5,8 @@ the GPU test runs a two-chain case (seeds.txt field generation, chain averaging)
 """
 from __future__ import annotations
 
+import io
+
 import json
 
 import numpy as np
@@ -64,6 +66,15 @@ def _check_outputs(z, pd, d):
     for label in labels:
         mine = golden_info_value({"info": info}, label)
         assert abs(mine - golden_info_value(z, label)) < 1e-12, label
+    # the same rows in the same order as the reference's info.txt
+    # (write_info, Therm/Local_control_print, main.f90:59-76)
+    def rows(text):
+        return [line.split(":")[0].strip() for line in text.splitlines() if ":" in line]
+    assert rows(info) == rows(str(z["info"]))
+    for label in [r for r in rows(str(z["info"])) if r.startswith("Phase_")]:
+        # the reference's phases are its rounding (~1e-12); zero in the reduced form
+        tol = 1e-9 if "mean_abs_arg" in label else 1e-12
+        assert abs(golden_info_value({"info": info}, label) - golden_info_value(z, label)) < tol, label
 
 
 def test_read_params_roundtrip(tmp_path):
@@ -96,10 +107,21 @@ def test_driver_matches_reference_outputs_emulator(tmp_path, emu_lib, name, solv
     _check_outputs(z, pd, d)
 
 
+def test_driver_autotuned_schedule_gives_the_same_outputs_emulator(tmp_path, emu_lib):
+    """--autotune picks the fastest schedule; every schedule is the same Markov chain."""
+    z, pd, d = _write_case(tmp_path, "nc_traj_3x3_u2")
+    log = io.StringIO()
+    driver.run(d, "number_conserving", len(z["seeds"]), lib=emu_lib, autotune=True, log=log)
+    assert "schedule" in log.getvalue()
+    _check_outputs(z, pd, d)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,solver", [("nc_traj_3x3_c1", "number_conserving"),
                                          ("pairing_traj_3x3", "pairing"),
-                                         ("nc_warm_3x3", "number_conserving")])
+                                         ("nc_warm_3x3", "number_conserving"),
+                                         ("pairing_warm_3x3", "pairing"),
+                                         ("pairing_traj_3x3_d03", "pairing")])
 def test_driver_matches_reference_outputs_gpu(tmp_path, hip_lib, name, solver):
     z, pd, d = _write_case(tmp_path, name)
     driver.run(d, solver, len(z["seeds"]), lib=hip_lib)

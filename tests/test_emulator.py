@@ -18,7 +18,7 @@ from bafqmc_amd.model import ModelSpec
 from bafqmc_amd.solver import BatchedSolver
 from tests.test_gpu_parity import (check_sweep_vs_oracle, check_golden, check_mfma_layout, check_lane_ops, check_qrp,
                                    check_qrp_ties,
-                                   check_uneven_batch, check_gaussian, GOLDEN)
+                                   check_uneven_batch, check_gaussian, check_blowup, GOLDEN)
 
 
 def test_emu_mfma_layout(emu_lib):
@@ -61,13 +61,18 @@ def test_emu_global_path_matches_oracle(emu_lib, case, delayed, monkeypatch):
     check_sweep_vs_oracle(emu_lib, case)
 
 
-# the 12x12 goldens (N = 144, 288) run on the GPU only: minutes on the emulator
-EMU_GOLDEN = [g for g in GOLDEN if "12x12" not in g]
+# the 12x12 goldens (N = 144, 288) run on the GPU only: minutes on the emulator;
+# of the two Ltrot = 400 single-channel pairing goldens the emulator runs one
+EMU_GOLDEN = [g for g in GOLDEN if "12x12" not in g and g != "pairing_traj_3x3_d0"]
 
 
 @pytest.mark.parametrize("name", EMU_GOLDEN)
 def test_emu_matches_reference_fortran(emu_lib, name):
     check_golden(emu_lib, name)
+
+
+def test_emu_blowup_at_negative_u2_is_the_references(emu_lib):
+    check_blowup(emu_lib)
 
 
 @pytest.mark.parametrize("width", [0, 4, 8])

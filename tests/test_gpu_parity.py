@@ -206,7 +206,8 @@ def check_sweep_vs_oracle(lib, case):
 
 GOLDEN = ["nc_fixed_3x3_w1", "nc_fixed_3x3_w6", "pairing_fixed_3x3_w2", "nc_traj_3x3_c1",
           "nc_traj_3x3_u2", "pairing_traj_3x3", "nc_warm_3x3", "pairing_warm_3x3", "pairing_traj_6x6_c2",
-          "nc_traj_9x9_c4", "nc_transfer_6x6", "pairing_transfer_6x6", "nc_traj_12x12", "pairing_traj_12x12_c5"]
+          "nc_traj_9x9_c4", "nc_transfer_6x6", "pairing_transfer_6x6", "nc_traj_12x12", "pairing_traj_12x12_c5",
+          "pairing_traj_3x3_d0", "pairing_traj_3x3_d03"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -234,6 +235,8 @@ def check_golden(lib, name):
         s.warmup(pd["nwarm"])
         therm = s.read_bin()
         for k, chan in ((1, "U1"), (2, "U2")):
+            if abs(pd[f"u{k}"]) < 1e-10:
+                continue                   # the reference reports coupled channels only
             want = golden_info_value(z, f"Thermalize {chan} Accept Ratio")
             prop = sum(c["_stats"][f"therm_prop_u{k}"] for c in therm)
             acc = sum(c["_stats"][f"therm_acc_u{k}"] for c in therm)
@@ -272,6 +275,40 @@ def check_golden(lib, name):
             if len(seeds) == 1:
                 np.testing.assert_allclose(lw, z["out_log_weight"][-1][0], rtol=1e-10)
     s.close()
+
+
+def check_blowup(lib):
+    """Pairing with U2 < 0 has no finite trace (docs/algorithm.md:152-156): the
+    golden chain (3x3, U1 = 1, U2 = -0.5, Delta = 0.5, mu = -5, beta = 1) reaches
+    a pole of (1 - B)^-1 in its 15th sweep in the reference Fortran, where its
+    density jumps from 0.35 to 1.5e3.  The same seed here is the same Markov
+    chain: the same rows before the jump (1e-9), the jump in the same sweep, and
+    after it the same accept/reject sequence (rows to 1e-4: the drift of the
+    stabilisation is O(1) there), final fields and seed."""
+    z, pd = load_golden("pairing_blowup_3x3")
+    p = solver_params(pd)
+    phis, seeds = conf_set_batch(p, [int(x) for x in z["seeds"]])
+    s = BatchedSolver([ModelSpec(p)], [0], lib=lib)
+    s.set_fields(list(phis), seeds)
+    s.prepare()
+    ref = z["out_density_total"][:, 0]
+    jump = int(np.argmax(np.abs(ref) > 1.0))
+    assert jump == 14 and np.all(np.abs(ref[:jump]) < 1.0)
+    for b in range(pd["nbin"]):
+        s.sweep(1)
+        got = s.read_bin()[0]["density_total"]
+        rtol = 1e-9 if b < jump else 1e-4
+        assert abs(got - ref[b]) <= rtol * abs(ref[b]), (b, got, ref[b])
+    f, sd = s.get_fields()
+    assert sd[0] == int(z["seeds_out"][0])
+    np.testing.assert_allclose(f[0], z["fields_out"][0], rtol=0, atol=1e-9)
+    assert float(s.stats()[0, 4]) > 5.5e-5        # the drift flags it, as the reference's Max diff Matrix does
+    assert golden_info_value(z, "Max diff Matrix") > 5.5e-5
+    s.close()
+
+
+def test_blowup_at_negative_u2_is_the_references(hip_lib):
+    check_blowup(hip_lib)
 
 
 @pytest.mark.parametrize("delayed", [-1, 0])

@@ -3,23 +3,27 @@ published DQMC numbers on 3x3 clusters (benchmarks/paper/data/observables.csv,
 collected in tests/golden/paper_3x3.json by tests/golden/make_paper_fixture.py).
 
 Each case runs the reference's inputs (same Hamiltonian, Ltrot, Nwrap,
-shiftLoc, Gaussian warm-up of Nwarm sweeps) as 256 independent chains on the
-GPU (one per CU); the standard error comes from the spread of the chain means.  Required:
-|mean - reference DQMC| <= 2 sqrt(sigma^2 + sigma_ref^2) for every observable,
-and |mean - ED| <= 2 sigma wherever the reference's own published DQMC agrees
-with ED within 2 of its standard errors.  Where it does not (the pairing case
-at Delta = 0.3: the published DQMC is 7 standard errors from its ED, a
-systematic error of the method at dtau = 0.01 or of the ED's particle-number
-cutoff), the ED distance is reported, not asserted.  Dilute pairing chains
-equilibrate slowly from the initial fields, hence the long thermalisation
-there.  At Delta = 0 (density 3e-6 per site) the chains mix very slowly: the
-means of single chains over 100 sweeps spread from 0.3x to 2x the ensemble
-mean, and runs with 200 and 1000 thermalisation sweeps land 1.9 and 2.6
-standard errors above ED and the published DQMC (whose 100000-sweep chains
-average this out).  Trajectories at Delta = 0 agree with the reference's
-bit for bit (tests/test_gpu_parity.py), so this case is held to 3 standard
-errors and its z-scores are printed.  The run is seeded, so the outcome is
-reproducible.
+shiftLoc, initial field amplitude and Gaussian warm-up of Nwarm sweeps) as 256
+independent chains on the GPU (one per CU), THERM local sweeps unmeasured, then
+MEAS measured sweeps.  Every chain is counted; none may cross the reference's
+"ortho unstable" drift threshold (stabilization.f90:320,358) or turn
+non-finite.  The standard error comes from the spread of the chains' time
+averages.  Required: |mean - reference DQMC| <= 2 sqrt(sigma^2 + sigma_ref^2)
+for every observable, and |mean - ED| <= 2 sigma wherever the reference's own
+published DQMC agrees with ED within 2 of its standard errors.
+
+Where it does not (the pairing case at Delta = 0.3: the published DQMC is 7 of
+its standard errors from its ED), the ED distance is reported, not asserted:
+that ED is a finite space of at most 4 bosons, and at Delta = 0.3 the pair
+field populates its edge -- on the free gas (U1 = U2 = 0) the same space misses
+the closed form by 0.2 % (tests/test_ed_oracle.py), while the DQMC-ED gap here
+is 0.09 %.
+
+Chain means over 100 sweeps at Delta = 0 (density 3e-6 per site) spread from
+0.3x to 2x the ensemble mean; with so few sweeps the spread underestimates the
+error of the mean and runs landed 1.9 and 2.6 standard errors above ED.  Over
+1000 sweeps per chain the means are close to normal.  The run is seeded, so
+the outcome is reproducible.
 """
 from __future__ import annotations
 
@@ -30,69 +34,47 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from bafqmc_amd.analysis import chain_seeds
-from bafqmc_amd.model import Params, ModelSpec, conf_set_batch
-from bafqmc_amd.solver import BatchedSolver
+from bafqmc_amd.model import Params
+from tests.helpers import mean_and_error, run_statistics
 
 pytestmark = pytest.mark.gpu
 
 FIXTURE = json.loads((Path(__file__).resolve().parent / "golden" / "paper_3x3.json").read_text())
 NAMES = {"density_total": "density_total", "energy_density": "energy_density", "S_SF_K": "sf_K",
          "S_DW_K": "dw_K"}
-CHAINS, MEAS = 256, 100
+CHAINS, MEAS = 256, 1000
 THERM = {False: 50, True: 1000}      # local sweeps before measuring, by pairing
-ZMAX = {"triangle_pairing_delta_3x3_D0_mum5_b4": 3.0}
 UNSTABLE_DRIFT = 5.5e-5
 
 
-def run_case(lib, case):
-    d = FIXTURE[case]["params"]
+def case_params(d):
     p = Params(pairing=d["pairing"], u1=d["u1"], u2=d["u2"], mu=d["mu"], rdelta=d.get("rdelta", 0.0),
                nlx=d["nlx"], nly=d["nly"], ltrot=d["ltrot"], beta=d["beta"], nwrap=d["nwrap"],
                shift_loc=d["shift_loc"], ini_type=d["ini_type"], ini_ampl=d["ini_ampl"],
                is_warm=d["nwarm"] > 0, nwarm=d["nwarm"])
     p.nlx_therm, p.nly_therm, p.ltrot_therm = p.nlx, p.nly, p.ltrot
-    p.validate()
-    phis, seeds = conf_set_batch(p, chain_seeds(20260101 + len(case), CHAINS))
-    s = BatchedSolver([ModelSpec(p)], [0] * CHAINS, lib=lib)
-    s.set_fields(list(phis), seeds)
-    if p.nwarm:
-        s.warmup(p.nwarm)
-    s.prepare()
-    s.sweep(THERM[p.pairing], measure=False)
-    s.read_bin()
-    sums = {k: np.zeros(CHAINS) for k in NAMES}
-    drift = np.zeros(CHAINS)
-    for _ in range(MEAS):
-        s.sweep(1)
-        per = s.read_bin()
-        for c, b in enumerate(per):
-            for k, name in NAMES.items():
-                v = b[name]
-                sums[k][c] += v.real if isinstance(v, complex) else v
-            drift[c] = max(drift[c], b["_stats"]["xmax"])
-    s.close()
-    ok = drift < UNSTABLE_DRIFT
-    return {k: v[ok] / MEAS for k, v in sums.items()}, int((~ok).sum())
+    return p.validate()
 
 
 @pytest.mark.parametrize("case", list(FIXTURE))
 def test_agrees_with_ed_and_reference_dqmc(hip_lib, case):
-    means, unstable = run_case(hip_lib, case)
-    assert unstable <= CHAINS // 50, f"{unstable} chains unstable"
-    report = []
+    d = FIXTURE[case]["params"]
+    print(f"paper case {case}: {CHAINS} chains, {THERM[d['pairing']]} + {MEAS} sweeps", flush=True)
+    means, diag = run_statistics(hip_lib, case_params(d), CHAINS, THERM[d["pairing"]], MEAS,
+                                 20260101 + len(case))
+    unstable = int(np.sum(diag["drift"] >= UNSTABLE_DRIFT))
+    print(f"{case}: largest drift {diag['drift'].max():.2e}, unstable chains {unstable}", flush=True)
+    assert unstable == 0 and diag["finite"].all()
+    lines, failures = [], []
     for k, ref in FIXTURE[case]["observables"].items():
-        x = means[k]
-        m, se = float(np.mean(x)), float(np.std(x, ddof=1) / math.sqrt(len(x)))
+        m, se = mean_and_error(means[NAMES[k]])
         z_ed = (m - ref["ed"]) / se
         z_ref = (m - ref["dqmc"]) / math.hypot(se, ref["stderr"])
-        report.append(f"{k}: {m:.6g} +- {se:.2g}  ED {ref['ed']:.6g} (z {z_ed:+.2f})  "
-                      f"ref {ref['dqmc']:.6g} +- {ref['stderr']:.2g} (z {z_ref:+.2f})")
-        print(case, report[-1], flush=True)
-    for line, (k, ref) in zip(report, FIXTURE[case]["observables"].items()):
-        x = means[k]
-        m, se = float(np.mean(x)), float(np.std(x, ddof=1) / math.sqrt(len(x)))
-        zmax = ZMAX.get(case, 2.0)
-        assert abs(m - ref["dqmc"]) <= zmax * math.hypot(se, ref["stderr"]), line
-        if abs(ref["dqmc"] - ref["ed"]) <= 2.0 * ref["stderr"]:
-            assert abs(m - ref["ed"]) <= zmax * se, line
+        line = (f"{k}: {m:.6g} +- {se:.2g}  ED {ref['ed']:.6g} (z {z_ed:+.2f})  "
+                f"ref {ref['dqmc']:.6g} +- {ref['stderr']:.2g} (z {z_ref:+.2f})")
+        print(case, line, flush=True)
+        if abs(z_ref) > 2.0:
+            failures.append(line)
+        if abs(ref["dqmc"] - ref["ed"]) <= 2.0 * ref["stderr"] and abs(z_ed) > 2.0:
+            failures.append(line)
+    assert not failures, failures
