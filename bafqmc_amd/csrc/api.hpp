@@ -70,6 +70,12 @@ int rt_conv_bwd_weight(const float* gout, int cout, const float* in, int cin, fl
                        NetShape s);
 int rt_mse_grad(const float* y, const float* t, float* g, size_t n, float scale);
 int rt_adam(float* p, const float* g, float* m, float* v, float lr, int step, size_t n);
+// scratch of bafq_net_forward_noalloc: kNetScratchFloats floats of device memory
+// owned by the library (a static device array, one per GPU), never allocated
+constexpr size_t kNetScratchFloats = size_t(16) << 20;     // 64 MiB
+constexpr int kMaxScratchDevices = 64;
+int rt_net_scratch(float** ptr);
+int rt_launch_status();    // error of the most recent kernel launches on this thread
 // learned global moves (gmove.hpp); device pointers, y may be null (no network)
 int rt_gm_to_net(const double* phi, float* x, const GmArgs& a);
 int rt_gm_propose(double* phi, double* old, double* f0, const float* y, const GmArgs& a);
@@ -719,6 +725,40 @@ int bafq_net_forward(bafq_net* h, int batch, const float* x, float* y) {
     if (net_forward_dev(h, batch)) return 1;
     if (rt_sync() || rt_d2h(y, h->act[h->d.layers], n * sizeof(float))) return net_fail(h, rt_error());
     return 0;
+}
+
+int bafq_net_forward_noalloc(const bafq_net_desc* d, const float* params, int batch, const float* x, float* y) {
+    using namespace bafq;
+    if (!d || !params || !x || !y || batch < 1 || d->lt < 1 || d->ly < 1 || d->lx < 1 || d->channels < 1 ||
+        d->hidden < 1 || d->layers < 2)
+        return BAFQ_NOALLOC_INVALID;
+    const size_t vol = (size_t)d->lt * d->ly * d->lx;
+    const size_t per_field = 2 * (size_t)d->hidden * vol;       // two ping-pong hidden activations
+    if (per_field > kNetScratchFloats) return BAFQ_NOALLOC_TOO_LARGE;
+    float* scr = nullptr;
+    if (rt_net_scratch(&scr)) return BAFQ_NOALLOC_DEVICE;
+    // the batch runs in chunks whose hidden activations fit in the scratch
+    const int chunk = (int)std::min<size_t>((size_t)batch, kNetScratchFloats / per_field);
+    float* hid[2] = {scr, scr + (size_t)chunk * d->hidden * vol};
+    const NetShape s{d->lt, d->ly, d->lx};
+    const int L = d->layers;
+    const size_t io = (size_t)d->channels * vol;
+    for (int b0 = 0; b0 < batch; b0 += chunk) {
+        const int nb = std::min(chunk, batch - b0);
+        const float* in = x + (size_t)b0 * io;
+        size_t off = 0;                           // parameters: W[cout][cin][27], b[cout] per layer
+        for (int l = 0; l < L; ++l) {
+            const int cin = l == 0 ? d->channels : d->hidden;
+            const int cout = l == L - 1 ? d->channels : d->hidden;
+            float* out = l == L - 1 ? y + (size_t)b0 * io : hid[l & 1];
+            const size_t nw = (size_t)cout * cin * 27;
+            if (rt_conv_fwd(in, cin, params + off, params + off + nw, out, cout, nb, s, l < L - 1 ? 1 : 0))
+                return BAFQ_NOALLOC_DEVICE;
+            off += nw + cout;
+            in = out;
+        }
+    }
+    return rt_launch_status() ? BAFQ_NOALLOC_DEVICE : 0;
 }
 
 int bafq_net_time_forward(bafq_net* h, int batch, int reps, double* ms) {

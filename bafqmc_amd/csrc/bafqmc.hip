@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <mutex>
 #include <string>
 
 #include "api.hpp"
@@ -405,6 +406,32 @@ int rt_adam(float* p, const float* g, float* m, float* v, float lr, int step, si
     hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, 0, p, g, m, v, lr, step, n);
     return check(hipGetLastError(), "k_adam");
 }
+
+// Hidden activations of bafq_net_forward_noalloc: a file-scope device array,
+// one copy per GPU, held until the process ends.  In host code its name is
+// only a stand-in: the device address comes from hipGetSymbolAddress, taken
+// once per device with that device current.
+static __device__ __align__(256) float scratch[kNetScratchFloats];
+
+int rt_net_scratch(float** ptr) {
+    static std::mutex mu;
+    static float* per_device[kMaxScratchDevices] = {};
+    int dev = -1;
+    if (check(hipGetDevice(&dev), "hipGetDevice")) return 1;
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        g_err = "device index beyond the scratch table";
+        return 1;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!per_device[dev]) {
+        void* p = nullptr;
+        if (check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return 1;
+        per_device[dev] = (float*)p;
+    }
+    *ptr = per_device[dev];
+    return 0;
+}
+int rt_launch_status() { return check(hipGetLastError(), "kernel launch"); }
 
 // Lane-layout probe of the FP32 MFMA 16x16x4: as k_debug_mfma.
 __global__ void __launch_bounds__(64) k_debug_mfma32(float* out) {

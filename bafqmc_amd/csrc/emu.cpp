@@ -409,6 +409,14 @@ int rt_adam(float* p, const float* g, float* m, float* v, float lr, int step, si
     return 0;
 }
 
+// The emulator's "device" is host memory: the scratch is a static host array.
+alignas(256) static float scratch[kNetScratchFloats];
+int rt_net_scratch(float** ptr) {
+    *ptr = scratch;
+    return 0;
+}
+int rt_launch_status() { return 0; }
+
 int rt_debug_lanes(const double* in, const int32_t* idx, double* out) {
     run_groups(1, 64, 0, [&](EmuCtx& c) {
         const int l = c.lane();

@@ -63,6 +63,7 @@ SYMBOLS = {
     "bafq_net_set_params": (C.c_int, [C.c_void_p, c_float_p]),
     "bafq_net_get_params": (C.c_int, [C.c_void_p, c_float_p]),
     "bafq_net_forward": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p]),
+    "bafq_net_forward_noalloc": (C.c_int, [C.POINTER(NetDesc), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "bafq_net_gradient": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, c_double_p, c_float_p]),
     "bafq_net_time_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p]),
     "bafq_net_train": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, C.c_int, C.c_float, c_double_p]),

@@ -184,6 +184,24 @@ int bafq_net_get_params(bafq_net* h, float* params);
 int bafq_net_forward(bafq_net* h, int batch, const float* x, float* y);
 /* loss = |f(x) - target|^2 / size and its gradient (grad may be NULL) */
 int bafq_net_gradient(bafq_net* h, int batch, const float* x, const float* target, double* loss, float* grad);
+/* y = f(x) without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  params, x and y are DEVICE
+ * pointers the caller owns (params in the layout of bafq_net_set_params, x
+ * and y [batch][channels][lt][ly][lx], x and y not overlapping).  The hidden
+ * activations live in the library's static device scratch of 16 Mi floats
+ * (64 MiB per GPU, held until the process ends); the batch runs in chunks of
+ * floor(16 Mi / (2 hidden lt ly lx)) fields.  Kernels are enqueued on the
+ * default stream: synchronise before reading y.  Calls that overlap on one
+ * GPU share its scratch and race.  Nothing needs clearing between calls
+ * (every scratch entry a call reads it has written).  Same arithmetic as
+ * bafq_net_forward.  Returns 0, or: */
+enum {
+    BAFQ_NOALLOC_INVALID = 1,    /* null pointer, batch < 1 or invalid description */
+    BAFQ_NOALLOC_TOO_LARGE = 2,  /* one field's activations exceed the scratch: 2 hidden lt ly lx > 16 Mi */
+    BAFQ_NOALLOC_DEVICE = 3      /* a HIP error (device lookup or kernel launch) */
+};
+int bafq_net_forward_noalloc(const bafq_net_desc* d, const float* params, int batch, const float* x, float* y);
 /* Device time in milliseconds of one forward pass over the first `batch`
  * inputs already on the device (the last forward or training input), the
  * average of `reps` passes between HIP events on the stream they run on. */
