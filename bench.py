@@ -62,14 +62,14 @@ CONFIGS = {
     "c4": dict(name="number-conserving 9x9 beta=8 Ltrot=80 U1=1 U2=0 mu=-5, 256 chains, "
                     "4-layer 32-channel FP32 proposal network",
                p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=80, beta=8.0, nwrap=10,
-                      shift_loc=1.5), chains=256, net=dict(layers=4, hidden=32)),
+                      shift_loc=1.5), chains=256, net=dict(layers=4, hidden=32), slug="nc9x9_256chains"),
     "c5": dict(name="pairing 12x12 beta=10 Ltrot=100 U1=1 U2=-0.5 Delta=0.3 mu=-5, 128 chains, "
                     "4-layer 32-channel FP32 proposal network",
                p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12, ltrot=100, beta=10.0,
                       nwrap=10, shift_loc=1.0), chains=128, net=dict(layers=4, hidden=32)),
     "c2": dict(name="pairing 6x6 beta=8 Ltrot=80 U1=1 U2=-0.5 Delta=0.5 mu=-5, 512 chains",
                p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=80, beta=8.0,
-                      nwrap=10, shift_loc=1.0), chains=512, net=dict(layers=4, hidden=32)),
+                      nwrap=10, shift_loc=1.0), chains=512, net=dict(layers=4, hidden=32), slug="pairing6x6_512chains"),
     "uneven": dict(name="uneven batch: number-conserving 3x3/6x6/9x9/12x12 x beta 4/8 x U2 0/-0.5, "
                         "U1=1 mu=-5 dtau=0.1, 16 chains each, 256 chains",
                    models=[dict(pairing=False, u1=1.0, u2=u2, mu=-5.0, nlx=L, nly=L, ltrot=int(round(b / 0.1)),
@@ -134,13 +134,19 @@ def net_flops(desc, batch):
     return tot * batch
 
 
-def measured_traffic(config, kernel):
+def measured_traffic(cfg, kernel, nwrap):
     """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-calibrated) of this workload, when there is one."""
-    f = ROOT / "profiles" / f"{kernel}_{config}_hbm.json"
+    WRITE_SIZE summary (copy-calibrated) of this workload at this Nwrap, when
+    there is one."""
+    if "slug" not in cfg:
+        return None
+    f = ROOT / "profiles" / f"{kernel}_{cfg['slug']}_hbm.json"
     if not f.exists():
         return None
-    return json.loads(f.read_text())["traffic_bytes_per_launch"]
+    rec = json.loads(f.read_text())
+    if rec.get("nwrap", nwrap) != nwrap or rec.get("mu", cfg["p"]["mu"]) != cfg["p"]["mu"]:
+        return None
+    return rec["traffic_bytes_per_launch"]
 
 
 def make_params(d):
@@ -364,15 +370,16 @@ def run_learned(args, cfg, p, spec, nchains):
     if s.schedule()[0][0] != 1:
         for k0, k1 in (("k_sweep_lds", "k_sweep"), ("k_measure_lds", "k_measure"), ("k_prepare_lds", "k_prepare")):
             kernels[k1] = dict(kernels.pop(k0), kernel=k1)
-    # per learned step: network passes (hmc_steps + 1 per chunk), prepare, measurement
+    # per learned step: network passes (one per integrator step and chunk of chains: the output at the
+    # current fields is kept from the previous move), G rebuild (prepare), measurement
     share = {"measure": float(np.mean(phase_ms["measure"])), "prepare": float(np.mean(phase_ms["move_prepare"])),
-             "conv": conv_ms * (hmc_steps + 1) * math.ceil(nchains / min(256, nchains))}
+             "conv": conv_ms * hmc_steps * math.ceil(nchains / min(256, nchains))}
     dominant = max(share, key=share.get)
     top = {"measure": kernels.get("k_measure_lds", kernels.get("k_measure")),
            "prepare": kernels.get("k_prepare_lds", kernels.get("k_prepare")),
            "conv": kernels["k_conv_mfma (one forward pass)"]}[dominant]
     top = dict(top)
-    top["traffic"] = measured_traffic(args.config, top["kernel"])
+    top["traffic"] = measured_traffic(cfg, top["kernel"], p.nwrap)
     unstable = int(np.sum(np.maximum(st_plain[:, 4], st_learned[:, 4]) >= UNSTABLE_DRIFT))
     if args.dump:
         np.savez(args.dump, sf_plain=sf0, sf_learned=sf1, wall_plain=wall0, wall_learned=wall1)
@@ -441,7 +448,7 @@ def run_plain(args, cfg, p, spec, nchains):
     a = statistics(sf, wall, args.steps, "plain")
     kern = "k_sweep_lds" if s.schedule()[0][0] == 1 else "k_sweep"
     rf = roofline(kern, flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch_of(p)) * nchains, kms)
-    rf["traffic"] = measured_traffic(args.config, kern)
+    rf["traffic"] = measured_traffic(cfg, kern, p.nwrap)
     out = {
         "metric": METRIC, "value": a["eff_samples_per_s"], "unit": "effective S_SF(K) samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": a["ms_per_step"],
@@ -534,6 +541,8 @@ def main():
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--mu", type=float, default=None, help="chemical potential (default: the workload's)")
+    ap.add_argument("--nwrap", type=int, default=None,
+                    help="stabilisation interval (default: the workload's, or the autotuner's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", default="auto",
                     help="'auto' (autotuner, before the timed region), 'default' (library choice) or 'PATH,DELAYED'")
@@ -576,6 +585,8 @@ def main():
         return
     if args.mu is not None:
         cfg = dict(cfg, p=dict(cfg["p"], mu=args.mu), name=cfg["name"].replace(f"mu={cfg['p']['mu']:g}", f"mu={args.mu:g}"))
+    if args.nwrap is not None:
+        cfg = dict(cfg, p=dict(cfg["p"], nwrap=args.nwrap))
     p = make_params(cfg["p"])
     spec = ModelSpec(p)
     if args.mode == "learned":

@@ -366,6 +366,57 @@ def test_uneven_batch_across_paths_equals_single_chains(hip_lib):
         s1.close()
 
 
+def test_uneven_workload_batch_equals_single_chains(hip_lib):
+    """The uneven batch at its full size: 256 number-conserving chains, 16 each
+    of lattices 3x3, 6x6, 9x9, 12x12 x beta 4, 8 (dtau 0.1, Ltrot 40, 80) x
+    U2 0, -0.5 (U1 = 1, mu = -5), in one handle with schedules from the
+    autotuner; after two sweeps one chain of every model (16 chains) equals a
+    run of that chain alone under the same schedule, bit for bit (observables,
+    G, fields, seed)."""
+    from bafqmc_amd import autotune
+    from bafqmc_amd.analysis import chain_seeds
+    params = []
+    for L in (3, 6, 9, 12):
+        for beta in (4.0, 8.0):
+            for u2 in (0.0, -0.5):
+                params.append(solver_params(dict(pairing=False, u1=1.0, u2=u2, mu=-5.0, nlx=L, nly=L,
+                                                 ltrot=int(round(beta / 0.1)), beta=beta, nwrap=10, shift_loc=1.5,
+                                                 ini_type=2, ini_ampl=1.0)))
+    chain_model = np.arange(256) % len(params)
+    choices = autotune.tune(hip_lib, params, chain_model, sweeps=1, max_sample=16)
+    schedules = [c.schedule for c in choices]
+    print("uneven workload schedules:", schedules, flush=True)
+    assert {sc[0] for sc in schedules} == {0, 1}          # both paths in one launch graph
+    models = [ModelSpec(p) for p in params]
+    phis, seeds = [], []
+    for c in range(256):
+        f, sd = conf_set_batch(params[chain_model[c]], chain_seeds(777 + c, 1))
+        phis.append(f[0])
+        seeds.append(sd[0])
+    s = BatchedSolver(models, chain_model, lib=hip_lib, schedule=schedules)
+    s.set_fields(phis, seeds)
+    s.prepare()
+    s.sweep(2)
+    batch = s.read_bin()
+    sample = [int(np.flatnonzero(chain_model == m)[m % 16]) for m in range(len(params))]
+    gb = {c: s.green(c) for c in sample}
+    fb, sb = s.get_fields()
+    s.close()
+    for c in sample:
+        m = chain_model[c]
+        s1 = BatchedSolver([models[m]], [0], lib=hip_lib, schedule=[schedules[m]])
+        s1.set_fields([phis[c]], [seeds[c]])
+        s1.prepare()
+        s1.sweep(2)
+        one = s1.read_bin()[0]
+        for k in ("density_total", "energy_density", "sf_K", "dw_K", "kinetic"):
+            assert np.array_equal(batch[c][k], one[k], equal_nan=True), (c, m, k)
+        assert np.array_equal(gb[c], s1.green(0), equal_nan=True), (c, m)
+        f1, s1d = s1.get_fields()
+        assert np.array_equal(fb[c], f1[0]) and sb[c] == s1d[0], (c, m)
+        s1.close()
+
+
 def check_uneven_batch(hip_lib, mixed=False, light=False):
     """Chains of different lattices/solvers in one launch evolve exactly as alone
     (mixed: half of the models on the device-memory path, so both persistent

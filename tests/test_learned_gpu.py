@@ -256,3 +256,81 @@ def test_gradient_network_moves_sample_the_ed_answer(trained_grad, hip_lib):
         ed = FIXTURE[CASE]["observables"][names[k]]["ed"]
         print(f"{k}: {m:.6g} +- {se:.2g}  ED {ed:.6g}  z {(m - ed) / se:+.2f}", flush=True)
         assert abs(m - ed) <= 2.0 * se, (k, m, se, ed)
+
+
+@pytest.fixture(scope="module")
+def trained_force(hip_lib):
+    """The Hamiltonian moves' network: trained on the residual force
+    d log W / d phi + phi of fields from the plain sampler (collect_forces)."""
+    from bafqmc_amd.proposal import collect_forces
+    p = _params()
+    s = _solver(hip_lib, p, 8642)
+    x, t = collect_forces(s, p, 8)
+    t64 = t.astype(np.float64)
+    rms = np.sqrt(np.mean(t64 ** 2, axis=(0, 2, 3, 4)))
+    fscale = np.where(rms > 0, rms, 1.0)
+    net = ConvNet(p.ltrot, p.nly, p.nlx, 2, 8, 3, max_batch=256, seed=3, lib=hip_lib)
+    _, r2 = train_on_sweeps(net, x, (t64 / fscale[None, :, None, None, None]).astype(np.float32), 30, 3e-3,
+                            np.random.default_rng(5), batch=64)
+    print(f"force network: held-out R2 {r2:.4f} on {len(x)} samples, residual force rms {rms}", flush=True)
+    yield p, s, net, fscale, r2
+    s.close()
+    net.close()
+
+
+def test_hamiltonian_moves_with_and_without_the_network(trained_force):
+    """Acceptance of the quarter-rotation Hamiltonian move with the learned
+    force and without it (then an exact draw from the Gaussian part alone).
+    On this dilute benchmark (density 0.006 per site) the determinant's force
+    is small against the Gaussian's, so both are accepted almost always; the
+    learned force must not lower the acceptance, and both rates are printed."""
+    p, s, net, fscale, r2 = trained_force
+    fields, seeds = s.get_fields()
+    rng = np.random.default_rng(23)
+    rates = {}
+    for name, use in (("no network", None), ("learned force", net)):
+        s.set_fields(fields, seeds)
+        s.prepare()
+        acc = [float(GM.hmc_global_device(s, use, 0.5 * math.pi, 1, rng, force_scale=fscale).mean())
+               for _ in range(8)]
+        rates[name] = float(np.mean(acc))
+        print(f"quarter-rotation move, {name}: acceptance {rates[name]:.4f} (force network R2 {r2:.3f})",
+              flush=True)
+    s.set_fields(fields, seeds)
+    s.prepare()
+    assert rates["no network"] > 0.3
+    assert rates["learned force"] >= rates["no network"] - 0.03
+
+
+def test_hamiltonian_moves_sample_the_ed_answer(trained_force, hip_lib):
+    """The learned form of the bench -- one Hamiltonian move per step and the
+    estimators on the rebuilt G(beta) -- reproduces ED within 2 standard errors
+    (no local sweeps after the thermalisation)."""
+    from bafqmc_amd.solver import COMPLEX_SLOTS, OBS_SLOTS, O_NOBS
+    p, _, net, fscale, _ = trained_force
+    s = _solver(hip_lib, p, 97531)
+    rng = np.random.default_rng(4049)
+    keys = {"density_total": ("density_total", False), "energy_density": ("energy_density", False),
+            "sf_K": ("sf_K", True), "dw_K": ("dw_K", True)}
+    sums = {k: np.zeros(NCH) for k in keys}
+    nobs = np.zeros(NCH)
+    moves, accepted = 400, 0.0
+    s.read_bin_raw()
+    for _ in range(moves):
+        accepted += float(GM.hmc_global_device(s, net, 0.5 * math.pi, 1, rng, force_scale=fscale).mean())
+        s.measure_pass(1, mode="current")
+        obs, _ = s.read_bin_raw()
+        for k, (slot, cplx) in keys.items():
+            sums[k] += obs[:, COMPLEX_SLOTS[slot] if cplx else OBS_SLOTS[slot]]
+        nobs += obs[:, O_NOBS]
+    s.close()
+    print(f"Hamiltonian-move acceptance {accepted / moves:.3f}", flush=True)
+    assert accepted / moves > 0.3
+    names = {"density_total": "density_total", "energy_density": "energy_density", "sf_K": "S_SF_K",
+             "dw_K": "S_DW_K"}
+    for k in keys:
+        x = sums[k] / nobs
+        m, se = float(np.mean(x)), float(np.std(x, ddof=1) / math.sqrt(NCH))
+        ed = FIXTURE[CASE]["observables"][names[k]]["ed"]
+        print(f"{k}: {m:.6g} +- {se:.2g}  ED {ed:.6g}  z {(m - ed) / se:+.2f}", flush=True)
+        assert abs(m - ed) <= 2.0 * se, (k, m, se, ed)
