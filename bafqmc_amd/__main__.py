@@ -1,6 +1,7 @@
 """Command line: the drop-in for ``mpirun -np R bosonDQMC.out`` in a run directory.
 
     python -m bafqmc_amd run [--solver number_conserving|pairing] [--chains R] [DIR]
+    python -m bafqmc_amd campaign --manifest M --output O [--solver S] [--np R] [--case ID ...]
 
 DIR (default: the current directory) holds paramC_sets.txt, confin.txt and
 seeds.txt; the per-bin observable files, info.txt and confout.txt are written
@@ -31,7 +32,20 @@ def main(argv=None):
                    help="with --autotune, also choose N_Ortho among the divisors of Ltrot near the input's, "
                         "admitting only intervals whose measured drift stays below 1e-7 (results change at "
                         "that level)")
+    cp = sub.add_parser("campaign", help="run the cases of a reference manifest as batched launches")
+    cp.add_argument("--manifest", required=True, help="the reference's manifest.json (cases: id, input_dir)")
+    cp.add_argument("--output", required=True, help="run directories <output>/<case id>/dqmc are created here")
+    cp.add_argument("--solver", default="number_conserving", choices=["number_conserving", "pairing"])
+    cp.add_argument("--case", action="append", default=[], help="run only this case ID; repeatable")
+    cp.add_argument("--np", type=int, default=1, help="chains per case (the reference's MPI ranks)")
+    cp.add_argument("--autotune", action="store_true")
+    cp.add_argument("--tune-nwrap", action="store_true")
     a = ap.parse_args(argv)
+    if a.cmd == "campaign":
+        from .campaign import run_campaign
+        run_campaign(a.manifest, a.output, a.solver, a.np, only=a.case, autotune=a.autotune,
+                     tune_nwrap=a.tune_nwrap)
+        return 0
     from .driver import run
     run(a.directory, a.solver, a.chains, poles=False if a.no_poles else None, autotune=a.autotune,
         tune_nwrap=a.tune_nwrap)

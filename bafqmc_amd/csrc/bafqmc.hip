@@ -330,7 +330,7 @@ int rt_gm_restore(double* phi, const double* old, const int* accept, int per, si
 }
 
 template <int NB>
-__global__ void __launch_bounds__(512) k_conv_mfma(ConvArgs a) {
+__global__ void __launch_bounds__(kConvWaves * 64) k_conv_mfma(ConvArgs a) {
     extern __shared__ __align__(16) char smem[];
     Ctx c{smem};
     conv_mfma_body<NB>(c, a);

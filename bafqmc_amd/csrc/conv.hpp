@@ -6,7 +6,7 @@
 // the per-element definition it must equal).  As a GEMM: M = Cout, N = the
 // lattice positions, K = Cin x 27.
 //
-// One workgroup (8 wavefronts) computes all output channels of one chain over
+// One workgroup (16 wavefronts) computes all output channels of one chain over
 // a block of `tt` consecutive time slices (tt * Ly * Lx positions, in tiles of
 // 16).  It stages in LDS
 //   * the input slab: slices t0-1 .. t0+tt (periodic in t), every input
@@ -30,9 +30,12 @@
 
 namespace bafq {
 
-constexpr int kConvWaves = 8;         // 512 threads
-constexpr int kConvTiles = 6;         // position tiles of 16 per wavefront (<= 48 per workgroup)
+constexpr int kConvWaves = 16;        // 1024 threads: four wavefronts per SIMD hide the LDS and
+                                      // matrix-core latencies (3.74 vs 4.64 ms per forward pass of the
+                                      // 4 x 32 network over 256 fields of the 9x9 x 80 lattice with 8)
+constexpr int kConvTiles = 3;         // position tiles of 16 per wavefront (<= 48 per workgroup)
 constexpr int kConvMaxPos = kConvWaves * kConvTiles * 16;
+constexpr int kConvStage = 8;         // global loads in flight per thread while staging LDS
 
 struct ConvArgs {
     const float* in;       // [batch][cin][lt][ly][lx]
@@ -84,15 +87,27 @@ DEV void conv_mfma_body(const Ctx& c, const ConvArgs& a) {
     float* slab = (float*)c.smem;
     float* wl = slab + conv_slab_floats(a.tt, cinp, plane);
 
-    // input slab: slices t0-1 .. t0+tt (periodic), zero channels beyond cin
+    // input slab: slices t0-1 .. t0+tt (periodic), zero channels beyond cin;
+    // each thread issues kConvStage loads before it stores any, so that many
+    // global-memory round trips are in flight instead of one
     const int nslab = (a.tt + 2) * cinp * plane;
-    for (int e = c.tid(); e < nslab; e += c.nthreads()) {
-        const int ts = e / (cinp * plane), r = e - ts * cinp * plane;
-        const int ci = r / plane, yx = r - ci * plane;
-        int tg = t0 - 1 + ts;
-        tg = tg < 0 ? tg + lt : tg;
-        tg = tg >= lt ? tg % lt : tg;
-        slab[e] = ci < a.cin ? a.in[(((size_t)b * a.cin + ci) * lt + tg) * plane + yx] : 0.0f;
+    const int nth = c.nthreads();
+    for (int e0 = c.tid(); e0 < nslab; e0 += kConvStage * nth) {
+        float v[kConvStage];
+        for (int u = 0; u < kConvStage; ++u) {
+            const int e = e0 + u * nth;
+            v[u] = 0.0f;
+            if (e < nslab) {
+                const int ts = e / (cinp * plane), r = e - ts * cinp * plane;
+                const int ci = r / plane, yx = r - ci * plane;
+                int tg = t0 - 1 + ts;
+                tg = tg < 0 ? tg + lt : tg;
+                tg = tg >= lt ? tg % lt : tg;
+                if (ci < a.cin) v[u] = a.in[(((size_t)b * a.cin + ci) * lt + tg) * plane + yx];
+            }
+        }
+        for (int u = 0; u < kConvStage; ++u)
+            if (e0 + u * nth < nslab) slab[e0 + u * nth] = v[u];
     }
 
     const int lane = c.lane(), wave = c.wave();
@@ -116,10 +131,19 @@ DEV void conv_mfma_body(const Ctx& c, const ConvArgs& a) {
     for (int g = 0; g < 3; ++g) {
         c.sync();
         const int nw = 9 * cinp * coutp;
-        for (int e = c.tid(); e < nw; e += c.nthreads()) {
-            const int dd = e / (cinp * coutp), r = e - dd * cinp * coutp;
-            const int ci = r / coutp, co = r - ci * coutp;
-            wl[e] = conv_weight(a, co, ci, g * 9 + dd);
+        for (int e0 = c.tid(); e0 < nw; e0 += kConvStage * nth) {
+            float v[kConvStage];
+            for (int u = 0; u < kConvStage; ++u) {
+                const int e = e0 + u * nth;
+                v[u] = 0.0f;
+                if (e < nw) {
+                    const int dd = e / (cinp * coutp), r = e - dd * cinp * coutp;
+                    const int ci = r / coutp, co = r - ci * coutp;
+                    v[u] = conv_weight(a, co, ci, g * 9 + dd);
+                }
+            }
+            for (int u = 0; u < kConvStage; ++u)
+                if (e0 + u * nth < nw) wl[e0 + u * nth] = v[u];
         }
         c.sync();
         for (int dd = 0; dd < 9; ++dd) {

@@ -130,87 +130,145 @@ def info_header(p: Params, nchains: int, spectrum) -> str:
     return "\n".join(rows) + "\n"
 
 
-def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, poles: bool | None = None,
-        log=sys.stdout, autotune: bool = False, tune_nwrap: bool = False):
-    """One reference run in ``run_dir``; returns a dict of run statistics.
-    autotune: run the fastest execution schedule (bafqmc_amd.autotune; the same
-    Markov chain); tune_nwrap: let it also choose N_Ortho under its drift bound
-    (info.txt then reports the N_Ortho used)."""
-    t0 = time.perf_counter()
+@dataclasses.dataclass
+class Case:
+    """One reference run directory inside a batched launch: its parameters,
+    initial fields and the counters info.txt reports."""
+    run_dir: Path
+    pairing: bool
+    nchains: int
+    params: Params
+    phis: list
+    seeds: list
+    poles: bool
+    spec: ModelSpec = None
+    therm: list = None
+    acc: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(2))
+    proposals: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(2))
+    xmax: float = 0.0
+    xmean: float = 0.0
+    nwrap_total: float = 0.0
+
+
+def load_case(run_dir, solver: str = "number_conserving", nchains: int = 1, poles: bool | None = None) -> Case:
+    """Read_input and conf_in of one run directory (calc_basic.f90:48-106,
+    fields.f90:35-113)."""
     run_dir = Path(run_dir)
     pairing = solver in ("pairing", "pa", "nambu")
     p = read_params(run_dir / "paramC_sets.txt", pairing)
-    if poles is None:
-        poles = not pairing          # the pairing executable writes no pole files
     phis, seeds = read_fields(run_dir, p, nchains)
+    return Case(run_dir, pairing, nchains, p, list(phis), list(seeds),
+                (not pairing) if poles is None else poles)
+
+
+def _warm(p: Params) -> int:
+    return p.nwarm if (p.is_warm and p.nwarm > 0) else 0
+
+
+def run_cases(cases: list, lib=None, log=sys.stdout, autotune: bool = False, tune_nwrap: bool = False):
+    """Run several reference run directories as ONE batched solver: every
+    case's chains share the launches (one model per case; lattices, Ltrot,
+    Hamiltonians and the two solvers may differ), and each case's output files,
+    info.txt and confout.txt are written into its own directory exactly as a
+    run of that case alone writes them (a chain evolves in a batch bit for bit
+    as alone).  The cases must share the warm-up length, Nbin and Nsweep (the
+    launches advance every chain together); "Tot CPU time" is the wall time of
+    the whole batch.  Returns one dict of run statistics per case."""
+    if not cases:
+        return []
+    keys = {(_warm(c.params), c.params.nbin, c.params.nsweep) for c in cases}
+    if len(keys) != 1:
+        raise ValueError("cases in one batch need the same Nwarm, Nbin and Nsweep: "
+                         f"{sorted(keys)}")
+    nwarm, nbin, nsweep = keys.pop()
+    t0 = time.perf_counter()
     schedule = None
     if autotune:
         from . import autotune as tuner
         from .native import load_library
-        choice = tuner.tune_model(lib or load_library(), p, chains=min(nchains, 64), sweeps=2,
-                                  tune_nwrap=tune_nwrap)
-        schedule = [choice.schedule]
-        if choice.nwrap != p.nwrap:
-            p = dataclasses.replace(p, nwrap=choice.nwrap)
-        print(f"bafqmc_amd: schedule {choice.schedule}, N_Ortho {p.nwrap} "
-              f"({choice.ms_per_sweep:.2f} ms per sweep of {min(nchains, 64)} chains)", file=log)
-    spec = ModelSpec(p)
-    (run_dir / "info.txt").write_text(info_header(p, nchains, spec.spectrum))
-    s = BatchedSolver([spec], [0] * nchains, lib=lib, schedule=schedule)
+        chain_model = np.repeat(np.arange(len(cases)), [c.nchains for c in cases])
+        choices = tuner.tune(lib or load_library(), [c.params for c in cases], chain_model, sweeps=2,
+                             tune_nwrap=tune_nwrap, max_sample=64)
+        schedule = [ch.schedule for ch in choices]
+        for c, ch in zip(cases, choices):
+            if ch.nwrap != c.params.nwrap:
+                c.params = dataclasses.replace(c.params, nwrap=ch.nwrap)
+            print(f"bafqmc_amd: {c.run_dir.name}: schedule {ch.schedule}, N_Ortho {c.params.nwrap} "
+                  f"({ch.ms_per_sweep:.2f} ms per sweep)", file=log)
+    total = sum(c.nchains for c in cases)
+    for c in cases:
+        c.spec = ModelSpec(c.params)
+        (c.run_dir / "info.txt").write_text(info_header(c.params, c.nchains, c.spec.spectrum))
+    bounds = np.cumsum([0] + [c.nchains for c in cases])
+    chain_model = [k for k, c in enumerate(cases) for _ in range(c.nchains)]
+    s = BatchedSolver([c.spec for c in cases], chain_model, lib=lib, schedule=schedule)
     try:
-        s.set_fields(list(phis), seeds)
-        therm = None
-        if p.is_warm and p.nwarm > 0:
+        s.set_fields([f for c in cases for f in c.phis], [sd for c in cases for sd in c.seeds])
+        if nwarm:
             # Local_sweep_therm per warm-up sweep (local_sweep.f90:64-78)
-            s.warmup(p.nwarm)
+            s.warmup(nwarm)
             therm = s.read_bin()
+            for k, c in enumerate(cases):
+                c.therm = therm[bounds[k]:bounds[k + 1]]
         else:
-            _append(run_dir / "info.txt", " Skipping Bosonic warm-up\n")
+            for c in cases:
+                _append(c.run_dir / "info.txt", " Skipping Bosonic warm-up\n")
         s.prepare()
         s.read_bin()           # clear prepare's counters
-        acc = np.zeros(2)
-        proposals = np.zeros(2)
-        xmax, xmean, nwrap_total = 0.0, 0.0, 0.0
-        for b in range(p.nbin):
-            s.sweep(p.nsweep, measure=True)
+        for b in range(nbin):
+            s.sweep(nsweep, measure=True)
             per = s.read_bin()
-            write_bin(run_dir, per, pairing)
-            for d in per:
-                st = d["_stats"]
-                if st["prop_u1"] > 0:
-                    acc[0] += st["acc_u1"] / st["prop_u1"]
-                if st["prop_u2"] > 0:
-                    acc[1] += st["acc_u2"] / st["prop_u2"]
-                proposals += (st["prop_u1"], st["prop_u2"])
-                xmax = max(xmax, st["xmax"])
-                xmean += st["xmean_sum"]
-                nwrap_total += st["nwrap"]
-            if poles:
-                fields, _ = s.get_fields()
-                zs, scal = [], []
-                for c in range(nchains):
-                    z, sc = pole_diagnostics(s.green(c), fields[c])
-                    zs.append(z)
-                    scal.append(sc)
-                write_poles(run_dir, zs, scal)
+            fields = s.get_fields()[0] if any(c.poles for c in cases) else None
+            for k, c in enumerate(cases):
+                mine = per[bounds[k]:bounds[k + 1]]
+                write_bin(c.run_dir, mine, c.pairing)
+                for d in mine:
+                    st = d["_stats"]
+                    if st["prop_u1"] > 0:
+                        c.acc[0] += st["acc_u1"] / st["prop_u1"]
+                    if st["prop_u2"] > 0:
+                        c.acc[1] += st["acc_u2"] / st["prop_u2"]
+                    c.proposals += (st["prop_u1"], st["prop_u2"])
+                    c.xmax = max(c.xmax, st["xmax"])
+                    c.xmean += st["xmean_sum"]
+                    c.nwrap_total += st["nwrap"]
+                if c.poles:
+                    zs, scal = [], []
+                    for j in range(bounds[k], bounds[k + 1]):
+                        z, sc = pole_diagnostics(s.green(j), fields[j])
+                        zs.append(z)
+                        scal.append(sc)
+                    write_poles(c.run_dir, zs, scal)
         fields, seeds_out = s.get_fields()
     finally:
         s.close()
-    write_fields(run_dir, fields, seeds_out)
     wall = time.perf_counter() - t0
+    out = []
+    for k, c in enumerate(cases):
+        write_fields(c.run_dir, fields[bounds[k]:bounds[k + 1]], seeds_out[bounds[k]:bounds[k + 1]])
+        _append(c.run_dir / "info.txt", "\n".join(_info_tail(c, wall)) + "\n")
+        out.append({"wall_s": wall, "accept": c.acc / max(c.nchains * c.params.nbin, 1), "xmax": c.xmax})
+    print(f"bafqmc_amd: {len(cases)} case(s), {total} chains x {nbin} bins in {wall:.2f} s", file=log)
+    return out
+
+
+def _info_tail(c: Case, wall: float):
+    """Therm_control_print, Local_control_print and the end of main
+    (local_sweep.f90:163-216, main.f90:59-76)."""
+    p = c.params
     tail = []
-    if therm is not None:
+    if c.therm is not None:
         for k, name in ((0, "U1"), (1, "U2")):
-            prop = sum(d["_stats"][f"therm_prop_u{k + 1}"] for d in therm)
+            prop = sum(d["_stats"][f"therm_prop_u{k + 1}"] for d in c.therm)
             if prop > 0:
-                accept = sum(d["_stats"][f"therm_acc_u{k + 1}"] for d in therm) / prop
+                accept = sum(d["_stats"][f"therm_acc_u{k + 1}"] for d in c.therm) / prop
                 tail.append(f" Thermalize {name} Accept Ratio                     : {accept}")
-    denom = max(nchains * p.nbin, 1)
+    denom = max(c.nchains * p.nbin, 1)
     for k, name in ((0, "U1"), (1, "U2")):
         if abs((p.u1, p.u2)[k]) <= 1e-10:
             continue
-        tail.append(f" Accept_{name}_shift                                : {acc[k] / denom}")
-        if pairing:
+        tail.append(f" Accept_{name}_shift                                : {c.acc[k] / denom}")
+        if c.pairing:
             # Local_control_print (src/pairing/src/local_sweep.f90:163-216): the
             # phase of the local ratio det(P)^(-1/2).  In the reduced 2Ns form the
             # ratio is |det P_2Ns|^-1 times positive Gaussian factors, so its
@@ -218,10 +276,18 @@ def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, 
             tail += [f" Phase_{name}_mean_abs_arg_proposed                : {0.0}",
                      f" Phase_{name}_mean_abs_arg_accepted                : {0.0}",
                      f" Phase_{name}_branch_pi_fraction                   : {0.0}",
-                     f" Phase_{name}_local_proposals                      : {float(proposals[k])}"]
-    tail += [f" Max diff Matrix                                : {xmax}",
-             f" Mean diff Matrix                               : {xmean / max(nwrap_total, 1.0)}",
+                     f" Phase_{name}_local_proposals                      : {float(c.proposals[k])}"]
+    tail += [f" Max diff Matrix                                : {c.xmax}",
+             f" Mean diff Matrix                               : {c.xmean / max(c.nwrap_total, 1.0)}",
              f" Tot CPU time                                   : {wall}"]
-    _append(run_dir / "info.txt", "\n".join(tail) + "\n")
-    print(f"bafqmc_amd: {nchains} chains x {p.nbin} bins in {wall:.2f} s", file=log)
-    return {"wall_s": wall, "accept": acc / denom, "xmax": xmax}
+    return tail
+
+
+def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, poles: bool | None = None,
+        log=sys.stdout, autotune: bool = False, tune_nwrap: bool = False):
+    """One reference run in ``run_dir``; returns a dict of run statistics.
+    autotune: run the fastest execution schedule (bafqmc_amd.autotune; the same
+    Markov chain); tune_nwrap: let it also choose N_Ortho under its drift bound
+    (info.txt then reports the N_Ortho used)."""
+    return run_cases([load_case(run_dir, solver, nchains, poles)], lib=lib, log=log, autotune=autotune,
+                     tune_nwrap=tune_nwrap)[0]
