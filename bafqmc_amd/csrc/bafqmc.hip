@@ -153,30 +153,44 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(const float* gout, int co
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
         gin[e] = conv_bwd_data_elem(gout, cout, W, cin, act, mask != 0, s, e);
 }
-// one workgroup per weight; its threads split the batch x volume sum
+// one workgroup per (output, input) channel pair computes the weights of all 27
+// taps: each position's output gradient is read once and multiplied by the 27
+// neighbours of its input (cached), instead of one workgroup re-reading both
+// arrays for every tap; per weight the threads' partial sums and their tree
+// reduction are those of a workgroup per weight
 __global__ void __launch_bounds__(256) k_conv_bwd_weight(const float* gout, int cout, const float* in, int cin,
                                                          float* gW, int batch, NetShape s) {
-    __shared__ float part[256];
-    const int e = blockIdx.x;
-    const int v = s.vol();
-    const int co = e / (cin * 27), r0 = e - co * cin * 27;
-    const int ci = r0 / 27, d = r0 - ci * 27;
-    const int dt = d / 9 - 1, dy = (d / 3) % 3 - 1, dx = d % 3 - 1;
-    float acc = 0.0f;
+    __shared__ float part[27][256];
+    const int co = blockIdx.x / cin, ci = blockIdx.x - co * cin;
+    const int v = s.vol(), plane = s.ly * s.lx;
+    float acc[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) acc[d] = 0.0f;
     for (int q = threadIdx.x; q < batch * v; q += blockDim.x) {
         const int bb = q / v, p = q - bb * v;
-        const int t = p / (s.ly * s.lx), r1 = p - t * s.ly * s.lx;
+        const int t = p / plane, r1 = p - t * plane;
         const int y = r1 / s.lx, x = r1 - y * s.lx;
-        const int src = ((wrap_idx(t + dt, s.lt) * s.ly) + wrap_idx(y + dy, s.ly)) * s.lx + wrap_idx(x + dx, s.lx);
-        acc += gout[((size_t)bb * cout + co) * v + p] * in[((size_t)bb * cin + ci) * v + src];
+        const float g = gout[((size_t)bb * cout + co) * v + p];
+        const float* src = in + ((size_t)bb * cin + ci) * v;
+        int tw[3], yw[3], xw[3];
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            tw[o] = wrap_idx(t + o - 1, s.lt) * plane;
+            yw[o] = wrap_idx(y + o - 1, s.ly) * s.lx;
+            xw[o] = wrap_idx(x + o - 1, s.lx);
+        }
+#pragma unroll
+        for (int d = 0; d < 27; ++d) acc[d] += g * src[tw[d / 9] + yw[(d / 3) % 3] + xw[d % 3]];
     }
-    part[threadIdx.x] = acc;
+#pragma unroll
+    for (int d = 0; d < 27; ++d) part[d][threadIdx.x] = acc[d];
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        if ((int)threadIdx.x < w)
+            for (int d = 0; d < 27; ++d) part[d][threadIdx.x] += part[d][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) gW[e] = part[0];
+    if ((int)threadIdx.x < 27) gW[(size_t)blockIdx.x * 27 + threadIdx.x] = part[threadIdx.x][0];
 }
 __global__ void __launch_bounds__(256) k_conv_bwd_bias(const float* gout, int cout, float* gb, int batch, NetShape s) {
     __shared__ float part[256];
@@ -393,7 +407,7 @@ int rt_conv_bwd_data(const float* gout, int cout, const float* W, float* gin, in
 }
 int rt_conv_bwd_weight(const float* gout, int cout, const float* in, int cin, float* gW, float* gb, int batch,
                        NetShape s) {
-    hipLaunchKernelGGL(k_conv_bwd_weight, dim3(cout * cin * 27), dim3(256), 0, 0, gout, cout, in, cin, gW, batch, s);
+    hipLaunchKernelGGL(k_conv_bwd_weight, dim3(cout * cin), dim3(256), 0, 0, gout, cout, in, cin, gW, batch, s);
     if (check(hipGetLastError(), "k_conv_bwd_weight")) return 1;
     hipLaunchKernelGGL(k_conv_bwd_bias, dim3(cout), dim3(256), 0, 0, gout, cout, gb, batch, s);
     return check(hipGetLastError(), "k_conv_bwd_bias");

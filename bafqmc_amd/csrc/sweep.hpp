@@ -605,38 +605,58 @@ DEV void site_loop_delayed_t(const Ctx& c, const ModelDev& m, const ChainDev& ch
         // different SIMDs; rows r, r + wbase, ... per thread).  The inner index
         // runs in chunks of four whose loads are issued together (rows beyond cols
         // are read from inside the buffer and replaced by zero).
+        // In device memory (n up to 288 with wbase = 128) a thread has up to
+        // three rows; their G0 loads, and the loads of each chunk of the inner
+        // index, are issued together so that a factor step waits for one round
+        // trip per chunk instead of one per row (same arithmetic, same order).
+        constexpr int kRows = kGlb ? 3 : 1;
         const bool urow = tid < wbase;
-        for (int x = urow ? tid : tid - wbase; tid < 2 * wbase && x < n; x += wbase) {   // r or cb
-            const int xa = urow ? x * n + s0 : s0 * n + x;         // G0[r,s0] or G0[s0,cb]
-            const int xb = urow ? x * n + s1 : s1 * n + x;
-            cd a0 = ldc(G, xa), a1 = cd{0.0, 0.0};
-            if (pairing) a1 = ldc(G, xb);
-            // urow: v = U[r,j], p = W[j,s];  else: v = W[j,cb], p = U[s,j]
-            const auto vb = urow ? Ut : W;
-            const auto pb = urow ? W : Ut;
+        // urow: v = U[r,j], p = W[j,s];  else: v = W[j,cb], p = U[s,j]
+        const auto vb = urow ? Ut : W;
+        const auto pb = urow ? W : Ut;
+        for (int xb = urow ? tid : tid - wbase; tid < 2 * wbase && xb < n; xb += kRows * wbase) {   // r or cb
+            cd a0[kRows], a1[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                const int x = xb + r * wbase < n ? xb + r * wbase : xb;      // rows beyond n repeat row xb
+                a0[r] = ldc(G, urow ? x * n + s0 : s0 * n + x);              // G0[r,s0] or G0[s0,cb]
+                a1[r] = cd{0.0, 0.0};
+                if (pairing) a1[r] = ldc(G, urow ? x * n + s1 : s1 * n + x);
+            }
             for (int j0 = 0; j0 < cols; j0 += 4) {
-                cd v[4], p0[4], p1[4];
+                cd v[kRows][4], p0[4], p1[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int j = j0 + u < dk ? j0 + u : dk - 1;
-                    v[u] = ldc(vb, j * n + x);
                     p0[u] = ldc(pb, j * n + s0);
                     p1[u] = cd{0.0, 0.0};
                     if (pairing) p1[u] = ldc(pb, j * n + s1);
+#pragma unroll
+                    for (int r = 0; r < kRows; ++r) {
+                        const int x = xb + r * wbase < n ? xb + r * wbase : xb;
+                        v[r][u] = ldc(vb, j * n + x);
+                    }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const cd vv = csel(j0 + u < cols, v[u], cd{0.0, 0.0});
-                    a0 = a0 - vv * p0[u];
-                    if (pairing) a1 = a1 - vv * p1[u];
-                }
+                for (int r = 0; r < kRows; ++r)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const cd vv = csel(j0 + u < cols, v[r][u], cd{0.0, 0.0});
+                        a0[r] = a0[r] - vv * p0[u];
+                        if (pairing) a1[r] = a1[r] - vv * p1[u];
+                    }
             }
-            if (urow) {
-                stc(Ut, cols * n + x, a0 * m00 + a1 * m10);
-                if (pairing) stc(Ut, (cols + 1) * n + x, a0 * m01 + a1 * m11);
-            } else {
-                stc(W, cols * n + x, cd{(x == s0 ? 1.0 : 0.0) - a0.r, -a0.i});
-                if (pairing) stc(W, (cols + 1) * n + x, cd{(x == s1 ? 1.0 : 0.0) - a1.r, -a1.i});
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                const int x = xb + r * wbase;
+                if (x >= n) break;
+                if (urow) {
+                    stc(Ut, cols * n + x, a0[r] * m00 + a1[r] * m10);
+                    if (pairing) stc(Ut, (cols + 1) * n + x, a0[r] * m01 + a1[r] * m11);
+                } else {
+                    stc(W, cols * n + x, cd{(x == s0 ? 1.0 : 0.0) - a0[r].r, -a0[r].i});
+                    if (pairing) stc(W, (cols + 1) * n + x, cd{(x == s1 ? 1.0 : 0.0) - a1[r].r, -a1[r].i});
+                }
             }
         }
         cols += ksec;
