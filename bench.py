@@ -66,7 +66,11 @@ CONFIGS = {
     "c5": dict(name="pairing 12x12 beta=10 Ltrot=100 U1=1 U2=-0.5 Delta=0.3 mu=-5, 128 chains, "
                     "4-layer 32-channel FP32 proposal network",
                p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12, ltrot=100, beta=10.0,
-                      nwrap=10, shift_loc=1.0), chains=128, net=dict(layers=4, hidden=32)),
+                      nwrap=10, shift_loc=1.0), chains=128, net=dict(layers=4, hidden=32),
+               # a strong determinant force: several integrator steps, a longer fit, and measurements at
+               # the stabilisation points (a single slice's estimate varies 10x more per step than a sweep's)
+               learned=dict(hmc_steps=[4, 8, 16], train_s=60.0, epochs=60, measure="checkpoints"),
+               slug="pairing12x12_128chains"),
     "c2": dict(name="pairing 6x6 beta=8 Ltrot=80 U1=1 U2=-0.5 Delta=0.5 mu=-5, 512 chains",
                p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=80, beta=8.0,
                       nwrap=10, shift_loc=1.0), chains=512, net=dict(layers=4, hidden=32), slug="pairing6x6_512chains"),
@@ -550,15 +554,17 @@ def main():
                     help="keep the workload's Nwrap (default: the autotuner also chooses Nwrap among the "
                          "divisors of Ltrot near it, admitting only intervals whose measured drift stays "
                          "below 1e-7)")
-    ap.add_argument("--hmc-steps", type=lambda v: [int(x) for x in v.split(",")], default=[1, 2],
-                    help="integrator step counts tried (each a quarter rotation in total)")
-    ap.add_argument("--measure", default="current", choices=["all", "checkpoints", "current"],
+    ap.add_argument("--hmc-steps", type=lambda v: [int(x) for x in v.split(",")], default=None,
+                    help="integrator step counts tried, a quarter rotation in total (default 1,2; the 12x12 "
+                         "pairing workload 4,8,16)")
+    ap.add_argument("--measure", default=None, choices=["all", "checkpoints", "current"],
                     help="learned step's measurement: every slice, the stabilisation points only, or the "
                          "rebuilt G(beta) of the move once")
     ap.add_argument("--layers", type=int, default=0, help="convolution layers (default: the workload's)")
     ap.add_argument("--hidden", type=int, default=0, help="hidden channels (default: the workload's)")
-    ap.add_argument("--epochs", type=int, default=20)
-    ap.add_argument("--train-s", type=float, default=20.0, help="training time budget in seconds")
+    ap.add_argument("--epochs", type=int, default=None, help="training epochs (default 20; 12x12 pairing 60)")
+    ap.add_argument("--train-s", type=float, default=None,
+                    help="training time budget in seconds (default 20; 12x12 pairing 60)")
     ap.add_argument("--dump", default="", help="learned mode: write both phases' per-chain S_SF(K) series")
     ap.add_argument("--mode", default="learned", choices=["plain", "learned"])
     ap.add_argument("--copy-probe", action="store_true",
@@ -579,6 +585,10 @@ def main():
         return
     from bafqmc_amd.model import ModelSpec
     cfg = CONFIGS[args.config]
+    learned_defaults = dict(dict(hmc_steps=[1, 2], train_s=20.0, epochs=20, measure="current"), **cfg.get("learned", {}))
+    for k, v in learned_defaults.items():
+        if getattr(args, k) is None:
+            setattr(args, k, v)
     nchains = args.chains or cfg["chains"]
     if "models" in cfg:
         run_uneven(args, cfg, nchains)
