@@ -2,12 +2,12 @@
 """Effective S_SF(K) samples/s of the batched BAFQMC sampler on one MI355X,
 learned-proposal form vs the plain batched sweep at equal error bars.
 
-Workload (default, `--config c4`): 256 independent number-conserving chains
+Workload (default, `--config nc9x9`): 256 independent number-conserving chains
 on a 9x9 triangular lattice (N = 81), t = 1, U1 = 1, U2 = 0, mu = -5,
 beta = 8, dtau = 0.1 (Ltrot = 80), Nwrap = 10, shiftLoc = 1.5, fields i.i.d.
 N(0,1) from per-chain seeds; proposal network: 4 periodic 3x3x3 convolution
 layers, 32 hidden channels, FP32, He-normal initialisation from a seed.
-`--config c5` runs the 12x12 pairing workload (N = 288, Ltrot = 100,
+`--config pairing12x12` runs the 12x12 pairing workload (N = 288, Ltrot = 100,
 128 chains) the same way.
 
 * plain step: one batched sweep (sweep_L + sweep_R of every chain, 2 Ltrot
@@ -59,11 +59,11 @@ UNSTABLE_DRIFT = 5.5e-5
 PEAK_HBM_GBS = 8000.0
 
 CONFIGS = {
-    "c4": dict(name="number-conserving 9x9 beta=8 Ltrot=80 U1=1 U2=0 mu=-5, 256 chains, "
+    "nc9x9": dict(name="number-conserving 9x9 beta=8 Ltrot=80 U1=1 U2=0 mu=-5, 256 chains, "
                     "4-layer 32-channel FP32 proposal network",
                p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=80, beta=8.0, nwrap=10,
                       shift_loc=1.5), chains=256, net=dict(layers=4, hidden=32), slug="nc9x9_256chains"),
-    "c5": dict(name="pairing 12x12 beta=10 Ltrot=100 U1=1 U2=-0.5 Delta=0.3 mu=-5, 128 chains, "
+    "pairing12x12": dict(name="pairing 12x12 beta=10 Ltrot=100 U1=1 U2=-0.5 Delta=0.3 mu=-5, 128 chains, "
                     "4-layer 32-channel FP32 proposal network",
                p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12, ltrot=100, beta=10.0,
                       nwrap=10, shift_loc=1.0), chains=128, net=dict(layers=4, hidden=32),
@@ -71,7 +71,7 @@ CONFIGS = {
                # the stabilisation points (a single slice's estimate varies 10x more per step than a sweep's)
                learned=dict(hmc_steps=[4, 8, 16], train_s=60.0, epochs=60, measure="checkpoints"),
                slug="pairing12x12_128chains"),
-    "c2": dict(name="pairing 6x6 beta=8 Ltrot=80 U1=1 U2=-0.5 Delta=0.5 mu=-5, 512 chains",
+    "pairing6x6": dict(name="pairing 6x6 beta=8 Ltrot=80 U1=1 U2=-0.5 Delta=0.5 mu=-5, 512 chains",
                p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=80, beta=8.0,
                       nwrap=10, shift_loc=1.0), chains=512, net=dict(layers=4, hidden=32), slug="pairing6x6_512chains"),
     "uneven": dict(name="uneven batch: number-conserving 3x3/6x6/9x9/12x12 x beta 4/8 x U2 0/-0.5, "
@@ -80,10 +80,10 @@ CONFIGS = {
                                 beta=b, nwrap=10, shift_loc=1.5)
                            for L in (3, 6, 9, 12) for b in (4.0, 8.0) for u2 in (0.0, -0.5)],
                    chains=256),
-    "nc12": dict(name="number-conserving 12x12 beta=8 Ltrot=80 U1=1 U2=-0.5 mu=-5, 96 chains",
+    "nc12x12": dict(name="number-conserving 12x12 beta=8 Ltrot=80 U1=1 U2=-0.5 mu=-5, 96 chains",
                  p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=12, nly=12, ltrot=80, beta=8.0, nwrap=10,
                         shift_loc=1.5), chains=96, net=dict(layers=4, hidden=32)),
-    "c1": dict(name="number-conserving 3x3 beta=4 Ltrot=40 U1=1 U2=0 mu=-5, 512 chains",
+    "nc3x3": dict(name="number-conserving 3x3 beta=4 Ltrot=40 U1=1 U2=0 mu=-5, 512 chains",
                p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=3, nly=3, ltrot=40, beta=4.0, nwrap=10,
                       shift_loc=1.5), chains=512, net=dict(layers=3, hidden=8)),
 }
@@ -542,7 +542,7 @@ def main():
     ap.add_argument("--stat-steps", type=int, default=400,
                     help="steps of each form for the statistics (tau_int, error bars); the timed steps are "
                          "the first --steps of them")
-    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="nc9x9", choices=sorted(CONFIGS))
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--mu", type=float, default=None, help="chemical potential (default: the workload's)")
     ap.add_argument("--nwrap", type=int, default=None,

@@ -51,7 +51,7 @@ CASES = {
                                    nwrap=6, nbin=3, shift_loc=0.0), seeds=[1357911], fixed=True),
     "pairing_fixed_3x3_w2": dict(p=dict(pairing=True, u1=0.7, u2=-0.1, mu=-5.0, rdelta=0.3, nlx=3, nly=3, ltrot=6,
                                         beta=0.6, nwrap=2, nbin=3, shift_loc=0.0), seeds=[1357911], fixed=True),
-    # config-1 Hamiltonian, two chains generated from seeds.txt (one per MPI rank)
+    # the 3x3 beta = 4 benchmark Hamiltonian, two chains generated from seeds.txt (one per MPI rank)
     "nc_traj_3x3_c1": dict(p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=3, nly=3, ltrot=40, beta=4.0,
                                   nwrap=10, nbin=5, shift_loc=1.5, ini_type=2, ini_ampl=1.0),
                            seeds=[13579113, 24681357]),
@@ -60,7 +60,7 @@ CASES = {
     "pairing_traj_3x3": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=3, nly=3, ltrot=20,
                                     beta=2.0, nwrap=10, nbin=4, shift_loc=1.0, ini_type=2, ini_ampl=1.0),
                              seeds=[11223344, 55667788]),
-    # config-2 chain: pairing 6x6, beta=8, Ltrot=80 (one chain, one bin)
+    # a pairing 6x6 chain, beta=8, Ltrot=80 (one chain, one bin)
     "pairing_traj_6x6_c2": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=80,
                                        beta=8.0, nwrap=10, nbin=1, shift_loc=1.0, ini_type=2, ini_ampl=1.0),
                                 seeds=[13579113]),
@@ -71,7 +71,7 @@ CASES = {
     "pairing_warm_3x3": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=3, nly=3, ltrot=20,
                                     beta=2.0, nwrap=10, nbin=2, shift_loc=1.0, ini_type=2, ini_ampl=1.0, nwarm=3),
                              seeds=[424242]),
-    # config-4 lattice: NC 9x9, beta=8, Ltrot=80 (one chain, one bin)
+    # the learned workload's lattice: NC 9x9, beta=8, Ltrot=80 (one chain, one bin)
     "nc_traj_9x9_c4": dict(p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=80, beta=8.0,
                                   nwrap=10, nbin=1, shift_loc=1.5, ini_type=2, ini_ampl=1.0), seeds=[24681357]),
     # thermalisation lattice smaller than the lattice: fields generated on 3x3 x 20
@@ -82,7 +82,7 @@ CASES = {
     "pairing_transfer_6x6": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=6, nly=6, ltrot=40,
                                         beta=4.0, nlx_therm=3, nly_therm=3, ltrot_therm=20, nwrap=10, nbin=2,
                                         shift_loc=1.0, ini_type=2, ini_ampl=1.0), seeds=[97531]),
-    # 12x12 lattices on the device-memory path: NC (N = 144) and the config-5 pairing
+    # 12x12 lattices on the device-memory path: NC (N = 144) and the 12x12 pairing
     # chain (reference N = 576, reduced N = 288), one chain, one bin
     "nc_traj_12x12": dict(p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=12, nly=12, ltrot=80, beta=8.0,
                                  nwrap=10, nbin=1, shift_loc=1.5, ini_type=2, ini_ampl=1.0), seeds=[13579113]),
