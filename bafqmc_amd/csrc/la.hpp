@@ -909,18 +909,21 @@ DEV void solve_cols_t(const Ctx& c, cd* Y, const cd* Xs, bool conj_t, LDS_AS con
     c.sync();
 }
 
-// One noinline instance per register-slot count R (4 R >= n), so that each
-// gets its own register allocation.
+// One instance per register-slot count R (4 R >= n), inlined into the
+// stabilisation routines that call them: as out-of-line calls each saved and
+// restored the callee-saved registers it clobbers (some 400 scratch stores
+// per lane on entry), which cost 10 % of a G rebuild and 6 % of a sweep at
+// N = 81 (same results bit for bit).
 template <int R>
-DEVFN void qrp_cols_r(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
+DEV void qrp_cols_r(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
     qrp_cols_t<R>(c, as_lds(A), n, s, D);
 }
 template <int R>
-DEVFN void form_q_cols_r(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n, bool conj_t) {
+DEV void form_q_cols_r(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n, bool conj_t) {
     form_q_cols_t<R>(c, Q, as_lds(A), tau, n, conj_t);
 }
 template <int R>
-DEVFN void solve_cols_r(const Ctx& c, cd* Y, const cd* Xs, bool conj_t, const cd* A, const cd* tau,
+DEV void solve_cols_r(const Ctx& c, cd* Y, const cd* Xs, bool conj_t, const cd* A, const cd* tau,
                         const double* D, const int* perm, int n) {
     solve_cols_t<R>(c, Y, Xs, conj_t, as_lds(A), tau, D, perm, n);
 }
