@@ -35,7 +35,7 @@ constexpr int kConvWaves = 16;        // 1024 threads: four wavefronts per SIMD 
                                       // 4 x 32 network over 256 fields of the 9x9 x 80 lattice with 8)
 constexpr int kConvTiles = 3;         // position tiles of 16 per wavefront (<= 48 per workgroup)
 constexpr int kConvMaxPos = kConvWaves * kConvTiles * 16;
-constexpr int kConvStage = 8;         // global loads in flight per thread while staging LDS
+constexpr int kConvStage = 16;        // global loads in flight per thread while staging LDS
 
 struct ConvArgs {
     const float* in;       // [batch][cin][lt][ly][lx]
