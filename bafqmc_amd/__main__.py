@@ -32,6 +32,10 @@ def main(argv=None):
                    help="with --autotune, also choose N_Ortho among the divisors of Ltrot near the input's, "
                         "admitting only intervals whose measured drift stays below 1e-7 (results change at "
                         "that level)")
+    r.add_argument("--tau-conventions", choices=["bosonic", "reference"], default="bosonic",
+                   help="time-displaced pass (is_tau): the bosonic block form, or the reference's code as written "
+                        "(fermionic block sign, U2 propagated with the U1 field), whose info.txt drift lines it "
+                        "reproduces")
     cp = sub.add_parser("campaign", help="run the cases of a reference manifest as batched launches")
     cp.add_argument("--manifest", required=True, help="the reference's manifest.json (cases: id, input_dir)")
     cp.add_argument("--output", required=True, help="run directories <output>/<case id>/dqmc are created here")
@@ -47,8 +51,9 @@ def main(argv=None):
                      tune_nwrap=a.tune_nwrap)
         return 0
     from .driver import run
+    from .solver import TAU_BOSONIC, TAU_REFERENCE
     run(a.directory, a.solver, a.chains, poles=False if a.no_poles else None, autotune=a.autotune,
-        tune_nwrap=a.tune_nwrap)
+        tune_nwrap=a.tune_nwrap, tau_mode=TAU_REFERENCE if a.tau_conventions == "reference" else TAU_BOSONIC)
     return 0
 
 

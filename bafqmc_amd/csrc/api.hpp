@@ -19,7 +19,7 @@
 
 namespace bafq {
 
-enum KernelId { K_PREPARE = 0, K_SWEEP = 1, K_WARMUP = 2, K_GRAD = 3, K_MEASURE = 4 };
+enum KernelId { K_PREPARE = 0, K_SWEEP = 1, K_WARMUP = 2, K_GRAD = 3, K_MEASURE = 4, K_SWEEP_TAU = 5 };
 enum PathId { P_GLOBAL = 0, P_LDS = 1 };
 
 // back-end hooks (0 on success)
@@ -136,6 +136,8 @@ struct bafq_handle {
     int* gm_accept = nullptr;
     size_t chain_block_bytes = 0;             // matrix block of chain 0 (with one model: of every chain)
     double last_ms = 0.0;
+    bafq::cd* tau_ws = nullptr;               // time-displaced workspaces (+ records), on first use
+    bafq::cd* tau_rec = nullptr;
 };
 
 namespace bafq {
@@ -395,6 +397,8 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
         ch.wa = (cd*)(base + o.wa); ch.w1 = (cd*)(base + o.w1); ch.w2 = (cd*)(base + o.w2); ch.w3 = (cd*)(base + o.w3);
         ch.obs = h->d_obs + (size_t)h->obs_stride * c;
         ch.stats = h->d_stats + (size_t)S_LEN * c;
+        ch.tau = nullptr;
+        ch.tau_rec = nullptr;
         ids[c] = c;
         h->chain_n.push_back(m.n);
         h->chain_ns.push_back(m.ns);
@@ -438,6 +442,7 @@ void bafq_destroy(bafq_handle* h) {
     if (h->gm_fcur) bafq::rt_free(h->gm_fcur);
     if (h->gm_state) bafq::rt_free(h->gm_state);
     if (h->d_grad) bafq::rt_free(h->d_grad);
+    if (h->tau_ws) bafq::rt_free(h->tau_ws);
     bafq::rt_release_launches(h->launches);
     if (h->dev) bafq::rt_free(h->dev);
     delete h;
@@ -481,13 +486,16 @@ int bafq_get_fields(bafq_handle* h, double* phi, int32_t* seeds) {
     return 0;
 }
 
-static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure) {
+static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure, int tau = 0,
+                      int tau_record = 0) {
     using namespace bafq;
     LaunchArgs a;
     a.models = h->d_models;
     a.chains = h->d_chains;
     a.nsweep = nsweep;
     a.measure = measure;
+    a.tau = tau;
+    a.tau_record = tau_record;
     a.queue = nullptr;
     a.grad = h->d_grad;
     a.phi0 = h->d_phi;
@@ -508,7 +516,7 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure)
             l.a.ids = h->d_path_ids[p];
             l.a.count = h->path_count[p];
             l.a.queue = h->d_queue + p;
-            l.grid = h->grid[(k == K_GRAD || k == K_MEASURE) ? K_SWEEP : k][p];
+            l.grid = h->grid[(k == K_GRAD || k == K_MEASURE || k == K_SWEEP_TAU) ? K_SWEEP : k][p];
             l.block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
             l.smem = h->max_lds[p];
         }
@@ -536,6 +544,55 @@ int bafq_sweep(bafq_handle* h, int n_sweeps, int measure) {
     h->gm_fvalid = 0;
     if (n_sweeps <= 0) return 0;
     return launch_all(h, bafq::K_SWEEP, n_sweeps, measure);
+}
+
+int bafq_sweep_tau(bafq_handle* h, int n_sweeps, int measure, int mode, int record) {
+    using namespace bafq;
+    if (!h) return 1;
+    if (mode != TAU_BOSONIC && mode != TAU_REFERENCE)
+        return fail(h, "time-displaced mode must be 1 (bosonic) or 2 (the reference's conventions)");
+    h->gm_fvalid = 0;
+    if (n_sweeps <= 0) return 0;
+    const size_t nc = h->chains.size();
+    const bool rec = record != 0;
+    if (!h->tau_ws || (rec && !h->tau_rec)) {
+        // one allocation per handle: the workspaces, then (if asked) the records
+        std::vector<size_t> ws(nc), rs(nc);
+        size_t total = 0, rtotal = 0;
+        for (size_t c = 0; c < nc; ++c) {
+            const int n = h->chain_n[c], lt = h->chain_lt[c];
+            ws[c] = tau_ws_elems(n, lt, false);
+            rs[c] = rec ? tau_ws_elems(n, lt, true) - ws[c] : 0;
+            total += ws[c];
+            rtotal += rs[c];
+        }
+        if (h->tau_ws) rt_free(h->tau_ws);
+        h->tau_ws = nullptr;
+        h->tau_rec = nullptr;
+        if (rt_malloc((void**)&h->tau_ws, sizeof(cd) * (total + rtotal)))
+            return fail(h, std::string("time-displaced workspace allocation failed: ") + rt_error());
+        if (rec) h->tau_rec = h->tau_ws + total;
+        size_t o = 0, ro = 0;
+        for (size_t c = 0; c < nc; ++c) {
+            h->chains[c].tau = h->tau_ws + o;
+            h->chains[c].tau_rec = rec ? h->tau_rec + ro : nullptr;
+            o += ws[c];
+            ro += rs[c];
+        }
+        if (rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * nc)) return fail(h, rt_error());
+    }
+    return launch_all(h, K_SWEEP_TAU, n_sweeps, measure, mode, rec ? 1 : 0);
+}
+
+int bafq_get_green_tau(bafq_handle* h, int chain, double* out) {
+    using namespace bafq;
+    if (!h || !out || chain < 0 || chain >= (int)h->chains.size()) return 1;
+    if (!h->chains[chain].tau_rec)
+        return fail(h, "no recorded time-displaced pass (bafq_sweep_tau with record = 1 first)");
+    if (rt_sync()) return fail(h, rt_error());
+    const size_t n = h->chain_n[chain], lt = h->chain_lt[chain];
+    if (rt_d2h(out, h->chains[chain].tau_rec, sizeof(cd) * lt * 4 * n * n)) return fail(h, rt_error());
+    return 0;
 }
 
 int bafq_measure_pass(bafq_handle* h, int n_passes, int mode) {
@@ -566,7 +623,7 @@ int bafq_read_bin(bafq_handle* h, double* obs, double* stats) {
     if (rt_d2h(st.data(), h->d_stats, sizeof(double) * S_LEN * nc)) return fail(h, rt_error());
     for (size_t c = 0; c < nc; ++c)
         for (int k = 0; k < S_LEN; ++k)
-            if (k != S_LOGDET && k != S_XMAX) st[c * S_LEN + k] = 0.0;
+            if (k != S_LOGDET && k != S_XMAX && !(k >= S_TAU_XMAX && k < S_TAU_XMAX + 3)) st[c * S_LEN + k] = 0.0;
     if (stats) {
         const double hz = rt_clock_hz();
         for (size_t c = 0; c < nc; ++c)

@@ -65,6 +65,18 @@ __global__ void __launch_bounds__(512) k_measure_lds(LaunchArgs a) {
     body_measure<true>(c, a, (int*)smem);
 }
 
+__global__ void __launch_bounds__(512) k_sweep_tau(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem + kSlotBytes};
+    body_sweep<false, true>(c, a, (int*)smem);
+}
+
+__global__ void __launch_bounds__(512) k_sweep_tau_lds(LaunchArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    Ctx c{smem + kSlotBytes};
+    body_sweep<true, true>(c, a, (int*)smem);
+}
+
 __global__ void __launch_bounds__(64) k_reset_queue(int* q) {
     if (threadIdx.x == 0) *q = 0;
 }
@@ -547,6 +559,7 @@ static const void* kernel_of(KernelId k, PathId path) {
     if (k == K_PREPARE) return path == P_LDS ? (const void*)k_prepare_lds : (const void*)k_prepare;
     if (k == K_GRAD) return path == P_LDS ? (const void*)k_grad_lds : (const void*)k_grad;
     if (k == K_MEASURE) return path == P_LDS ? (const void*)k_measure_lds : (const void*)k_measure;
+    if (k == K_SWEEP_TAU) return path == P_LDS ? (const void*)k_sweep_tau_lds : (const void*)k_sweep_tau;
     return path == P_LDS ? (const void*)k_sweep_lds : (const void*)k_sweep;
 }
 
@@ -564,13 +577,13 @@ int rt_resident_groups(KernelId k, PathId path, int block, size_t smem) {
 }
 
 // Launch state of one handle: its timing events and instantiated graphs, keyed
-// by (kernel, sweeps, measure); each graph holds, per path, a queue reset
+// by (kernel, sweeps, measure, time-displaced mode and recording); each graph holds, per path, a queue reset
 // followed by the persistent kernel, the paths as independent branches that
 // run concurrently.
 struct GraphCache {
     hipEvent_t ev[2] = {nullptr, nullptr};
     struct Entry {
-        int kind, nsweep, measure;
+        int kind, nsweep, measure, tau, tau_record;
         std::vector<LaunchArgs> args;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -661,9 +674,12 @@ static int build_graph(GraphCache::Entry& e, KernelId k, const PathLaunch* pl, i
 int rt_launch_paths(void** cache, KernelId k, const PathLaunch* pl, int n, double* ms) {
     GraphCache* gc = launch_state(cache);
     const int nsweep = n ? pl[0].a.nsweep : 0, measure = n ? pl[0].a.measure : 0;
+    const int tau = n ? pl[0].a.tau : 0, tau_record = n ? pl[0].a.tau_record : 0;
     GraphCache::Entry* hit = nullptr;
     for (GraphCache::Entry& e : gc->entries)
-        if (e.kind == (int)k && e.nsweep == nsweep && e.measure == measure) hit = &e;
+        if (e.kind == (int)k && e.nsweep == nsweep && e.measure == measure && e.tau == tau &&
+            e.tau_record == tau_record)
+            hit = &e;
     if (!hit) {
         if (gc->entries.size() >= 8) gc->clear();
         gc->entries.emplace_back();
@@ -671,6 +687,8 @@ int rt_launch_paths(void** cache, KernelId k, const PathLaunch* pl, int n, doubl
         hit->kind = (int)k;
         hit->nsweep = nsweep;
         hit->measure = measure;
+        hit->tau = tau;
+        hit->tau_record = tau_record;
         if (build_graph(*hit, k, pl, n)) {
             gc->clear();
             return 1;

@@ -277,6 +277,9 @@ int rt_launch_paths(void**, KernelId k, const PathLaunch* pl, int n, double* ms)
             } else if (k == K_MEASURE) {
                 if (l.path == P_LDS) body_measure<true>(c, l.a, slot);
                 else body_measure<false>(c, l.a, slot);
+            } else if (k == K_SWEEP_TAU) {
+                if (l.path == P_LDS) body_sweep<true, true>(c, l.a, slot);
+                else body_sweep<false, true>(c, l.a, slot);
             } else {
                 if (l.path == P_LDS) body_sweep<true>(c, l.a, slot);
                 else body_sweep<false>(c, l.a, slot);

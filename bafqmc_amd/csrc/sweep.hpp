@@ -42,7 +42,9 @@ enum StatIndex {
     // device-clock ticks spent per phase (thread 0 of the chain's workgroup)
     S_T_SITE, S_T_KIN, S_T_MEAS, S_T_WRAP, S_T_HS,
     // parts of S_T_WRAP: B-products of the U factor, the two pivoted QRs, the rest of stab_green
-    S_T_WPROD, S_T_WQR, S_T_WGREEN, S_LEN = 20
+    S_T_WPROD, S_T_WQR, S_T_WGREEN,
+    // Wrap_tau drifts of G(0,tau), G(tau,0), G(tau,tau): running maxima, sums, count
+    S_TAU_XMAX = 20, S_TAU_XSUM = 23, S_TAU_NWRAP = 26, S_LEN = 27
 };
 constexpr int kNumTimers = 8;
 
@@ -76,6 +78,8 @@ struct ChainDev {
     cd *wa, *w1, *w2, *w3;
     double* obs;
     double* stats;
+    cd* tau;                           // time-displaced workspace (null until requested)
+    cd* tau_rec;                       // Ltrot x 4 time-displaced functions of the last pass, or null
 };
 
 struct LaunchArgs {
@@ -88,6 +92,8 @@ struct LaunchArgs {
     int* queue;          // work counter of the launch (zeroed before it), see next_item
     double* grad;        // K_GRAD: d log W / d phi, laid out like the fields starting at phi0
     const double* phi0;
+    int tau;             // K_SWEEP_TAU: convention of the time-displaced pass (TauMode)
+    int tau_record;      // K_SWEEP_TAU: keep the last pass's functions at every slice
 };
 
 // Bytes at the start of a workgroup's dynamic LDS that hold the work-queue
@@ -229,15 +235,16 @@ DEV void grad_extract(const Ctx& c, const ModelDev& m, const double* phi_t, GP G
     c.sync();
 }
 
-// f[r] = exp(s_r alpha_nf phi_r), finv = 1 / f, for all rows (LDS).
-DEV void hs_factors(const Ctx& c, const ModelDev& m, const Lds& L, const double* phi_t, int nf) {
+// f[r] = exp(s_r alpha_nf phi_r), finv = 1 / f, for all rows (LDS); phi of
+// field channel fld (0: the channel's own, nf).
+DEV void hs_factors(const Ctx& c, const ModelDev& m, const Lds& L, const double* phi_t, int nf, int fld = 0) {
     const int ns = m.ns, k = nsec(m);
     auto f = as_lds(L.f);
     auto finv = as_lds(L.finv);
     for (int r = c.tid(); r < ns * k; r += c.nthreads()) {
         const int sec = r / ns, i = r - sec * ns;
         const double s = sec_sign(m, nf, sec);
-        const double ph = phi_t[i * 2 + (nf - 1)];
+        const double ph = phi_t[i * 2 + ((fld ? fld : nf) - 1)];
         const cd a = m.alpha[nf - 1];
         stc(f, r, cexp(cd{s * a.r * ph, s * a.i * ph}));
         stc(finv, r, cexp(cd{-s * a.r * ph, -s * a.i * ph}));
@@ -1436,6 +1443,12 @@ DEV void sweep_right_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
     }
 }
 
+}  // namespace bafq
+
+#include "dyn.hpp"
+
+namespace bafq {
+
 // ---------------------------------------------------------------- kernel bodies
 // Persistent workgroups take chains from a queue: lane 0 of the workgroup
 // increments the launch's counter and shares the index through LDS.  The
@@ -1460,7 +1473,10 @@ DEV void body_prepare(const Ctx& c, const LaunchArgs& a, int* slot) {
     }
 }
 
-template <bool kLds>
+// kTau: Local_sweep with toggle = .true. (local_sweep.f90:119-123): the
+// time-displaced pass between sweep_L and sweep_R (a kernel of its own, so the
+// plain sweep's code is not affected by it).
+template <bool kLds, bool kTau = false>
 DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
@@ -1469,13 +1485,14 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
         int seed = *ch.seed;
         if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
         for (int s = 0; s < a.nsweep; ++s) {
-            if (kLds) {
-                sweep_left_lds(c, m, ch, L, seed, a.measure != 0);
-                sweep_right_lds(c, m, ch, L, seed, a.measure != 0);
-            } else {
-                sweep_left_global(c, m, ch, L, seed, a.measure != 0);
-                sweep_right_global(c, m, ch, L, seed, a.measure != 0);
+            if (kLds) sweep_left_lds(c, m, ch, L, seed, a.measure != 0);
+            else sweep_left_global(c, m, ch, L, seed, a.measure != 0);
+            if (kTau && ch.tau) {
+                if (kLds) global_from_lds(c, ch.g, L.mat, m.n);    // G(0) for the pass; LDS keeps it
+                tau_pass(c, m, ch, L, a.tau, a.tau_record != 0 && s == a.nsweep - 1);
             }
+            if (kLds) sweep_right_lds(c, m, ch, L, seed, a.measure != 0);
+            else sweep_right_global(c, m, ch, L, seed, a.measure != 0);
         }
         if (kLds) global_from_lds(c, ch.g, L.mat, m.n);
         if (c.tid() == 0) *ch.seed = seed;

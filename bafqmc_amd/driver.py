@@ -13,8 +13,13 @@ with the R MPI ranks replaced by R chains of one batched GPU solver:
 * number-conserving pole diagnostics per chain and bin (pole_diagnostics.f90:91-187);
 * at the end ``info.txt`` and ``confout.txt`` (main.f90:59-76, fields.f90:236-278).
 
-The time-displaced path (``is_tau``) writes no estimators in the reference
-(src/number_conserving/src/obser_tau.f90:39-54) and is not run here.
+With ``is_tau`` each bin after ``Nthermal`` runs the time-displaced pass
+between sweep_L and sweep_R (main.f90:49-55, local_sweep.f90:119-123); its
+estimators are empty in the reference (src/number_conserving/src/obser_tau.f90:39-54),
+so its output is info.txt's six Wrap_tau drift lines (dynamics.f90:60-82).
+``tau_mode``: the bosonic block form (default) or the reference's code as
+written (``bafqmc_amd.solver.TAU_REFERENCE``, whose drift lines match the
+reference's).
 """
 from __future__ import annotations
 
@@ -28,7 +33,7 @@ from pathlib import Path
 import numpy as np
 
 from .model import ModelSpec, Params, read_params, read_fields, write_fields
-from .solver import BatchedSolver, NC_OUTPUTS, PAIRING_OUTPUTS
+from .solver import BatchedSolver, NC_OUTPUTS, PAIRING_OUTPUTS, TAU_BOSONIC
 
 TINY = np.finfo(float).tiny
 
@@ -148,6 +153,9 @@ class Case:
     xmax: float = 0.0
     xmean: float = 0.0
     nwrap_total: float = 0.0
+    tau_xmax: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+    tau_xsum: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+    tau_ran: bool = False
 
 
 def load_case(run_dir, solver: str = "number_conserving", nchains: int = 1, poles: bool | None = None) -> Case:
@@ -165,7 +173,8 @@ def _warm(p: Params) -> int:
     return p.nwarm if (p.is_warm and p.nwarm > 0) else 0
 
 
-def run_cases(cases: list, lib=None, log=sys.stdout, autotune: bool = False, tune_nwrap: bool = False):
+def run_cases(cases: list, lib=None, log=sys.stdout, autotune: bool = False, tune_nwrap: bool = False,
+              tau_mode: int = TAU_BOSONIC):
     """Run several reference run directories as ONE batched solver: every
     case's chains share the launches (one model per case; lattices, Ltrot,
     Hamiltonians and the two solvers may differ), and each case's output files,
@@ -215,8 +224,18 @@ def run_cases(cases: list, lib=None, log=sys.stdout, autotune: bool = False, tun
                 _append(c.run_dir / "info.txt", " Skipping Bosonic warm-up\n")
         s.prepare()
         s.read_bin()           # clear prepare's counters
+        taus = {(c.params.is_tau, c.params.nthermal) for c in cases}
+        if len(taus) != 1:
+            raise ValueError(f"cases in one batch need the same is_tau and Nthermal: {sorted(taus)}")
+        is_tau, nthermal = taus.pop()
         for b in range(nbin):
-            s.sweep(nsweep, measure=True)
+            # istau_tmp of main.f90:49-52: the time-displaced pass in bins after Nthermal
+            if is_tau and b + 1 > nthermal:
+                s.sweep_tau(nsweep, measure=True, mode=tau_mode)
+                for c in cases:
+                    c.tau_ran = True
+            else:
+                s.sweep(nsweep, measure=True)
             per = s.read_bin()
             fields = s.get_fields()[0] if any(c.poles for c in cases) else None
             for k, c in enumerate(cases):
@@ -232,6 +251,8 @@ def run_cases(cases: list, lib=None, log=sys.stdout, autotune: bool = False, tun
                     c.xmax = max(c.xmax, st["xmax"])
                     c.xmean += st["xmean_sum"]
                     c.nwrap_total += st["nwrap"]
+                    c.tau_xmax = np.maximum(c.tau_xmax, [st["tau_xmax_0t"], st["tau_xmax_t0"], st["tau_xmax_tt"]])
+                    c.tau_xsum += [st["tau_xsum_0t"], st["tau_xsum_t0"], st["tau_xsum_tt"]]
                 if c.poles:
                     zs, scal = [], []
                     for j in range(bounds[k], bounds[k + 1]):
@@ -277,6 +298,14 @@ def _info_tail(c: Case, wall: float):
                      f" Phase_{name}_mean_abs_arg_accepted                : {0.0}",
                      f" Phase_{name}_branch_pi_fraction                   : {0.0}",
                      f" Phase_{name}_local_proposals                      : {float(c.proposals[k])}"]
+    if c.tau_ran:
+        # Dyn_control_print (dynamics.f90:60-82): the maxima and the sums over
+        # ISIZE * Nbin * Nst * Nsweep of the drifts of G(0,tau), G(tau,0),
+        # G(tau,tau), under the reference's labels GRtt, GRt0, GR0t in that order
+        n = max(c.nchains * p.nbin * p.nst * p.nsweep, 1)
+        for k, lab in enumerate(("GRtt", "GRt0", "GR0t")):
+            tail += [f" Max diff {lab}                                  : {c.tau_xmax[k]}",
+                     f" Mean diff {lab}                                 : {c.tau_xsum[k] / n}"]
     tail += [f" Max diff Matrix                                : {c.xmax}",
              f" Mean diff Matrix                               : {c.xmean / max(c.nwrap_total, 1.0)}",
              f" Tot CPU time                                   : {wall}"]
@@ -284,10 +313,10 @@ def _info_tail(c: Case, wall: float):
 
 
 def run(run_dir, solver: str = "number_conserving", nchains: int = 1, lib=None, poles: bool | None = None,
-        log=sys.stdout, autotune: bool = False, tune_nwrap: bool = False):
+        log=sys.stdout, autotune: bool = False, tune_nwrap: bool = False, tau_mode: int = TAU_BOSONIC):
     """One reference run in ``run_dir``; returns a dict of run statistics.
     autotune: run the fastest execution schedule (bafqmc_amd.autotune; the same
     Markov chain); tune_nwrap: let it also choose N_Ortho under its drift bound
     (info.txt then reports the N_Ortho used)."""
     return run_cases([load_case(run_dir, solver, nchains, poles)], lib=lib, log=log, autotune=autotune,
-                     tune_nwrap=tune_nwrap)[0]
+                     tune_nwrap=tune_nwrap, tau_mode=tau_mode)[0]

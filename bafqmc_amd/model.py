@@ -68,6 +68,11 @@ class Params:
     def n(self) -> int:
         return 2 * self.ns if self.pairing else self.ns
 
+    @property
+    def nst(self) -> int:
+        """Stabilisation points per sweep direction (Params_set, calc_basic.f90:100-104)."""
+        return self.ltrot // self.nwrap
+
     def validate(self):
         """Checks of read_input/Params_set (calc_basic.f90:61-62,100-104)."""
         if self.u1 < -ZERO:

@@ -54,6 +54,8 @@ SYMBOLS = {
     "bafq_warmup": (C.c_int, [C.c_void_p, C.c_int]),
     "bafq_prepare": (C.c_int, [C.c_void_p]),
     "bafq_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "bafq_sweep_tau": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bafq_get_green_tau": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "bafq_read_bin": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     "bafq_get_stats": (C.c_int, [C.c_void_p, c_double_p]),
     "bafq_net_create": (C.c_int, [C.POINTER(NetDesc), C.c_int, C.POINTER(C.c_void_p)]),

@@ -29,7 +29,11 @@ O_SITE0 = 40
 STATS = {"prop_u1": 0, "acc_u1": 1, "prop_u2": 2, "acc_u2": 3, "xmax": 4, "xmean_sum": 5,
          "nwrap": 6, "logdet": 7, "therm_prop_u1": 8, "therm_acc_u1": 9, "therm_prop_u2": 10,
          "therm_acc_u2": 11, "t_site": 12, "t_kinetic": 13, "t_measure": 14, "t_wrap": 15, "t_hs": 16,
-         "t_wrap_products": 17, "t_wrap_stab_u": 18, "t_wrap_green": 19}
+         "t_wrap_products": 17, "t_wrap_stab_u": 18, "t_wrap_green": 19,
+         # Wrap_tau drifts of G(0,tau), G(tau,0), G(tau,tau): running maxima, sums, count
+         "tau_xmax_0t": 20, "tau_xmax_t0": 21, "tau_xmax_tt": 22, "tau_xsum_0t": 23, "tau_xsum_t0": 24,
+         "tau_xsum_tt": 25, "tau_nwrap": 26}
+TAU_BOSONIC, TAU_REFERENCE = 1, 2
 
 NC_OUTPUTS = ["density_up", "density_do", "density_total", "density_site_total", "kinetic", "doubleOcc",
               "squareOcc", "num_up", "num_do", "numsquare_up", "numsquare_do", "onsite_n2_up",
@@ -158,6 +162,19 @@ class BatchedSolver:
 
     def sweep(self, n=1, measure=True):
         self._check(self.lib.bafq_sweep(self.h, int(n), 1 if measure else 0), "bafq_sweep")
+
+    def sweep_tau(self, n=1, measure=True, mode=TAU_BOSONIC, record=False):
+        """Sweeps with the time-displaced pass before every sweep_R (Local_sweep
+        with toggle = .true., local_sweep.f90:119-123)."""
+        self._check(self.lib.bafq_sweep_tau(self.h, int(n), 1 if measure else 0, int(mode), 1 if record else 0),
+                    "bafq_sweep_tau")
+
+    def green_tau(self, chain):
+        """[tau][G00, Gt0, G0t, Gtt] of the last recorded pass, Ltrot x 4 x N x N."""
+        n, lt = self.n[chain], self.params(chain).ltrot
+        g = np.zeros((lt, 4, n, n), np.complex128)
+        self._check(self.lib.bafq_get_green_tau(self.h, int(chain), dptr(g.view(np.float64))), "bafq_get_green_tau")
+        return g
 
     MEASURE_MODES = {"all": 0, "checkpoints": 1, "current": 2}
 

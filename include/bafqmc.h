@@ -98,6 +98,34 @@ int bafq_prepare(bafq_handle* h);
  * pair are captured once in a HIP graph and replayed). */
 int bafq_sweep(bafq_handle* h, int n_sweeps, int measure);
 
+/* Conventions of the time-displaced pass. */
+enum { BAFQ_TAU_BOSONIC = 1, BAFQ_TAU_REFERENCE = 2 };
+
+/* bafq_sweep with the time-displaced pass before every sweep_R: Local_sweep
+ * with toggle = .true. (src/number_conserving/src/local_sweep.f90:119-123),
+ * i.e. Dyn_reset + Dyn_sweep_R (dynamics.f90:38-57) on the fields sweep_L
+ * left: G(tau,0), G(0,tau), G(tau,tau) propagated slice by slice from G(0)
+ * (multiply.f90:48-64,88-97) and recomputed every Nwrap slices by Wrap_tau /
+ * stab_green_big (stabilization.f90:192-254,368-404), whose drifts go to the
+ * stats slots 20-22 (running maxima of G(0,tau), G(tau,0), G(tau,tau), the
+ * reference's "Max diff GRtt/GRt0/GR0t" in that order), 23-25 (sums, reset by
+ * bafq_read_bin) and 26 (count).  The pass touches no field, seed, G or
+ * equal-time accumulator: the chain is the one bafq_sweep runs.
+ * mode BAFQ_TAU_BOSONIC: the 2N x 2N block matrix with -D_L, whose inverse
+ * holds the bosonic chain's functions (G(0) = (1 - B)^{-1}); mode
+ * BAFQ_TAU_REFERENCE: the reference's code as written, +D_L
+ * (stabilization.f90:206-208, the fermionic form) and the U2 operator
+ * propagated with the U1 field (dynamics.f90:50-52), whose drift lines it
+ * reproduces.  record != 0 keeps (G00, Gt0, G0t, Gtt) at tau = 0..Ltrot-1 of
+ * the last sweep's pass (Obs_tau_calc's arguments, obser_tau.f90:39-54) for
+ * bafq_get_green_tau.  The workspace (17 N^2 complex per chain, plus
+ * 4 Ltrot N^2 when recording) is allocated on first use. */
+int bafq_sweep_tau(bafq_handle* h, int n_sweeps, int measure, int mode, int record);
+
+/* The recorded time-displaced functions of one chain: Ltrot x 4 x N x N
+ * complex, [tau][G00, Gt0, G0t, Gtt] (PropGreen, process_matrix.f90:17-24). */
+int bafq_get_green_tau(bafq_handle* h, int chain, double* out);
+
 /* n_passes measurement passes: one left sweep each with the fields frozen (no
  * proposals; the RNG streams are untouched) that measures the equal-time
  * estimators at every slice into the same accumulators as bafq_sweep, as

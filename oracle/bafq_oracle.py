@@ -18,7 +18,11 @@ Each function cites the reference lines it follows.  Pinning (see
 closed-form Gaussian/Bogoliubov limit (tests/physics/test_gaussian_solvers.py:44-77),
 independent dense fixed-field products (tests/physics/test_fixed_field_solver.py:55-119),
 brute-force determinant ratios, and golden outputs of the reference Fortran
-binaries built from source by ``oracle/Makefile.ref``.
+binaries built from source by ``oracle/Makefile.ref``.  The time-displaced
+pass (``Chain.tau_pass``, ``stab_green_big``) is pinned in
+``tests/test_time_displaced.py``: with the reference's conventions it
+reproduces the Fortran runs' Wrap_tau drift lines, with the bosonic ones the
+closed form from explicit products (``dense_green_tau``).
 """
 from __future__ import annotations
 
@@ -304,6 +308,46 @@ def compare_mat(a, b):
 
 
 # ---------------------------------------------------------------------------
+# Time-displaced Green's functions (stabilization.f90:192-254)
+# ---------------------------------------------------------------------------
+# Sign of the B(beta, tau) block of the 2N x 2N matrix.  The reference writes
+# mat_big(nr, nr + Ndim) = +DUL(nr) (stabilization.f90:206-208), which inverts
+# [[1, B(beta,tau)], [-B(tau,0), 1]] -- the fermionic (1 + B)^{-1} form; its own
+# propagation of the time-displaced functions (dynamics.f90:44-57 from
+# process_matrix.f90:78-85) is the bosonic one, G(0) = (1 - B)^{-1}.  With
+# -DUL the inverse is [[G(0,0), G(0,tau)], [G(tau,0), G(tau,tau)]] of the
+# bosonic chain.
+REFERENCE_TAU_SIGN = 1.0
+BOSONIC_TAU_SIGN = -1.0
+
+
+def udv(a):
+    """A = U diag(D) V by Householder QR without pivoting, D = |R_ii|, rows of
+    R divided by D (src/common/mymats.f90:79-113, ncon = 0)."""
+    q, r = np.linalg.qr(a)
+    d = np.abs(np.diag(r))
+    v = np.triu(r) / d[:, None]
+    return q, d, v
+
+
+def stab_green_big(pr: Prop, sign: float = BOSONIC_TAU_SIGN):
+    """(G00, G0t, Gt0, Gtt) from the UDV factors of B(tau,0) = UR DR VR and
+    B(beta,tau) = VL DL UL (stab_green_big, stabilization.f90:192-254)."""
+    n = pr.ur.shape[0]
+    inv = np.linalg.inv
+    big = np.zeros((2 * n, 2 * n), complex)
+    big[:n, :n] = inv(pr.vr @ pr.vl)
+    big[n:, n:] = inv(pr.ul @ pr.ur)
+    big[np.arange(n), n + np.arange(n)] = sign * pr.dl
+    big[n + np.arange(n), np.arange(n)] = -pr.dr
+    bu, bd, bv = udv(big)
+    left = sla.block_diag(inv(pr.vr), inv(pr.ul)) @ inv(bv) / bd[None, :]
+    right = inv(bu) @ sla.block_diag(inv(pr.vl), inv(pr.ur))
+    tot = left @ right
+    return tot[:n, :n], tot[:n, n:], tot[n:, :n], tot[n:, n:]
+
+
+# ---------------------------------------------------------------------------
 # The chain: both solvers share the sweep orchestration
 # (local_sweep.f90:55-161; identical in src/pairing/src/local_sweep.f90)
 # ---------------------------------------------------------------------------
@@ -329,6 +373,8 @@ class Chain:
         self.acc = np.zeros((2, 2))        # [channel, (proposed, accepted)] per bin
         self.acc_ratio_sum = np.zeros(2)   # sum over bins of per-bin ratios (Acc_calc_ratio)
         self.phase = np.zeros((2, 5))      # pairing phase counters per bin
+        self.tau_xmax = np.zeros(3)        # Wrap_tau drifts of G0t, Gt0, Gtt (PropGreen Xmaxm, Xmeanm)
+        self.tau_xmean = np.zeros(3)
         self.obs = None
         self.nobs = 0
 
@@ -344,15 +390,16 @@ class Chain:
     def _rows(self, i):
         return [sec * self.ns + i for sec in range(self.nsec)]
 
-    def scale_rows(self, mat, nf, nt, i, nflag):
-        """opU_mmult_R: row(s) of site i times exp(+-alpha phi); operator_Hubbard.f90:70-85."""
-        a = self.alpha[nf - 1] * self.phi[nt, i, nf - 1] * nflag
+    def scale_rows(self, mat, nf, nt, i, nflag, fld=None):
+        """opU_mmult_R: row(s) of site i times exp(+-alpha phi); operator_Hubbard.f90:70-85.
+        fld: the field channel of phi (default: the operator's own, nf)."""
+        a = self.alpha[nf - 1] * self.phi[nt, i, (fld or nf) - 1] * nflag
         for r, s in zip(self._rows(i), self._sector_signs(nf)):
             mat[r, :] *= np.exp(s * a)
 
-    def scale_cols(self, mat, nf, nt, i, nflag):
+    def scale_cols(self, mat, nf, nt, i, nflag, fld=None):
         """opU_mmult_L: column(s) of site i; operator_Hubbard.f90:87-102."""
-        a = self.alpha[nf - 1] * self.phi[nt, i, nf - 1] * nflag
+        a = self.alpha[nf - 1] * self.phi[nt, i, (fld or nf) - 1] * nflag
         for r, s in zip(self._rows(i), self._sector_signs(nf)):
             mat[:, r] *= np.exp(s * a)
 
@@ -541,14 +588,69 @@ class Chain:
                 self.wrap_R(nt)
             self.measure()
 
-    def bin(self, nsweep: int = 1):
-        """Local_sweep with is_beta=.true. (local_sweep.f90:136-161), then Obs ave."""
+    # -- time-displaced path (dynamics.f90:38-57, stabilization.f90:368-404) ---
+    def tau_pass(self, sign: float = BOSONIC_TAU_SIGN, record=None, u2_field: int = 2):
+        """Dyn_reset + Dyn_sweep_R with the fields frozen, run between sweep_L
+        and sweep_R (local_sweep.f90:119-123): G(tau,0), G(0,tau), G(tau,tau)
+        propagated slice by slice from G(0) (PropGr_reset, process_matrix.f90:78-85;
+        propgrU_R, propgrT_R, multiply.f90:48-64,88-97) and recomputed at every
+        stabilisation point (Wrap_tau) with the drift of each against its
+        propagated value.  ``record``: a list that receives (G00, Gt0, G0t, Gtt)
+        at tau = 0 .. Ltrot - 1, the points where Obs_tau_calc sees them.
+        ``u2_field``: the field channel the U2 operator is propagated with; the
+        reference passes nf = 1 for both operators (dynamics.f90:50-52), so its
+        U2 factor carries the U1 channel's field, while its checkpoint lists
+        (sweep_L) carry phi_2; ``REFERENCE_TAU_SIGN`` with ``u2_field = 1``
+        reproduces the reference's pass, the defaults give the bosonic chain's
+        time-displaced functions."""
+        pr, p = self.prop, self.p
+        n = self.n
+        pd = Prop(n, p.nst)                       # Prop_d = Prop (Prop_assign)
+        pd.ur, pd.vr, pd.dr = pr.ur.copy(), pr.vr.copy(), pr.dr.copy()
+        pd.ul, pd.vl, pd.dl = pr.ul.copy(), pr.vl.copy(), pr.dl.copy()
+        g00, gt0, gtt = pr.g.copy(), pr.g.copy(), pr.g.copy()
+        g0t = pr.g - np.eye(n)
+        for nt in range(1, p.ltrot + 1):
+            if record is not None:
+                record.append((g00.copy(), gt0.copy(), g0t.copy(), gtt.copy()))
+            for nf in (1, 2):
+                if not self.active[nf - 1]:
+                    continue
+                fld = u2_field if nf == 2 else 1
+                for i in range(self.ns):
+                    self.scale_rows(gt0, nf, nt - 1, i, 1, fld)
+                    self.scale_cols(g0t, nf, nt - 1, i, -1, fld)
+                    self.scale_rows(gtt, nf, nt - 1, i, 1, fld)
+                    self.scale_cols(gtt, nf, nt - 1, i, -1, fld)
+                for i in range(self.ns):
+                    self.scale_rows(pd.ur, nf, nt - 1, i, 1, fld)
+            gt0 = self.ep @ gt0
+            g0t = g0t @ self.em
+            gtt = (self.ep @ gtt) @ self.em
+            pd.ur = self.ep @ pd.ur
+            if nt % p.nwrap == 0:
+                st = nt // p.nwrap
+                pd.ul, pd.vl, pd.dl = pr.ullist[st].copy(), pr.vllist[st].copy(), pr.dllist[st].copy()
+                stab_ur(pd)
+                n00, n0t, nt0, ntt = stab_green_big(pd, sign)
+                for k, (new, old) in enumerate(((n0t, g0t), (nt0, gt0), (ntt, gtt))):
+                    dif = compare_mat(new, old)
+                    self.tau_xmax[k] = max(self.tau_xmax[k], dif)
+                    self.tau_xmean[k] += dif
+                g00, g0t, gt0, gtt = n00, n0t, nt0, ntt
+
+    def bin(self, nsweep: int = 1, tau: bool = False, tau_sign: float = BOSONIC_TAU_SIGN, record=None,
+            tau_u2_field: int = 2):
+        """Local_sweep with is_beta=.true. (local_sweep.f90:136-161), then Obs ave;
+        tau: the time-displaced pass before each sweep_R (toggle = .true.)."""
         self.obs = None
         self.nobs = 0
         self.acc[:] = 0
         self.phase[:] = 0
         for _ in range(nsweep):
             self.sweep_L()
+            if tau:
+                self.tau_pass(tau_sign, record, tau_u2_field)
             self.sweep_R()
         out = {k: v / self.nobs for k, v in self.obs.items()}
         for c in range(2):
@@ -732,6 +834,23 @@ def dense_green(ch: Chain, tau: int):
     for idx in (*range(tau, lt), *range(tau)):
         prod = bs[idx] @ prod
     return np.linalg.inv(np.eye(ch.n) - prod), prod
+
+
+def dense_green_tau(ch: Chain, tau: int):
+    """(G(0,0), G(tau,0), G(0,tau), G(tau,tau)) of the bosonic chain from
+    explicit products: G(0) = (1 - B(beta,0))^{-1}, G(tau,0) = B(tau,0) G(0),
+    G(0,tau) = (G(0) - 1) B(tau,0)^{-1} = G(0) B(beta,tau) (no inverse of B(tau,0),
+    whose pairing spectrum spans e^{+-tau |E|}), G(tau,tau) = (1 - B(tau,0) B(beta,tau))^{-1}."""
+    bs = slice_matrices(ch)
+    eye = np.eye(ch.n)
+    b_t0 = eye.astype(complex)
+    for idx in range(tau):
+        b_t0 = bs[idx] @ b_t0
+    b_bt = eye.astype(complex)
+    for idx in range(tau, ch.p.ltrot):
+        b_bt = bs[idx] @ b_bt
+    g0 = np.linalg.inv(eye - b_bt @ b_t0)
+    return g0, b_t0 @ g0, g0 @ b_bt, np.linalg.inv(eye - b_t0 @ b_bt)
 
 
 def gaussian_closed_form(delta: float, beta: float, mu: float, length: int = 3):

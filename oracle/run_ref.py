@@ -41,7 +41,7 @@ def param_text(p: dict) -> str:
         f"{p['nlx']} {p['nly']} {p['ltrot']} {p['beta']!r}",
         f"{p.get('nlx_therm', p['nlx'])} {p.get('nly_therm', p['nly'])} {p.get('ltrot_therm', p['ltrot'])}",
         f"{p['nwrap']} {p['nbin']} {p.get('nsweep', 1)} {p['shift_loc']!r}",
-        ".false. 0",
+        f"{'.true.' if p.get('is_tau') else '.false.'} {p.get('nthermal', 0)}",
         f"{'.true.' if p.get('nwarm', 0) else '.false.'} {p.get('nwarm', 0)} 1.0 1.0",
         f"{p.get('ini_type', 2)} {p.get('ini_ampl', 1.0)!r} {p.get('ini_bias1', 0.0)!r} {p.get('ini_bias2', 0.0)!r}",
     ]

@@ -103,6 +103,17 @@ CASES = {
     "pairing_blowup_3x3": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=3, nly=3, ltrot=10,
                                       beta=1.0, nwrap=10, nbin=20, shift_loc=1.0, ini_type=2, ini_ampl=1.0),
                                seeds=[13579113]),
+    # the time-displaced pass (is_tau) between sweep_L and sweep_R of every bin
+    # after Nthermal; its only output is info.txt's six Wrap_tau drift lines
+    "nc_tau_3x3": dict(p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=3, nly=3, ltrot=20, beta=2.0,
+                              nwrap=5, nbin=3, shift_loc=1.5, ini_type=2, ini_ampl=1.0, is_tau=True, nthermal=1),
+                       seeds=[97531, 13579]),
+    "nc_tau_6x6_u1": dict(p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=6, nly=6, ltrot=40, beta=4.0,
+                                 nwrap=10, nbin=2, shift_loc=1.5, ini_type=2, ini_ampl=1.0, is_tau=True,
+                                 nthermal=0), seeds=[24681357]),
+    "pairing_tau_3x3": dict(p=dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.5, nlx=3, nly=3, ltrot=20,
+                                   beta=2.0, nwrap=5, nbin=2, shift_loc=1.0, ini_type=2, ini_ampl=1.0,
+                                   is_tau=True, nthermal=0), seeds=[11223344]),
 }
 
 
