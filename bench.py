@@ -80,9 +80,9 @@ CONFIGS = {
                                 beta=b, nwrap=10, shift_loc=1.5)
                            for L in (3, 6, 9, 12) for b in (4.0, 8.0) for u2 in (0.0, -0.5)],
                    chains=256),
-    "nc12x12": dict(name="number-conserving 12x12 beta=8 Ltrot=80 U1=1 U2=-0.5 mu=-5, 96 chains",
+    "nc12x12": dict(name="number-conserving 12x12 beta=8 Ltrot=80 U1=1 U2=-0.5 mu=-5, 256 chains",
                  p=dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=12, nly=12, ltrot=80, beta=8.0, nwrap=10,
-                        shift_loc=1.5), chains=96, net=dict(layers=4, hidden=32)),
+                        shift_loc=1.5), chains=256, net=dict(layers=4, hidden=32), slug="nc12x12_256chains"),
     "nc3x3": dict(name="number-conserving 3x3 beta=4 Ltrot=40 U1=1 U2=0 mu=-5, 512 chains",
                p=dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=3, nly=3, ltrot=40, beta=4.0, nwrap=10,
                       shift_loc=1.5), chains=512, net=dict(layers=3, hidden=8)),
@@ -138,10 +138,11 @@ def net_flops(desc, batch):
     return tot * batch
 
 
-def measured_traffic(cfg, kernel, nwrap):
+def measured_traffic(cfg, kernel, nwrap, chains=None, schedule=None):
     """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-calibrated) of this workload at this Nwrap, when
-    there is one."""
+    WRITE_SIZE summary (copy-calibrated) of this workload at this Nwrap (and,
+    where the summary records them, this chain count and schedule), when there
+    is one."""
     if "slug" not in cfg:
         return None
     f = ROOT / "profiles" / f"{kernel}_{cfg['slug']}_hbm.json"
@@ -149,6 +150,10 @@ def measured_traffic(cfg, kernel, nwrap):
         return None
     rec = json.loads(f.read_text())
     if rec.get("nwrap", nwrap) != nwrap or rec.get("mu", cfg["p"]["mu"]) != cfg["p"]["mu"]:
+        return None
+    if chains is not None and rec.get("chains", chains) != chains:
+        return None
+    if schedule is not None and "schedule" in rec and list(rec["schedule"]) != list(schedule):
         return None
     return rec["traffic_bytes_per_launch"]
 
@@ -452,7 +457,7 @@ def run_plain(args, cfg, p, spec, nchains):
     a = statistics(sf, wall, args.steps, "plain")
     kern = "k_sweep_lds" if s.schedule()[0][0] == 1 else "k_sweep"
     rf = roofline(kern, flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch_of(p)) * nchains, kms)
-    rf["traffic"] = measured_traffic(cfg, kern, p.nwrap)
+    rf["traffic"] = measured_traffic(cfg, kern, p.nwrap, nchains, s.schedule()[0])
     out = {
         "metric": METRIC, "value": a["eff_samples_per_s"], "unit": "effective S_SF(K) samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": a["ms_per_step"],
@@ -590,6 +595,8 @@ def main():
         if getattr(args, k) is None:
             setattr(args, k, v)
     nchains = args.chains or cfg["chains"]
+    if nchains != cfg["chains"]:
+        cfg = dict(cfg, name=cfg["name"].replace(f"{cfg['chains']} chains", f"{nchains} chains"))
     if "models" in cfg:
         run_uneven(args, cfg, nchains)
         return
