@@ -240,10 +240,11 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
         // 4) that fit unless the schedule fixes the width.
         {
             const bafq_schedule want = schedule ? schedule[k] : bafq_schedule{-1, -1};
-            if (want.path < -1 || want.path > 1 || want.delayed < -1 || want.delayed > 16 ||
+            if (want.path < -1 || want.path > 1 || want.delayed < -1 || want.delayed > kMaxDkGlb ||
                 (want.delayed > 0 && want.delayed % 4)) {
                 *out = h;
-                return fail(h, "invalid schedule (path -1/0/1, delayed -1/0/4/8/12/16)");
+                return fail(h, "invalid schedule (path -1/0/1, delayed -1/0/4/8/12/16, or up to 32 on the "
+                               "device-memory path)");
             }
             const int lds_threads = rt_lds_block_threads();
             const int lds_waves = lds_threads / 64;
@@ -265,7 +266,11 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
             m.dk = 0;
             m.uw_glb = 0;
             if (want.delayed > 0) {
-                if (!fits(want.delayed)) {
+                if (want.delayed > 16 && path == 1) {
+                    *out = h;
+                    return fail(h, "delayed-update widths above 16 need the device-memory path");
+                }
+                if (want.delayed > 16 || !fits(want.delayed)) {
                     if (path == 1 || want.delayed > m.n) {
                         *out = h;
                         return fail(h, "schedule's delayed-update width does not fit");

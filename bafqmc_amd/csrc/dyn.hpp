@@ -62,17 +62,18 @@ DEV TauWs tau_ws(const ModelDev& m, const ChainDev& ch) {
     return T;
 }
 
-// X <- (solution), out = A^{-1} X from the pivoted QR of A held in F
-// (qrp: A P = Q diag(D) V): A^{-1} = P V^{-1} D^{-1} Q^H.  X is overwritten.
+// out = A^{-1} X from the pivoted QR of A held in F (qrp: A P = Q diag(D) V):
+// A^{-1} = P V^{-1} D^{-1} Q^H.  X is overwritten; out (n x n, not X) is the
+// work buffer of the blocked reflector application until the result lands there.
 DEV void qrp_solve(const Ctx& c, cd* out, cd* X, const cd* F, const QrScratch& s, const double* D, int n) {
-    apply_qh_left(c, X, F, s.tau, n);
+    apply_qh_left(c, X, F, s.tau, n, out);
     const int nn = n * n;
     for (int e = c.tid(); e < nn; e += c.nthreads()) {
         const int r = e / n;
         X[e] = X[e] * cd{1.0 / D[r], 0.0};
     }
     c.sync();
-    trsm_upper_left(c, X, F, n);
+    trsm_upper_left(c, X, F, n, out);
     for (int e = c.tid(); e < nn; e += c.nthreads()) {
         const int r = e / n, k = e - r * n;
         out[s.perm[r] * n + k] = X[e];

@@ -506,9 +506,189 @@ DEV void refl_apply_glb(const Ctx& c, cd* X, const cd* A, int n, int k, cd t, in
     }
 }
 
-// Q = H(0) ... H(n-1) formed explicitly (ZUNGQR).
-DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n) {
+// ------------------------- blocked reflector application and substitution (device memory)
+// For n >= kBlkMinN and with a work buffer of 32 n + 512 complex, Q formation,
+// Q^H X and the triangular solve run on blocks of kBlk reflectors or rows: a
+// block of reflectors H(kb) ... H(kb + nb - 1) = I - V T V^H (ZLARFT, forward,
+// columnwise) is applied as X <- X - V (op(T) (V^H X)), two FP64 MFMA
+// products that read the rows of X once each, in place of nb passes over the
+// whole matrix (ZLARFB).  Work layout: Vt[kBlk][n] (the block's reflectors as
+// rows, unit on the diagonal, zero above it), W[kBlk][n], Gram[kBlk][kBlk],
+// T[kBlk][kBlk].  Same mathematics as the per-reflector loops; the rounding
+// differs.
+constexpr int kBlk = 16;
+constexpr int kBlkMinN = 64;
+
+HOSTDEV size_t blk_work_elems(int n) { return (size_t)2 * kBlk * n + 2 * kBlk * kBlk; }
+
+// Vt[j][r] = entry r of reflector kb + j (j < nb), zero rows for j >= nb.
+DEV void blk_stage_reflectors(const Ctx& c, cd* Vt, const cd* A, int n, int kb, int nb) {
+    for (int e = c.tid(); e < kBlk * n; e += c.nthreads()) {
+        const int j = e / n, r = e - j * n;
+        const int k = kb + j;
+        cd v = cd{0.0, 0.0};
+        if (j < nb) v = r == k ? cd{1.0, 0.0} : (r > k ? A[r * n + k] : cd{0.0, 0.0});
+        Vt[e] = v;
+    }
+    c.sync();
+}
+
+// C[i][c0 + j] = sum_{r >= r0} conj(Vt[i][r]) B(r, c0 + j) for j < m (C with
+// leading dimension ldc_), on FP64 MFMA 16x16x4: output tiles of 16 columns,
+// one per wavefront in turn.
+template <class LB>
+DEV void blk_vh_times(const Ctx& c, cd* C, int ldc_, const cd* Vt, int n, int r0, int c0, int m, LB lb) {
+    const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
+    const int ntj = (m + 15) >> 4;
+    for (int tj = c.wave(); tj < ntj; tj += c.nwaves()) {
+        const int col = c0 + 16 * tj + li;
+        const bool cin = 16 * tj + li < m;
+        d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ri = rr, ir = rr;
+        for (int k0 = r0; k0 < n; k0 += 16) {
+            cd a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int ka = k0 + 4 * q + lk;        // A operand: row li, inner lk
+                const int kr = k0 + 4 * q + lk;        // B operand: inner lk, column li
+                a[q] = ka < n ? Vt[li * n + ka] : cd{0.0, 0.0};
+                b[q] = (kr < n && cin) ? lb(kr, col) : cd{0.0, 0.0};
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                rr = c.mfma(a[q].r, b[q].r, rr);
+                ii = c.mfma(a[q].i, b[q].i, ii);
+                ri = c.mfma(a[q].r, b[q].i, ri);
+                ir = c.mfma(a[q].i, b[q].r, ir);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = mfma_row(lane, r);
+            if (cin) C[row * ldc_ + col] = cd{rr[r] + ii[r], ri[r] - ir[r]};   // conj(v) b
+        }
+    }
+    c.sync();
+}
+
+// T of the block (upper triangular, H(kb) ... H(kb+nb-1) = I - V T V^H) from
+// the Gram matrix Gm = V^H V: T[i][i] = tau_i, T[0:i][i] = -tau_i T[0:i][0:i] Gm[0:i][i]
+// (ZLARFT 'F', 'C').  Lane a < 16 of wavefront 0 builds row a of T in
+// registers (row a needs only itself and Gm), then stores it.
+DEV void blk_build_t(const Ctx& c, cd* T, const cd* Gm, const cd* tau, int kb, int nb) {
+    if (c.wave() == 0) {
+        const int a = c.lane();
+        cd trow[kBlk];
+#pragma unroll
+        for (int b = 0; b < kBlk; ++b) trow[b] = cd{0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < kBlk; ++i) {
+            if (i < nb) {
+                cd z = cd{0.0, 0.0};          // sum_{b < i} T[a][b] Gm[b][i] (T[a][b] = 0 for b < a)
+#pragma unroll
+                for (int b = 0; b < i; ++b) z = z + trow[b] * Gm[b * kBlk + i];
+                const cd ti = tau[kb + i];
+                if (a < i) trow[i] = -(ti * z);
+                else if (a == i) trow[i] = ti;
+            }
+        }
+        if (a < kBlk)
+#pragma unroll
+            for (int b = 0; b < kBlk; ++b) T[a * kBlk + b] = trow[b];
+    }
+    c.sync();
+}
+
+// W[:, c0 : c0 + m] <- op(T) W, op(T) = T (herm false) or T^H; W with leading dimension n.
+DEV void blk_t_times(const Ctx& c, cd* W, const cd* T, int n, int c0, int m, bool herm) {
+    for (int j = c.tid(); j < m; j += c.nthreads()) {
+        cd w[kBlk], o[kBlk];
+#pragma unroll
+        for (int b = 0; b < kBlk; ++b) w[b] = W[b * n + c0 + j];
+#pragma unroll
+        for (int i = 0; i < kBlk; ++i) {
+            cd acc = cd{0.0, 0.0};
+#pragma unroll
+            for (int b = 0; b < kBlk; ++b) {
+                // T upper: T[i][b] for b >= i; T^H[i][b] = conj(T[b][i]) for b <= i
+                if (!herm && b >= i) acc = acc + T[i * kBlk + b] * w[b];
+                if (herm && b <= i) acc = acc + conj(T[b * kBlk + i]) * w[b];
+            }
+            o[i] = acc;
+        }
+#pragma unroll
+        for (int b = 0; b < kBlk; ++b) W[b * n + c0 + j] = o[b];
+    }
+    c.sync();
+}
+
+// X[r][col] -= sum_{k < cols} Ut[k][r] Wr[k][col] for rows r in [r_lo, r_hi) and
+// columns col in [c_lo, n) (16-aligned tiles; Ut and Wr with leading dimension
+// n, cols <= 16), on FP64 MFMA: the shape of the delayed-update flush.
+DEV void blk_rank_update(const Ctx& c, cd* X, const cd* Ut, const cd* Wr, int n, int cols, int r_lo, int r_hi,
+                         int c_lo) {
+    const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
+    const int ti0 = r_lo >> 4, ti1 = (r_hi + 15) >> 4, tj0 = c_lo >> 4, tj1 = (n + 15) >> 4;
+    const int nti = ti1 - ti0, ntj = tj1 - tj0;
+    for (int t = c.wave(); t < nti * ntj; t += c.nwaves()) {
+        const int ti = ti0 + t / ntj, tj = tj0 + t % ntj;
+        const int ar = 16 * ti + li, bc = 16 * tj + li;
+        cd a[4], b[4], g[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kk = 4 * q + lk;
+            a[q] = (ar < r_hi && kk < cols) ? Ut[kk * n + ar] : cd{0.0, 0.0};
+            b[q] = (bc < n && kk < cols) ? Wr[kk * n + bc] : cd{0.0, 0.0};
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
+            g[r] = (row < r_hi && col < n) ? X[row * n + col] : cd{0.0, 0.0};
+        }
+        d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ss = rr;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (4 * q < cols) {
+                rr = c.mfma(a[q].r, b[q].r, rr);
+                ii = c.mfma(a[q].i, b[q].i, ii);
+                ss = c.mfma(a[q].r + a[q].i, b[q].r + b[q].i, ss);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
+            if (row >= r_lo && row < r_hi && col >= c_lo && col < n)
+                X[row * n + col] = cd{g[r].r - (rr[r] - ii[r]), g[r].i - (ss[r] - rr[r] - ii[r])};
+        }
+    }
+    c.sync();
+}
+
+// One block of reflectors applied from the left to columns [c0, n) of X, rows
+// [kb, n): X <- (I - V op(T) V^H) X with op(T) = T^H (herm, for Q^H) or T.
+DEV void blk_apply(const Ctx& c, cd* X, const cd* A, const cd* tau, int n, int kb, int c0, bool herm, cd* work) {
+    const int nb = n - kb < kBlk ? n - kb : kBlk;
+    cd* Vt = work;
+    cd* W = work + (size_t)kBlk * n;
+    cd* Gm = work + (size_t)2 * kBlk * n;
+    cd* T = Gm + kBlk * kBlk;
+    blk_stage_reflectors(c, Vt, A, n, kb, nb);
+    blk_vh_times(c, Gm, kBlk, Vt, n, kb, 0, kBlk, [&](int r, int j) { return Vt[j * n + r]; });
+    blk_build_t(c, T, Gm, tau, kb, nb);
+    blk_vh_times(c, W, n, Vt, n, kb, c0, n - c0, [&](int r, int j) { return X[r * n + j]; });
+    blk_t_times(c, W, T, n, c0, n - c0, herm);
+    blk_rank_update(c, X, Vt, W, n, nb, kb, n, c0);
+}
+
+// Q = H(0) ... H(n-1) formed explicitly (ZUNGQR); blocked with a work buffer
+// of blk_work_elems(n) complex for n >= kBlkMinN (Q <- Q_b Q from the last
+// block to the first; block kb touches columns >= kb only, the others are
+// still those of the identity there).
+DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n, cd* work = nullptr) {
     mat_identity(c, Q, n);
+    if (work && n >= kBlkMinN) {
+        for (int kb = ((n - 1) / kBlk) * kBlk; kb >= 0; kb -= kBlk) blk_apply(c, Q, A, tau, n, kb, kb, false, work);
+        return;
+    }
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     for (int k = n - 1; k >= 0; --k) {
@@ -543,8 +723,12 @@ DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n) {
     }
 }
 
-// X <- Q^H X (ZUNMQR 'L','C').
-DEVFN void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n) {
+// X <- Q^H X (ZUNMQR 'L','C'); blocked as form_q with a work buffer.
+DEVFN void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n, cd* work = nullptr) {
+    if (work && n >= kBlkMinN) {
+        for (int kb = 0; kb < n; kb += kBlk) blk_apply(c, X, A, tau, n, kb, 0, true, work);
+        return;
+    }
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     for (int k = 0; k < n; ++k) {
@@ -579,8 +763,42 @@ DEVFN void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n)
     }
 }
 
-// X <- V^{-1} X, V = upper triangle of A (ZTRSM 'L','U','N','N').
-DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n) {
+// X <- V^{-1} X, V = upper triangle of A (ZTRSM 'L','U','N','N').  With a
+// work buffer (kBlk n complex) and n >= kBlkMinN: row blocks of kBlk from the
+// bottom, each solved in registers (one thread per column of X) and then
+// removed from the rows above by one MFMA product.
+DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n, cd* work = nullptr) {
+    if (work && n >= kBlkMinN) {
+        for (int r0 = ((n - 1) / kBlk) * kBlk; r0 >= 0; r0 -= kBlk) {
+            const int nb = n - r0 < kBlk ? n - r0 : kBlk;
+            for (int j = c.tid(); j < n; j += c.nthreads()) {
+                cd x[kBlk];
+#pragma unroll
+                for (int a = 0; a < kBlk; ++a) x[a] = a < nb ? X[(r0 + a) * n + j] : cd{0.0, 0.0};
+#pragma unroll
+                for (int a = kBlk - 1; a >= 0; --a) {
+                    if (a < nb) {
+                        x[a] = x[a] * cinv(A[(r0 + a) * n + r0 + a]);
+#pragma unroll
+                        for (int b = 0; b < a; ++b) x[b] = x[b] - A[(r0 + b) * n + r0 + a] * x[a];
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < kBlk; ++a)
+                    if (a < nb) X[(r0 + a) * n + j] = x[a];
+            }
+            c.sync();
+            if (r0 == 0) break;
+            // Ut[k][r] = A[r][r0 + k] for the rows above the block
+            for (int e = c.tid(); e < kBlk * n; e += c.nthreads()) {
+                const int k = e / n, r = e - k * n;
+                work[e] = (k < nb && r < r0) ? A[r * n + r0 + k] : cd{0.0, 0.0};
+            }
+            c.sync();
+            blk_rank_update(c, X, work, X + (size_t)r0 * n, n, nb, 0, r0, 0);
+        }
+        return;
+    }
     const int tid = c.tid(), nth = c.nthreads();
     for (int r = n - 1; r >= 0; --r) {
         const cd inv = cinv(A[r * n + r]);

@@ -49,6 +49,7 @@ enum StatIndex {
 constexpr int kNumTimers = 8;
 
 constexpr int kLdsWaves = 8;      // LDS path: 512 threads
+constexpr int kMaxDkGlb = 32;     // widest delayed-update buffers (in device memory)
 constexpr int kLdsMaxN = 96;      // LDS path: N <= 96 (G in LDS, <= 36 MFMA tiles)
 constexpr int kMaxTiles = 5;      // ceil(36 / 8) output tiles per wavefront
 
@@ -460,8 +461,9 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 // W[j][c]) so that consecutive lanes touch consecutive LDS addresses.
 
 // G0 -= U[:, :cols] W[:cols, :] on FP64 MFMA 16x16x4 (each output tile owned by
-// one wavefront); G in LDS or in device memory, U and W in LDS.
-template <class GP, class UP>
+// one wavefront); G in LDS or in device memory, U and W in LDS or device
+// memory; cols <= kMaxCols (16 or 32: blocks of 16 inner indices in turn).
+template <int kMaxCols = 16, class GP, class UP>
 DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols) {
     n = c.uniform(n);                 // scalar tests of the k-steps taken
     cols = c.uniform(cols);
@@ -470,31 +472,38 @@ DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols) {
     for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
         const int ti = t / nt, tj = t - ti * nt;
         const int ar = 16 * ti + li, bc = 16 * tj + li;
-        // operands and the tile of G0 are loaded before the products
-        cd a[4], b[4], g[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int kk = 4 * q + lk;
-            a[q] = cd{0.0, 0.0};
-            b[q] = cd{0.0, 0.0};
-            if (ar < n && kk < cols) a[q] = ldc(Ut, kk * n + ar);
-            if (bc < n && kk < cols) b[q] = ldc(W, kk * n + bc);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
-            g[r] = cd{0.0, 0.0};
-            if (row < n && col < n) g[r] = ldc(G, row * n + col);
-        }
+        cd g[4];
         // three real products per complex one (Gauss): Re = ar br - ai bi,
         // Im = (ar + ai)(br + bi) - ar br - ai bi; three independent chains
         d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ss = rr;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (4 * q < cols) {
-                rr = c.mfma(a[q].r, b[q].r, rr);
-                ii = c.mfma(a[q].i, b[q].i, ii);
-                ss = c.mfma(a[q].r + a[q].i, b[q].r + b[q].i, ss);
+        for (int kb = 0; kb < kMaxCols; kb += 16) {
+            if (kMaxCols > 16 && kb >= cols) break;
+            // operands (and with the first block the tile of G0) are loaded before the products
+            cd a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int kk = kb + 4 * q + lk;
+                a[q] = cd{0.0, 0.0};
+                b[q] = cd{0.0, 0.0};
+                if (ar < n && kk < cols) a[q] = ldc(Ut, kk * n + ar);
+                if (bc < n && kk < cols) b[q] = ldc(W, kk * n + bc);
+            }
+            if (kb == 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
+                    g[r] = cd{0.0, 0.0};
+                    if (row < n && col < n) g[r] = ldc(G, row * n + col);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (kb + 4 * q < cols) {
+                    rr = c.mfma(a[q].r, b[q].r, rr);
+                    ii = c.mfma(a[q].i, b[q].i, ii);
+                    ss = c.mfma(a[q].r + a[q].i, b[q].r + b[q].i, ss);
+                }
             }
         }
 #pragma unroll
@@ -514,12 +523,14 @@ DEVFN void delayed_flush_glb(const Ctx& c, cd* G, const cd* Ut, const cd* W, int
     delayed_flush_t(c, as_global(G), as_lds(Ut), as_lds(W), n, cols);
 }
 DEVFN void delayed_flush_glb_uw(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
-    delayed_flush_t(c, as_global(G), as_global(Ut), as_global(W), n, cols);
+    delayed_flush_t<kMaxDkGlb>(c, as_global(G), as_global(Ut), as_global(W), n, cols);
 }
 
 // Same decisions, RNG use and result as site_loop_t (LocalU_metro + LocalU_prop_L/R).
 // G in LDS (kGlb false) or in device memory; U and W in LDS, or (kUW) in the
-// chain's work matrices w2, w3 in device memory.  Requires dk <= 16, dk <= n.
+// chain's work matrices w2, w3 in device memory.  Requires dk <= n, and
+// dk <= 16 with the buffers in LDS, dk <= kMaxDkGlb = 32 in device memory
+// (fewer passes of the flush over a G in device memory).
 template <bool kGlb, bool kUW = false>
 DEV void site_loop_delayed_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t,
                              int nf, bool ascending, int& seed) {
@@ -560,6 +571,15 @@ DEV void site_loop_delayed_t(const Ctx& c, const ModelDev& m, const ChainDev& ch
             const cd w = ldc(W, ej * n + sb);
             cr = u.r * w.r - u.i * w.i;
             ci = u.r * w.i + u.i * w.r;
+        }
+        if constexpr (kUW) {
+            // inner indices 16..31 of buffers wider than 16 columns
+            if (ej + 16 < cols && (pairing || lane < 16)) {
+                const cd u = ldc(Ut, (ej + 16) * n + sa);
+                const cd w = ldc(W, (ej + 16) * n + sb);
+                cr += u.r * w.r - u.i * w.i;
+                ci += u.r * w.i + u.i * w.r;
+            }
         }
         const cd gss = ldc(G, sa * n + sb);
         Proposal P;
@@ -786,7 +806,7 @@ DEVFN void stab_ur_from(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
     trmm_perm_rows(c, ch.w1, A, ch.vr, L.qr.perm, n);
     mat_copy(c, ch.vr, ch.w1, n, false);
     if (cols) form_q_cols(c, ch.ur, A, L.qr.tau, n, false);
-    else form_q(c, ch.ur, A, L.qr.tau, n);
+    else form_q(c, ch.ur, A, L.qr.tau, n, ch.w3);
     vec_copy(c, ch.dr, L.dtmp, n);
 }
 
@@ -801,7 +821,7 @@ DEVFN void stab_ul_from(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
     if (cols) {
         form_q_cols(c, ch.ul, A, L.qr.tau, n, true);
     } else {
-        form_q(c, ch.w1, A, L.qr.tau, n);
+        form_q(c, ch.w1, A, L.qr.tau, n, ch.w3);
         mat_copy(c, ch.ul, ch.w1, n, true);
     }
     vec_copy(c, ch.dl, L.dtmp, n);
@@ -847,13 +867,13 @@ DEVFN double stab_green(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
         solve_cols(c, ch.w2, upper ? ul : ur, !upper, A, L.qr.tau, L.dtmp, L.qr.perm, n);
     } else {
         mat_copy(c, ch.w1, upper ? ul : ur, n, !upper);
-        apply_qh_left(c, ch.w1, A, L.qr.tau, n);
+        apply_qh_left(c, ch.w1, A, L.qr.tau, n, ch.w3);
         for (int e = c.tid(); e < nn; e += c.nthreads()) {
             const int r = e / n;
             ch.w1[e] = ch.w1[e] * cd{1.0 / L.dtmp[r], 0.0};
         }
         c.sync();
-        trsm_upper_left(c, ch.w1, A, n);
+        trsm_upper_left(c, ch.w1, A, n, ch.w3);
         for (int e = c.tid(); e < nn; e += c.nthreads()) {
             const int r = e / n, k = e - r * n;
             ch.w2[L.qr.perm[r] * n + k] = ch.w1[e];

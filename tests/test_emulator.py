@@ -61,6 +61,21 @@ def test_emu_global_path_matches_oracle(emu_lib, case, delayed, monkeypatch):
     check_sweep_vs_oracle(emu_lib, case)
 
 
+@pytest.mark.parametrize("width", [20, 32])
+@pytest.mark.parametrize("case", [4, 5])
+def test_emu_wide_device_memory_updates(emu_lib, case, width, monkeypatch):
+    """Delayed-update buffers wider than 16 columns in device memory: the
+    decisions sum the correction over two blocks of 16, the flush runs its
+    MFMA k-steps over both."""
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (0, width))
+    check_sweep_vs_oracle(emu_lib, case)
+
+
+def test_wide_updates_rejected_on_the_lds_path(emu_lib):
+    with pytest.raises(RuntimeError, match="above 16 need the device-memory path"):
+        BatchedSolver([ModelSpec(solver_params(CASES[4]))], [0], lib=emu_lib, schedule=(1, 32))
+
+
 # the 12x12 goldens (N = 144, 288) run on the GPU only: minutes on the emulator;
 # of the two Ltrot = 400 single-channel pairing goldens the emulator runs one
 EMU_GOLDEN = [g for g in GOLDEN if "12x12" not in g and g != "pairing_traj_3x3_d0"]

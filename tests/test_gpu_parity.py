@@ -328,6 +328,15 @@ def test_local_update_variants_match_reference(hip_lib, name, width, monkeypatch
     check_golden(hip_lib, name)
 
 
+@pytest.mark.parametrize("width", [20, 32])
+@pytest.mark.parametrize("name", ["pairing_traj_6x6_c2", "nc_traj_9x9_c4", "pairing_traj_12x12_c5"])
+def test_wide_device_memory_updates_match_reference(hip_lib, name, width, monkeypatch):
+    """Delayed-update buffers wider than 16 columns (device-memory path only,
+    buffers in the chain's work matrices): the reference's trajectory."""
+    monkeypatch.setattr(BatchedSolver, "default_schedule", (0, width))
+    check_golden(hip_lib, name)
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_uneven_batch_equals_single_chains(hip_lib, mixed):
     check_uneven_batch(hip_lib, mixed)

@@ -153,6 +153,8 @@ EMU_CASES = [
           shift_loc=1.0), None),
     (dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=3, nly=3, ltrot=20, beta=2.0, nwrap=10, shift_loc=1.5),
      (0, 4)),                                                  # device-memory path, narrow delayed updates
+    # N = 36: the 72 x 72 block matrix takes the blocked reflector application and solve
+    (dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=6, nly=6, ltrot=10, beta=1.0, nwrap=5, shift_loc=1.5), None),
 ]
 
 
@@ -161,7 +163,8 @@ EMU_CASES = [
 def test_emu_tau_pass_matches_oracle(emu_lib, case, mode):
     d, schedule = EMU_CASES[case]
     rng = np.random.default_rng(3 + case)
-    _device_vs_oracle(emu_lib, d, mode, rng.normal(size=(d["ltrot"], 9, 2)), 4242 + case, schedule)
+    ns = d["nlx"] * d["nly"]
+    _device_vs_oracle(emu_lib, d, mode, rng.normal(size=(d["ltrot"], ns, 2)), 4242 + case, schedule)
 
 
 def _write_case(tmp_path, name):
@@ -236,14 +239,16 @@ def test_gpu_tau_pass_matches_oracle(hip_lib, case, mode):
 def test_gpu_tau_pass_in_an_uneven_batch(hip_lib):
     """Chains of several sizes and both paths in one launch give each chain's
     single-run time-displaced functions and drifts bit for bit."""
-    models = [solver_params(GPU_CASES[k][0]) for k in (0, 1, 3, 4)]
+    picks = (0, 1, 4, 5)               # NC 3x3, pairing 3x3, pairing 6x6 (LDS path), NC 9x9 (device memory)
+    models = [solver_params(GPU_CASES[k][0]) for k in picks]
+    sched = [(-1, -1), (-1, -1), (-1, -1), (0, -1)]
     chain_model = [0, 2, 1, 3, 0, 2]
     phis, seeds = [], []
     for c, k in enumerate(chain_model):
         ph, sd = conf_set_batch(models[k], [1000 + 17 * c])
         phis += list(ph)
         seeds += list(sd)
-    s = BatchedSolver([ModelSpec(p) for p in models], chain_model, lib=hip_lib)
+    s = BatchedSolver([ModelSpec(p) for p in models], chain_model, lib=hip_lib, schedule=sched)
     try:
         s.set_fields(phis, seeds)
         s.prepare()
@@ -253,7 +258,7 @@ def test_gpu_tau_pass_in_an_uneven_batch(hip_lib):
     finally:
         s.close()
     for c, k in enumerate(chain_model):
-        one = BatchedSolver([ModelSpec(models[k])], [0], lib=hip_lib)
+        one = BatchedSolver([ModelSpec(models[k])], [0], lib=hip_lib, schedule=[sched[k]])
         try:
             one.set_fields([phis[c]], [seeds[c]])
             one.prepare()
