@@ -1175,10 +1175,14 @@ DEVFN void measure_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, cons
 // Wrap_R (stabilization.f90:330-366), nt % Nwrap == 0, nt != 0.
 DEVFN void wrap_r_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
     const int n = m.n, st = nt / m.nwrap;
+    Timer tm;
+    tm.start(c);
     load_ud(c, ch.wa, ch.ur, ch.dr, n, false);
     stab_ur_from(c, m, ch, L, ch.wa);
+    tm.stop(c, ch, S_T_WQR);
     const double ld = stab_green(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, list_mat(ch.ullist, st, n),
                                  list_vec(ch.dllist, st, n), list_mat(ch.vllist, st, n), ch.wa, ch.wa);
+    tm.stop(c, ch, S_T_WGREEN);
     store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
     mat_copy(c, ch.g, ch.wa, n, false);
     store_right(c, m, ch, st);
@@ -1187,10 +1191,14 @@ DEVFN void wrap_r_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 // Wrap_L (stabilization.f90:292-328), nt % Nwrap == 0, nt != Ltrot.
 DEVFN void wrap_l_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
     const int n = m.n, st = nt / m.nwrap;
+    Timer tm;
+    tm.start(c);
     load_ud(c, ch.wa, ch.ul, ch.dl, n, true);
     stab_ul_from(c, m, ch, L, ch.wa);
+    tm.stop(c, ch, S_T_WQR);
     const double ld = stab_green(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
                                  list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, ch.wa, ch.wa);
+    tm.stop(c, ch, S_T_WGREEN);
     store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
     mat_copy(c, ch.g, ch.wa, n, false);
     store_left(c, m, ch, st);

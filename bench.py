@@ -227,24 +227,35 @@ def choose_schedule(args, p, nchains):
     return [tuple(int(v) for v in args.schedule.split(","))], None
 
 
-def run_phase(s, step, K, S, label):
+PHASE_SLOTS = {"site": 12, "kinetic": 13, "measure": 14, "wrap": 15, "hs": 16, "wrap_products": 17,
+               "wrap_stab_u": 18, "wrap_green": 19}
+
+
+def run_phase(s, step, K, S, label, phases=None):
     """max(K, S) steps of one form; the first K are the timed region.  Returns
     (per-chain S_SF(K) series [chains, max(K, S)], wall seconds of the K timed
-    steps, per-step kernel milliseconds reported by `step`)."""
+    steps, per-step kernel milliseconds reported by `step`).  phases: a dict
+    that receives the device timers of the K timed steps (ms per chain-sweep,
+    thread 0 of each chain's workgroup)."""
     n = max(K, S)
     sf = np.zeros((s.nchains, n))
     kms = []
+    tim = np.zeros(len(PHASE_SLOTS))
     wall_k = float("nan")
     progress(f"{label}: {K} timed steps, {n} steps for the statistics")
     t0 = time.perf_counter()
     for k in range(n):
         kms.append(step())
-        obs, _ = s.read_bin_raw()
+        obs, st = s.read_bin_raw()
+        if k < K:
+            tim += st[:, list(PHASE_SLOTS.values())].sum(axis=0)
         sf[:, k] = obs[:, 20] / obs[:, 32]
         if k + 1 == K:
             wall_k = time.perf_counter() - t0
         if (k + 1) % 50 == 0:
             progress(f"{label} step {k + 1}/{n}")
+    if phases is not None:
+        phases.update({name: float(v) * 1e3 / (K * s.nchains) for name, v in zip(PHASE_SLOTS, tim)})
     return sf, wall_k, kms
 
 
@@ -452,7 +463,8 @@ def run_plain(args, cfg, p, spec, nchains):
         s.sweep(1)
         return s.last_kernel_ms()
 
-    sf, wall, kms = run_phase(s, step, args.steps, args.stat_steps, "plain")
+    phases = {}
+    sf, wall, kms = run_phase(s, step, args.steps, args.stat_steps, "plain", phases)
     st = s.stats()
     a = statistics(sf, wall, args.steps, "plain")
     kern = "k_sweep_lds" if s.schedule()[0][0] == 1 else "k_sweep"
@@ -466,6 +478,7 @@ def run_plain(args, cfg, p, spec, nchains):
         "config": {"workload": cfg["name"], "chains": nchains, "matrix_n": p.n, "ltrot": p.ltrot,
                    "nwrap": p.nwrap, "value_is": "plain batched sweep (no global moves)", "plain": a,
                    "unstable_chains": int(np.sum(st[:, 4] >= UNSTABLE_DRIFT)), "kernel": kern,
+                   "phase_ms_per_chain_sweep": phases,
                    "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1],
                                 "autotune_ms_per_sweep": tuned.table if tuned else None}},
         "roofline": rf,
