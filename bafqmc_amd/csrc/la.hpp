@@ -72,6 +72,10 @@ DEV void stc(P p, int i, cd v) {
     p[i].i = v.i;
 }
 
+// Matrices larger than this (the device-memory path) run the paired-tile
+// GEMM variants.
+constexpr int kPairMinN = 96;
+
 // ------------------------------------------------------------ reductions
 DEV double wave_sum(const Ctx& c, double v) {
     for (int m = 32; m >= 1; m >>= 1) v += c.shfl_xor(v, m);
@@ -176,59 +180,144 @@ DEV void mfma_gemm(const Ctx& c, int n, LA la, LB lb, ST st) {
     c.sync();
 }
 
+// mfma_gemm with two tiles of one tile row per wavefront iteration, sharing
+// their A fragments, the loads of both issued together (for operands in device
+// memory); the same products per tile.
+template <class LA, class LB, class ST>
+DEV void mfma_gemm2(const Ctx& c, int n, LA la, LB lb, ST st) {
+    constexpr int kChunk = 4, kT = 2;
+    const int nt = (n + 15) >> 4, nk = (n + 3) >> 2, nh = (nt + kT - 1) / kT;
+    const int lane = c.lane();
+    const int li = lane & 15, lk = lane >> 4;
+    for (int t = c.wave(); t < nt * nh; t += c.nwaves()) {
+        const int ti = t / nh, tj0 = kT * (t - ti * nh);
+        const int ar = 16 * ti + li;
+        // three real products per complex one (Gauss): Re = sum ar br - sum ai bi,
+        // Im = sum (ar + ai)(br + bi) - sum ar br - sum ai bi
+        d4 rr[kT], ii[kT], ss[kT];
+#pragma unroll
+        for (int u = 0; u < kT; ++u) {
+            rr[u] = d4{0.0, 0.0, 0.0, 0.0};
+            ii[u] = rr[u];
+            ss[u] = rr[u];
+        }
+        for (int k0 = 0; k0 < nk; k0 += kChunk) {
+            cd a[kChunk], b[kT][kChunk];
+#pragma unroll
+            for (int v = 0; v < kChunk; ++v) {
+                const int k = 4 * (k0 + v) + lk;
+                a[v] = cd{0.0, 0.0};
+                if (k < n && ar < n) a[v] = la(ar, k);
+#pragma unroll
+                for (int u = 0; u < kT; ++u) {
+                    const int bc = 16 * (tj0 + u) + li;
+                    b[u][v] = cd{0.0, 0.0};
+                    if (k < n && bc < n) b[u][v] = lb(k, bc);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < kChunk; ++v) {
+                if (k0 + v < nk) {
+#pragma unroll
+                    for (int u = 0; u < kT; ++u) {
+                        if (tj0 + u >= nt) break;
+                        rr[u] = c.mfma(a[v].r, b[u][v].r, rr[u]);
+                        ii[u] = c.mfma(a[v].i, b[u][v].i, ii[u]);
+                        ss[u] = c.mfma(a[v].r + a[v].i, b[u][v].r + b[u][v].i, ss[u]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kT; ++u) {
+            if (tj0 + u >= nt) break;
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ti + mfma_row(lane, r), col = 16 * (tj0 + u) + li;
+                if (row < n && col < n) st(row, col, cd{rr[u][r] - ii[u][r], ss[u][r] - rr[u][r] - ii[u][r]});
+            }
+        }
+    }
+    c.sync();
+}
+
 DEVFN void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool opB, int n) {
-    mfma_gemm(
-        c, n, [&](int r, int k) { return load_op(A, opA, r, k, n); },
-        [&](int k, int j) { return load_op(B, opB, k, j, n); }, [&](int r, int j, cd v) { C[r * n + j] = v; });
+    auto la = [&](int r, int k) { return load_op(A, opA, r, k, n); };
+    auto lb = [&](int k, int j) { return load_op(B, opB, k, j, n); };
+    auto st = [&](int r, int j, cd v) { C[r * n + j] = v; };
+    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st);
+    else mfma_gemm(c, n, la, lb, st);
 }
 
 // C = E X (side 0) or X E (side 1) on FP64 MFMA 16x16x4, E real N x N in MFMA
 // operand order (zero padded; side 0: Ez[ti][kk][lane] = E[16 ti + lane % 16][4 kk + lane / 16],
 // side 1: Ez[tj][kk][lane] = E[4 kk + lane / 16][16 tj + lane % 16]), X and C
-// complex N x N in device memory.  The loads of eight k-steps are issued
-// together before their products (the operands come from L2/HBM).
+// complex N x N in device memory.  A wavefront computes two output tiles at a
+// time that share their E fragments (side 0: one tile row and two tile
+// columns; side 1 the transpose); the loads of eight k-steps of both are
+// issued together before their products (the operands come from L2/HBM).
 DEVFN void gemm_rc(const Ctx& c, cd* C, const double* Ez, const cd* X, int side, int n) {
-    const int nt = (n + 15) >> 4, nk = (n + 3) >> 2;
+    const int nt = (n + 15) >> 4, nk = (n + 3) >> 2, nh = (nt + 1) >> 1;
     const int lane = c.lane();
     const int li = lane & 15, lk = lane >> 4;
     auto Xg = as_global(X);
     auto Cg = as_global(C);
-    for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
-        const int ti = t / nt, tj = t - ti * nt;
-        const int ar = 16 * ti + li, bc = 16 * tj + li;
-        auto ez = as_global(Ez + (size_t)(side == 0 ? ti : tj) * nk * 64 + lane);
-        d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+    for (int t = c.wave(); t < nt * nh; t += c.nwaves()) {
+        const int sh = t / nh, p = t - sh * nh;        // shared tile index, pair of the other index
+        const bool two = 2 * p + 1 < nt;               // wavefront-uniform
+        int ti[2], tj[2];
+        for (int u = 0; u < 2; ++u) {
+            ti[u] = side == 0 ? sh : 2 * p + u;
+            tj[u] = side == 0 ? 2 * p + u : sh;
+        }
+        auto ez = as_global(Ez + (size_t)sh * nk * 64 + lane);
+        d4 accr[2], acci[2];
+        for (int u = 0; u < 2; ++u) {
+            accr[u] = d4{0.0, 0.0, 0.0, 0.0};
+            acci[u] = accr[u];
+        }
         for (int kk0 = 0; kk0 < nk; kk0 += 8) {
             double e[8];
-            cd x[8];
+            cd x[2][8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int kk = kk0 + u, k = 4 * kk + lk;
-                e[u] = 0.0;
-                x[u] = cd{0.0, 0.0};
-                if (kk < nk) e[u] = ez[(size_t)kk * 64];
-                if (side == 0) {
-                    if (k < n && bc < n) x[u] = ldc(Xg, k * n + bc);
-                } else {
-                    if (ar < n && k < n) x[u] = ldc(Xg, ar * n + k);
+            for (int v = 0; v < 8; ++v) {
+                const int kk = kk0 + v, k = 4 * kk + lk;
+                e[v] = 0.0;
+                if (kk < nk) e[v] = ez[(size_t)kk * 64];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    x[u][v] = cd{0.0, 0.0};
+                    const int ar = 16 * ti[u] + li, bc = 16 * tj[u] + li;
+                    if (u == 1 && !two) continue;
+                    if (side == 0) {
+                        if (k < n && bc < n) x[u][v] = ldc(Xg, k * n + bc);
+                    } else {
+                        if (ar < n && k < n) x[u][v] = ldc(Xg, ar * n + k);
+                    }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (kk0 + u < nk) {
-                    if (side == 0) {
-                        accr = c.mfma(e[u], x[u].r, accr);
-                        acci = c.mfma(e[u], x[u].i, acci);
-                    } else {
-                        accr = c.mfma(x[u].r, e[u], accr);
-                        acci = c.mfma(x[u].i, e[u], acci);
+            for (int v = 0; v < 8; ++v) {
+                if (kk0 + v < nk) {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        if (u == 1 && !two) break;
+                        if (side == 0) {
+                            accr[u] = c.mfma(e[v], x[u][v].r, accr[u]);
+                            acci[u] = c.mfma(e[v], x[u][v].i, acci[u]);
+                        } else {
+                            accr[u] = c.mfma(x[u][v].r, e[v], accr[u]);
+                            acci[u] = c.mfma(x[u][v].i, e[v], acci[u]);
+                        }
                     }
                 }
             }
         }
-        for (int r = 0; r < 4; ++r) {
-            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
-            if (row < n && col < n) stc(Cg, row * n + col, cd{accr[r], acci[r]});
+        for (int u = 0; u < 2; ++u) {
+            if (u == 1 && !two) break;
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ti[u] + mfma_row(lane, r), col = 16 * tj[u] + li;
+                if (row < n && col < n) stc(Cg, row * n + col, cd{accr[u][r], acci[u][r]});
+            }
         }
     }
     c.sync();
@@ -815,17 +904,20 @@ DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n, cd* work = n
 
 // Y = V * X[perm, :]   (ZLAPMR forward + ZTRMM 'L','U','N'); V = upper triangle of A.
 DEVFN void trmm_perm_rows(const Ctx& c, cd* Y, const cd* A, const cd* X, const int* perm, int n) {
-    mfma_gemm(
-        c, n, [&](int r, int k) { return csel(k >= r, A[r * n + k], cd{0.0, 0.0}); },
-        [&](int k, int j) { return X[perm[k] * n + j]; }, [&](int r, int j, cd v) { Y[r * n + j] = v; });
+    auto la = [&](int r, int k) { return csel(k >= r, A[r * n + k], cd{0.0, 0.0}); };
+    auto lb = [&](int k, int j) { return X[perm[k] * n + j]; };
+    auto st = [&](int r, int j, cd v) { Y[r * n + j] = v; };
+    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st);
+    else mfma_gemm(c, n, la, lb, st);
 }
 
 // Y = X[:, perm] * V^H  (ZLAPMT forward + ZTRMM 'R','U','C').
 DEVFN void perm_cols_times_vh(const Ctx& c, cd* Y, const cd* X, const cd* A, const int* perm, int n) {
-    mfma_gemm(
-        c, n, [&](int r, int k) { return X[r * n + perm[k]]; },
-        [&](int k, int j) { return csel(k >= j, conj(A[j * n + k]), cd{0.0, 0.0}); },
-        [&](int r, int j, cd v) { Y[r * n + j] = v; });
+    auto la = [&](int r, int k) { return X[r * n + perm[k]]; };
+    auto lb = [&](int k, int j) { return csel(k >= j, conj(A[j * n + k]), cd{0.0, 0.0}); };
+    auto st = [&](int r, int j, cd v) { Y[r * n + j] = v; };
+    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st);
+    else mfma_gemm(c, n, la, lb, st);
 }
 
 // ------------------------------------ column-parallel QR family (LDS, n <= 96)

@@ -27,7 +27,7 @@ def test_emu_mfma_layout(emu_lib):
 
 def test_emu_gemm(emu_lib):
     rng = np.random.default_rng(3)
-    for n in (5, 18, 33):
+    for n in (5, 18, 33, 150):
         a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
         b = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
         for oa in (0, 1):

@@ -65,7 +65,7 @@ def test_mfma_layout(hip_lib):
     check_mfma_layout(hip_lib)
 
 
-@pytest.mark.parametrize("n", [9, 18, 36, 72, 81, 144])
+@pytest.mark.parametrize("n", [9, 18, 36, 72, 81, 144, 150, 288])
 def test_device_gemm(hip_lib, n):
     rng = np.random.default_rng(n)
     a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
