@@ -132,7 +132,8 @@ def _device_vs_oracle(lib, d, mode, phi, seed, schedule=None, tol=1e-10):
         if mode == TAU_REFERENCE:
             np.testing.assert_allclose(dev, ch.tau_xmax, rtol=1e-8)
         else:
-            assert np.all(dev < 1e-8) and np.all(ch.tau_xmax < 1e-8), (dev, ch.tau_xmax)
+            # rounding, summed over the N x N entries (1e-8 at the 12x12 pairing lattice)
+            assert np.all(dev < 1e-6) and np.all(ch.tau_xmax < 1e-6), (dev, ch.tau_xmax)
         assert st["tau_nwrap"] == p.nst
         # the chain is untouched by the pass: the same fields, seeds and estimators
         f1, s1 = s.get_fields()
@@ -220,6 +221,10 @@ GPU_CASES = EMU_CASES + [
     # NC 9x9 (N = 81) on the device-memory path
     (dict(pairing=False, u1=1.0, u2=0.0, mu=-5.0, nlx=9, nly=9, ltrot=20, beta=2.0, nwrap=10, shift_loc=1.5),
      (0, -1)),
+    # the 12x12 pairing lattice (N = 288, device-memory path; its 576 x 576 block
+    # matrix takes the QR's general loop and the blocked solve)
+    (dict(pairing=True, u1=1.0, u2=-0.5, mu=-5.0, rdelta=0.3, nlx=12, nly=12, ltrot=4, beta=0.4, nwrap=2,
+          shift_loc=1.0), None),
 ]
 
 
