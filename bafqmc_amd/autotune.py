@@ -2,7 +2,8 @@
 
 For every model of a batch it times one sweep of a sample of chains under each
 execution schedule the library accepts (include/bafqmc.h bafq_schedule: G in
-LDS or in device memory, delayed-update width 16/12/8/4 or per-site updates)
+LDS or in device memory, delayed-update width 16/12/8/4 (32 with G in device
+memory) or per-site updates)
 and keeps the fastest.
 Every schedule computes the same Markov chain, so the choice changes time
 only.  Optionally it also chooses the stabilisation interval Nwrap among the
@@ -28,7 +29,7 @@ from .analysis import chain_seeds
 from .model import ModelSpec, conf_set_batch
 from .solver import BatchedSolver, SolverError
 
-SCHEDULES = [(1, 16), (1, 12), (1, 8), (1, 4), (1, 0), (0, 16), (0, 8), (0, 4), (0, 0)]
+SCHEDULES = [(1, 16), (1, 12), (1, 8), (1, 4), (1, 0), (0, 32), (0, 16), (0, 8), (0, 4), (0, 0)]
 
 
 @dataclasses.dataclass

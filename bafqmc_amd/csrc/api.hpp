@@ -261,7 +261,7 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
             // entries each, so dk <= N)
             auto fits = [&](int dk) {
                 return path == 1 ? lds_bytes(m.n, lds_waves, true, dk) <= limit
-                                 : lds_bytes(m.n, glb_waves, false, dk) <= limit;
+                                 : lds_bytes(m.n, glb_waves, false, dk, dk) <= limit;
             };
             m.dk = 0;
             m.uw_glb = 0;
@@ -286,9 +286,15 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
                     }
                 if (path == 0 && m.dk < 16 && m.n >= 16) {
                     // G streams from HBM at every flush: the widest buffers, in device memory
-                    m.dk = 16;
+                    // (32 columns: blocks of 32 rows for the submatrix updates)
+                    m.dk = m.n >= kMaxDkGlb ? kMaxDkGlb : 16;
                     m.uw_glb = 1;
                 }
+            }
+            // the device-memory path keeps the submatrix of each block of sites in LDS
+            if (path == 0 && lds_bytes(m.n, glb_waves, false, m.uw_glb ? 0 : m.dk, m.dk) > limit) {
+                *out = h;
+                return fail(h, "schedule's delayed-update width does not fit in LDS");
             }
             h->schedule.push_back(bafq_schedule{path, m.dk});
         }
@@ -411,7 +417,7 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
         const int path = h->schedule[chain_model[c]].path;
         pids[path].push_back(c);
         const size_t lb = path == P_LDS ? lds_bytes(m.n, rt_lds_block_threads() / 64, true, m.dk)
-                                        : lds_bytes(m.n, nw_glb, false, m.uw_glb ? 0 : m.dk);
+                                        : lds_bytes(m.n, nw_glb, false, m.uw_glb ? 0 : m.dk, m.dk);
         h->max_lds[path] = std::max(h->max_lds[path], lb + kSlotBytes);
     }
     rt_h2d(h->d_chains, h->chains.data(), sizeof(ChainDev) * n_chains);

@@ -146,10 +146,17 @@ struct Lds {
     cd* mat;           // n x n (LDS path only)
     cd* du;            // dk x n   delayed-update factors, U transposed: G -= du^T dw (LDS path, dk > 0)
     cd* dw;            // dk x n
+    // submatrix updates (device-memory path, sb = dk > 0): G[R,R] of a block of
+    // rows R (|R| <= sb) in two ping-pong buffers, its value at the block start,
+    // the accepted rows' local indices and changes
+    cd* sbuf;          // 2 x sb x sb
+    cd* sg0;           // sb x sb
+    int* sloc;         // sb
+    cd* sd;            // sb
 };
 
 HOSTDEV size_t lds_align(size_t b) { return (b + 15) & ~(size_t)15; }
-HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat, int dk = 0) {
+HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat, int dk = 0, int sb = 0) {
     size_t b = 0;
     b += lds_align((size_t)n * sizeof(cd));           // tau
     b += lds_align((size_t)n * sizeof(int) + 16);     // perm
@@ -163,10 +170,16 @@ HOSTDEV size_t lds_bytes(int n, int nwaves, bool with_mat, int dk = 0) {
     b += lds_align((size_t)n * sizeof(Proposal));
     if (with_mat) b += lds_align((size_t)n * n * sizeof(cd));
     b += 2 * lds_align((size_t)n * dk * sizeof(cd));
+    if (sb > 0) {
+        b += lds_align((size_t)2 * sb * sb * sizeof(cd));
+        b += lds_align((size_t)sb * sb * sizeof(cd));
+        b += lds_align((size_t)sb * sizeof(int));
+        b += lds_align((size_t)sb * sizeof(cd));
+    }
     return b + 64;
 }
 
-DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0) {
+DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0, int sb = 0) {
     Lds L;
     char* p = base;
     auto take = [&](size_t bytes) {
@@ -190,6 +203,10 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0) {
     L.mat = with_mat ? (cd*)take((size_t)n * n * sizeof(cd)) : nullptr;
     L.du = dk > 0 ? (cd*)take((size_t)n * dk * sizeof(cd)) : nullptr;
     L.dw = dk > 0 ? (cd*)take((size_t)n * dk * sizeof(cd)) : nullptr;
+    L.sbuf = sb > 0 ? (cd*)take((size_t)2 * sb * sb * sizeof(cd)) : nullptr;
+    L.sg0 = sb > 0 ? (cd*)take((size_t)sb * sb * sizeof(cd)) : nullptr;
+    L.sloc = sb > 0 ? (int*)take((size_t)sb * sizeof(int)) : nullptr;
+    L.sd = sb > 0 ? (cd*)take((size_t)sb * sizeof(cd)) : nullptr;
     return L;
 }
 
@@ -462,9 +479,11 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 
 // G0 -= U[:, :cols] W[:cols, :] on FP64 MFMA 16x16x4 (each output tile owned by
 // one wavefront); G in LDS or in device memory, U and W in LDS or device
-// memory; cols <= kMaxCols (16 or 32: blocks of 16 inner indices in turn).
+// memory; cols <= kMaxCols (16 or 32).  A tile's operands for all its k-steps
+// and its tile of G0 are loaded before the products (one round trip per tile).
 template <int kMaxCols = 16, class GP, class UP>
 DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols) {
+    constexpr int kQ = kMaxCols / 4;
     n = c.uniform(n);                 // scalar tests of the k-steps taken
     cols = c.uniform(cols);
     const int nt = (n + 15) >> 4;
@@ -472,38 +491,30 @@ DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols) {
     for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
         const int ti = t / nt, tj = t - ti * nt;
         const int ar = 16 * ti + li, bc = 16 * tj + li;
-        cd g[4];
+        cd a[kQ], b[kQ], g[4];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int kk = 4 * q + lk;
+            a[q] = cd{0.0, 0.0};
+            b[q] = cd{0.0, 0.0};
+            if (ar < n && kk < cols) a[q] = ldc(Ut, kk * n + ar);
+            if (bc < n && kk < cols) b[q] = ldc(W, kk * n + bc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
+            g[r] = cd{0.0, 0.0};
+            if (row < n && col < n) g[r] = ldc(G, row * n + col);
+        }
         // three real products per complex one (Gauss): Re = ar br - ai bi,
         // Im = (ar + ai)(br + bi) - ar br - ai bi; three independent chains
         d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ss = rr;
 #pragma unroll
-        for (int kb = 0; kb < kMaxCols; kb += 16) {
-            if (kMaxCols > 16 && kb >= cols) break;
-            // operands (and with the first block the tile of G0) are loaded before the products
-            cd a[4], b[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int kk = kb + 4 * q + lk;
-                a[q] = cd{0.0, 0.0};
-                b[q] = cd{0.0, 0.0};
-                if (ar < n && kk < cols) a[q] = ldc(Ut, kk * n + ar);
-                if (bc < n && kk < cols) b[q] = ldc(W, kk * n + bc);
-            }
-            if (kb == 0) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * ti + mfma_row(lane, r), col = 16 * tj + li;
-                    g[r] = cd{0.0, 0.0};
-                    if (row < n && col < n) g[r] = ldc(G, row * n + col);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (kb + 4 * q < cols) {
-                    rr = c.mfma(a[q].r, b[q].r, rr);
-                    ii = c.mfma(a[q].i, b[q].i, ii);
-                    ss = c.mfma(a[q].r + a[q].i, b[q].r + b[q].i, ss);
-                }
+        for (int q = 0; q < kQ; ++q) {
+            if (4 * q < cols) {
+                rr = c.mfma(a[q].r, b[q].r, rr);
+                ii = c.mfma(a[q].i, b[q].i, ii);
+                ss = c.mfma(a[q].r + a[q].i, b[q].r + b[q].i, ss);
             }
         }
 #pragma unroll
@@ -722,6 +733,243 @@ DEVFN void site_loop_delayed_glb(const Ctx& c, const ModelDev& m, const ChainDev
                                  int nf, bool ascending, int& seed) {
     if (m.uw_glb) site_loop_delayed_t<true, true>(c, m, ch, L, G, t, nf, ascending, seed);
     else site_loop_delayed_t<true, false>(c, m, ch, L, G, t, nf, ascending, seed);
+}
+
+// U = G0[:, A] X for the accepted rows of a block (device-memory path): Ut[j][x]
+// = sum_a G[x][row_a] X[a][j], x < n, j < na, on FP64 MFMA 16x16x4 (complex by
+// three real products); X (na x na, leading dimension sb) in LDS, the rows
+// row_a = grow(loc[a]) of G gathered from device memory.
+template <class GP, class UP, class RowOf>
+DEV void sub_form_u(const Ctx& c, UP Ut, GP G, LDS_AS const cd* X, int sb, LDS_AS const int* loc, int n, int na,
+                    RowOf grow) {
+    const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
+    const int ntx = (n + 15) >> 4, ntj = (na + 15) >> 4, nk = (na + 3) >> 2;
+    for (int t = c.wave(); t < ntx * ntj; t += c.nwaves()) {
+        const int tx = t / ntj, tj = t - tx * ntj;
+        const int ar = 16 * tx + li, bc = 16 * tj + li;
+        d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ss = rr;
+        for (int k0 = 0; k0 < nk; k0 += 4) {
+            cd a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = 4 * (k0 + u) + lk;
+                a[u] = cd{0.0, 0.0};
+                b[u] = cd{0.0, 0.0};
+                if (k < na && ar < n) a[u] = ldc(G, ar * n + grow(loc[k]));
+                if (k < na && bc < na) b[u] = ldc(X, k * sb + bc);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k0 + u < nk) {
+                    rr = c.mfma(a[u].r, b[u].r, rr);
+                    ii = c.mfma(a[u].i, b[u].i, ii);
+                    ss = c.mfma(a[u].r + a[u].i, b[u].r + b[u].i, ss);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int x = 16 * tx + mfma_row(lane, r), j = 16 * tj + li;
+            if (x < n && j < na) stc(Ut, j * n + x, cd{rr[r] - ii[r], ss[r] - rr[r] - ii[r]});
+        }
+    }
+}
+
+// Local updates of one slice and channel on the device-memory path by
+// submatrix delayed updates: the same decisions, RNG use and result as
+// site_loop_t (LocalU_metro + LocalU_prop_L/R).  The sites are taken in blocks
+// of sb / ksec in visiting order (sb = dk).  For a block with rows R (each site
+// its row s0 = i, and for pairing s1 = Ns + i):
+//   * G[R,R] is gathered into LDS and kept as G0[R,R];
+//   * each site is decided from the current G[R,R] (every wavefront decides it
+//     itself, so a rejected site costs no barrier), and an accepted site
+//     updates only G[R,R] <- G[R,R] - G[R,S] M (1 - G)[S,R] (ping-pong
+//     buffers, one barrier);
+//   * at the end of the block the accepted rows A with their changes Delta_A
+//     update G in one rank-|A| step, G <- G - G0[:,A] X (1 - G0)[A,:] with
+//     X = (I + Delta_A (1 - G0)[A,A])^{-1} Delta_A (Woodbury for the combined
+//     change of the accepted fields; the sequence of the sites' own updates
+//     G <- G - G[:,S] M (1 - G)[S,:], M = (1 + Delta_S (1 - G)[S,S])^{-1} Delta_S,
+//     gives the same G).  The inverse is a Gauss-Jordan sweep in acceptance
+//     order, whose pivots are the accepted sites' P (nonzero), U = G0[:,A] X
+//     runs on the matrix cores, then the flush G -= U W with W = (1 - G0)[A,:].
+// The per-site work is O(|R|^2) in LDS instead of O(N |A|) in device memory.
+template <bool kUW>
+DEV void site_loop_sub_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t, int nf,
+                         bool ascending, int& seed) {
+    const int n = m.n, ns = m.ns, sb = m.dk;
+    const bool pairing = m.kind == 1;
+    const int ksec = pairing ? 2 : 1;
+    const int bsites = sb / ksec;
+    const int tid = c.tid(), nth = c.nthreads();
+    auto G = as_global(G_);
+    cd* const du = kUW ? ch.w2 : L.du;
+    cd* const dw = kUW ? ch.w3 : L.dw;
+    auto Ut = [&] {
+        if constexpr (kUW) return as_global(du);
+        else return as_lds(du);
+    }();
+    auto W = [&] {
+        if constexpr (kUW) return as_global(dw);
+        else return as_lds(dw);
+    }();
+    double* phi_t = ch.phi + (size_t)t * ns * 2;
+    make_proposals(c, m, L, phi_t, nf, ascending, seed);
+    seed = lcg_jump(seed, 2 * ns);
+    auto props = as_lds(L.prop);
+    LDS_AS cd* const g0 = as_lds(L.sg0);
+    LDS_AS int* const loc = as_lds(L.sloc);
+    LDS_AS cd* const dl = as_lds(L.sd);
+    int nacc = 0;
+    for (int q0 = 0; q0 < ns; q0 += bsites) {
+        const int nbs = ns - q0 < bsites ? ns - q0 : bsites;
+        const int nr = nbs * ksec;
+        // global row of local row lr of R: sector lr % ksec of the block's site lr / ksec
+        auto grow = [&](int lr) {
+            const int qq = lr / ksec, sec = lr - qq * ksec, q = q0 + qq;
+            return sec * ns + (ascending ? q : ns - 1 - q);
+        };
+        LDS_AS cd* cur = as_lds(L.sbuf);
+        LDS_AS cd* nxt = cur + sb * sb;
+        c.sync();                                    // the previous block's readers are done
+        for (int e = tid; e < nr * nr; e += nth) {
+            const int r = e / nr, k = e - r * nr;
+            const cd v = ldc(G, grow(r) * n + grow(k));
+            stc(g0, r * sb + k, v);
+            stc(cur, r * sb + k, v);
+        }
+        c.sync();
+        int na = 0;
+        for (int qq = 0; qq < nbs; ++qq) {
+            const int q = q0 + qq;
+            const int i = ascending ? q : ns - 1 - q;
+            const int l0 = ksec * qq, l1 = l0 + 1;
+            Proposal P;
+            {
+                auto pp = props + i;
+                P.phi_new = pp->phi_new;
+                P.ratio0 = pp->ratio0;
+                P.u = pp->u;
+                P.d0 = cd{pp->d0.r, pp->d0.i};
+                P.d1 = cd{pp->d1.r, pp->d1.i};
+            }
+            const cd g00 = ldc(cur, l0 * sb + l0);
+            bool acc;
+            cd p00, p01 = cd{0.0, 0.0}, p10 = cd{0.0, 0.0}, p11 = cd{1.0, 0.0}, det;
+            if (!pairing) {
+                det = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
+                p00 = det;
+                acc = fabs(P.ratio0 / abs2(det)) > P.u;
+            } else {
+                const cd g01 = ldc(cur, l0 * sb + l1);
+                const cd g10 = ldc(cur, l1 * sb + l0);
+                const cd g11 = ldc(cur, l1 * sb + l1);
+                p00 = cd{1.0, 0.0} + P.d0 * (cd{1.0, 0.0} - g00);
+                p01 = -(P.d0 * g01);
+                p10 = -(P.d1 * g10);
+                p11 = cd{1.0, 0.0} + P.d1 * (cd{1.0, 0.0} - g11);
+                det = p00 * p11 - p01 * p10;
+                acc = fabs(P.ratio0 / sqrt(abs2(det))) > P.u;
+            }
+            if (!acc) continue;
+            ++nacc;
+            cd m00, m01 = cd{0.0, 0.0}, m10 = cd{0.0, 0.0}, m11 = cd{0.0, 0.0};
+            const cd idet = cinv(det);
+            if (!pairing) {
+                m00 = idet * P.d0;
+            } else {
+                m00 = (p11 * idet) * P.d0;
+                m01 = (-(p01 * idet)) * P.d1;
+                m10 = (-(p10 * idet)) * P.d0;
+                m11 = (p00 * idet) * P.d1;
+            }
+            if (tid == 0) {
+                (props + i)->pad = 1.0;                // accepted: phi is written after the loop
+                loc[na] = l0;
+                stc(dl, na, P.d0);
+                if (pairing) {
+                    loc[na + 1] = l1;
+                    stc(dl, na + 1, P.d1);
+                }
+            }
+            // G[R,R] <- G[R,R] - G[R,S] M (1 - G)[S,R]
+            for (int e = tid; e < nr * nr; e += nth) {
+                const int r = e / nr, k = e - r * nr;
+                const cd h0 = cd{l0 == k ? 1.0 : 0.0, 0.0} - ldc(cur, l0 * sb + k);
+                cd v = ldc(cur, r * sb + k) - ldc(cur, r * sb + l0) * (m00 * h0);
+                if (pairing) {
+                    const cd h1 = cd{l1 == k ? 1.0 : 0.0, 0.0} - ldc(cur, l1 * sb + k);
+                    v = v - ldc(cur, r * sb + l0) * (m01 * h1) - ldc(cur, r * sb + l1) * (m10 * h0 + m11 * h1);
+                }
+                stc(nxt, r * sb + k, v);
+            }
+            na += ksec;
+            c.sync();
+            LDS_AS cd* const tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+        }
+        if (na == 0) continue;
+        // ---- X = (I + Delta_A (1 - G0)[A,A])^{-1} Delta_A by a Gauss-Jordan sweep
+        c.sync();                                    // every wavefront's decisions of the block are done
+        LDS_AS cd* A = as_lds(L.sbuf);
+        LDS_AS cd* B = A + sb * sb;
+        for (int e = tid; e < na * na; e += nth) {
+            const int a = e / na, b = e - a * na;
+            const int la = loc[a], lb = loc[b];
+            const cd h = cd{la == lb ? 1.0 : 0.0, 0.0} - ldc(g0, la * sb + lb);
+            stc(A, a * sb + b, cd{a == b ? 1.0 : 0.0, 0.0} + ldc(dl, a) * h);
+        }
+        c.sync();
+        for (int p = 0; p < na; ++p) {
+            const cd ip = cinv(ldc(A, p * sb + p));
+            for (int e = tid; e < na * na; e += nth) {
+                const int a = e / na, b = e - a * na;
+                cd v;
+                if (a == p && b == p) v = ip;
+                else if (a == p) v = ldc(A, p * sb + b) * ip;
+                else if (b == p) v = -(ldc(A, a * sb + p) * ip);
+                else v = ldc(A, a * sb + b) - ldc(A, a * sb + p) * (ldc(A, p * sb + b) * ip);
+                stc(B, a * sb + b, v);
+            }
+            c.sync();
+            LDS_AS cd* const tmp = A;
+            A = B;
+            B = tmp;
+        }
+        // X = inverse Delta_A (columns scaled), into B
+        for (int e = tid; e < na * na; e += nth) {
+            const int a = e / na, b = e - a * na;
+            stc(B, a * sb + b, ldc(A, a * sb + b) * ldc(dl, b));
+        }
+        c.sync();
+        // ---- U = G0[:,A] X and W = (1 - G0)[A,:] from G (still G0), then the flush
+        sub_form_u(c, Ut, G, B, sb, loc, n, na, grow);
+        for (int e = tid; e < na * n; e += nth) {
+            const int j = e / n, x = e - j * n;
+            const int row = grow(loc[j]);
+            const cd g = ldc(G, row * n + x);
+            stc(W, j * n + x, cd{(x == row ? 1.0 : 0.0) - g.r, -g.i});
+        }
+        c.sync();
+        if (kUW) delayed_flush_glb_uw(c, G_, du, dw, n, na);
+        else delayed_flush_glb(c, G_, du, dw, n, na);
+    }
+    for (int i = tid; i < ns; i += nth) {
+        auto pp = props + i;
+        if (pp->pad != 0.0) phi_t[i * 2 + nf - 1] = pp->phi_new;
+    }
+    if (tid == 0) {
+        ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
+        ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
+    }
+    c.sync();
+}
+
+DEVFN void site_loop_sub_glb(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t, int nf,
+                             bool ascending, int& seed) {
+    if (m.uw_glb) site_loop_sub_t<true>(c, m, ch, L, G, t, nf, ascending, seed);
+    else site_loop_sub_t<false>(c, m, ch, L, G, t, nf, ascending, seed);
 }
 
 // ---------------------------------------------------------------- MFMA GEMM in LDS
@@ -1257,7 +1505,7 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
             if (!m.active[nf - 1]) continue;
             if (grad) grad_extract(c, m, phi_t, as_global(ch.g), t, nf, grad);
             else if (frozen) {}
-            else if (m.dk > 0) site_loop_delayed_glb(c, m, ch, L, ch.g, t, nf, false, seed);
+            else if (m.dk > 0) site_loop_sub_glb(c, m, ch, L, ch.g, t, nf, false, seed);
             else site_loop(c, m, ch, L, ch.g, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
@@ -1284,7 +1532,7 @@ DEV void sweep_right_global(const Ctx& c, const ModelDev& m, const ChainDev& ch,
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.g, n, L.f, L.finv);      // G <- e^{V} G e^{-V} (old fields)
             tm.stop(c, ch, S_T_HS);
-            if (m.dk > 0) site_loop_delayed_glb(c, m, ch, L, ch.g, t, nf, true, seed);
+            if (m.dk > 0) site_loop_sub_glb(c, m, ch, L, ch.g, t, nf, true, seed);
             else site_loop(c, m, ch, L, ch.g, t, nf, true, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
@@ -1495,7 +1743,7 @@ DEV void body_prepare(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk, kLds ? 0 : m.dk);
         if (kLds) prepare_lds(c, m, ch, L);
         else prepare_global(c, m, ch, L);
     }
@@ -1509,7 +1757,7 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk, kLds ? 0 : m.dk);
         int seed = *ch.seed;
         if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
         for (int s = 0; s < a.nsweep; ++s) {
@@ -1536,7 +1784,7 @@ DEV void body_grad(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk, kLds ? 0 : m.dk);
         int seed = *ch.seed;
         double* grad = a.grad + (ch.phi - a.phi0);
         if (kLds) {
@@ -1616,7 +1864,7 @@ DEV void body_measure(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
-        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk);
+        const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk, kLds ? 0 : m.dk);
         int seed = *ch.seed;
         if (a.measure == 3) {
             // the Green's function as it is (G(0) = G(beta) after a rebuild or a sweep)
