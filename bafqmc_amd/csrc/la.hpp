@@ -325,7 +325,8 @@ DEVFN void gemm_rc(const Ctx& c, cd* C, const double* Ez, const cd* X, int side,
 
 // ------------------------------------------------------ pivoted QR (ZGEQP3)
 // Scratch in LDS: tau[n] (cd), perm[n] (int), nrm[n] (double), bcast (>= 8
-// doubles), v[n] (cd, the current reflector).
+// doubles), v[2n] (cd: the current reflector, then the device-memory QR's
+// diagonal candidates).
 struct QrScratch {
     cd* tau;
     int* perm;
@@ -343,7 +344,8 @@ constexpr int kGroup = 8;   // lanes cooperating on one column
 // once; v comes from LDS.  With NG = kGroup the arithmetic and its order are
 // those of the two-pass loop of qrp_apply_loop.
 template <int R, int NG>
-DEV void qrp_apply_regs(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, LDS_AS double* nrm, cd ctau) {
+DEV void qrp_apply_regs(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, LDS_AS double* nrm, cd ctau,
+                        LDS_AS cd* alpha) {
     constexpr int kGroup = NG;
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
@@ -379,6 +381,7 @@ DEV void qrp_apply_regs(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, L
                 const cd y = x[i] - ldc(v, r) * t;
                 if (ok) A[r * n + j] = y;
                 if (r > k) a += abs2(y);
+                if (ok && r == k + 1) stc(alpha, j, y);      // the next step's diagonal entry
             }
         }
         a = group_sum(c, a, kGroup);
@@ -386,7 +389,8 @@ DEV void qrp_apply_regs(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, L
     }
 }
 
-DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, LDS_AS double* nrm, cd ctau) {
+DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, LDS_AS double* nrm, cd ctau,
+                        LDS_AS cd* alpha) {
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     for (int jb = k + 1; jb < n; jb += ngrp) {
@@ -409,6 +413,7 @@ DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, L
                 const cd x = A[r * n + j] - ldc(v, r) * t;
                 A[r * n + j] = x;
                 if (r > k) a += abs2(x);
+                if (r == k + 1) stc(alpha, j, x);
             }
         }
         a = group_sum(c, a, kGroup);
@@ -422,18 +427,24 @@ DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, L
 // D[k] = |R_kk|, perm[k] = original index of column k, tau[k] the scalar
 // factors with H(k) = I - tau_k v_k v_k^H, Q = H(0) H(1) ... H(n-1).
 // Column norms are recomputed exactly after every step (no downdating).
+// Two barriers per step: every wavefront finds the pivot itself from the
+// norms in LDS, and every thread computes the step's ZLARFG scalars itself
+// from the pivot column's norm and its diagonal entry, which the previous
+// step's update left in LDS (alpha); the column swap, the reflector's scaling
+// and its copy to LDS are one pass over the rows.  The scratch's reflector
+// buffer v holds 2n entries: v[0, n) the reflector, v[n, 2n) alpha.
 DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
     // the scratch arrays are in LDS: typed pointers held in registers, so that
     // stores through A (which may alias any generic address) do not force
     // their reloading
     LDS_AS double* const nrm = as_lds(s.nrm);
     LDS_AS int* const perm = as_lds(s.perm);
-    LDS_AS double* const bcast = as_lds(s.bcast);
     LDS_AS cd* const tau = as_lds(s.tau);
     LDS_AS cd* const vs = as_lds(s.v);
+    LDS_AS cd* const alpha = vs + n;
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
-    // initial squared column norms
+    // initial squared column norms and diagonal candidates
     for (int jb = 0; jb < n; jb += ngrp) {
         const int j = jb + grp;
         double a = 0.0;
@@ -443,91 +454,74 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
         if (j < n && gl == 0) {
             nrm[j] = a;
             perm[j] = j;
+            stc(alpha, j, A[j]);                 // row 0 of column j
         }
     }
     c.sync();
     for (int k = 0; k < n; ++k) {
-        // ---- pivot: first index of the largest remaining column norm
-        if (c.wave() == 0) {
-            double best = -1.0;
-            int bi = n;
-            for (int jb = k; jb < n; jb += 64) {
-                const int j = jb + c.lane();
-                if (j < n) {
-                    // a NaN norm (a chain that has blown up) wins, so the pivot is always a column
-                    const double v = nrm[j] == nrm[j] ? nrm[j] : INFINITY;
-                    if (v > best) { best = v; bi = j; }
-                }
-            }
-            for (int m = 32; m >= 1; m >>= 1) {
-                const double ob = c.shfl_xor(best, m);
-                const int oi = c.shfl_xor_i(bi, m);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-            }
-            if (c.lane() == 0) bcast[0] = (double)bi;
-        }
-        c.sync();
-        const int p = (int)bcast[0];
-        if (p != k) {
-            for (int r = tid; r < n; r += nth) {
-                const cd t = A[r * n + k];
-                A[r * n + k] = A[r * n + p];
-                A[r * n + p] = t;
-            }
-            if (tid == 0) {
-                const int ti = perm[k]; perm[k] = perm[p]; perm[p] = ti;
-                const double tn = nrm[k]; nrm[k] = nrm[p]; nrm[p] = tn;
+        // ---- pivot: first index of the largest remaining column norm (every wavefront)
+        double best = -1.0;
+        int bi = n;
+        for (int jb = k; jb < n; jb += 64) {
+            const int j = jb + c.lane();
+            if (j < n) {
+                // a NaN norm (a chain that has blown up) wins, so the pivot is always a column
+                const double v = nrm[j] == nrm[j] ? nrm[j] : INFINITY;
+                if (v > best) { best = v; bi = j; }
             }
         }
-        c.sync();
-        // ---- Householder reflector for column k (ZLARFG conventions)
-        if (c.wave() == 0) {
-            double x2 = 0.0;
-            for (int rb = k + 1; rb < n; rb += 64) {
-                const int r = rb + c.lane();
-                if (r < n) x2 += abs2(A[r * n + k]);
-            }
-            x2 = wave_sum(c, x2);
-            if (c.lane() == 0) {
-                const cd alpha = A[k * n + k];
-                double beta, tr, ti;
-                cd scale;
-                if (x2 == 0.0 && alpha.i == 0.0) {
-                    tr = 0.0; ti = 0.0; beta = alpha.r; scale = cd{0.0, 0.0};
-                } else {
-                    const double nr = sqrt(alpha.r * alpha.r + alpha.i * alpha.i + x2);
-                    beta = alpha.r >= 0.0 ? -nr : nr;
-                    tr = (beta - alpha.r) / beta;
-                    ti = -alpha.i / beta;
-                    scale = cinv(alpha - cd{beta, 0.0});
-                }
-                stc(tau, k, cd{tr, ti});
-                bcast[1] = scale.r;
-                bcast[2] = scale.i;
-                bcast[3] = beta;
+        for (int m = 32; m >= 1; m >>= 1) {
+            const double ob = c.shfl_xor(best, m);
+            const int oi = c.shfl_xor_i(bi, m);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        const int p = bi;
+        // ---- Householder reflector for the pivot column (ZLARFG conventions):
+        // beta = -sign(Re alpha) |x|, |x|^2 = the column's norm below row k - 1
+        const cd al = ldc(alpha, p);
+        const double a2 = al.r * al.r + al.i * al.i;
+        const double nr2 = nrm[p];
+        const double x2 = nr2 - a2 > 0.0 ? nr2 - a2 : (nr2 == nr2 ? 0.0 : nr2);
+        double beta, tr, ti;
+        cd scale;
+        if (x2 == 0.0 && al.i == 0.0) {
+            tr = 0.0; ti = 0.0; beta = al.r; scale = cd{0.0, 0.0};
+        } else {
+            const double nr = sqrt(a2 + x2);
+            beta = al.r >= 0.0 ? -nr : nr;
+            tr = (beta - al.r) / beta;
+            ti = -al.i / beta;
+            scale = cinv(al - cd{beta, 0.0});
+        }
+        // ---- columns k and p exchanged; column k becomes R above the diagonal,
+        // beta on it and the scaled reflector below it (also into LDS)
+        for (int r = tid; r < n; r += nth) {
+            const cd xk = A[r * n + k];
+            const cd xp = p != k ? A[r * n + p] : xk;
+            cd y = xp;
+            if (r == k) y = cd{beta, 0.0};
+            else if (r > k) y = xp * scale;
+            A[r * n + k] = y;
+            if (p != k) A[r * n + p] = xk;
+            if (r > k) stc(vs, r, y);
+        }
+        if (tid == 0) {
+            stc(vs, k, cd{1.0, 0.0});
+            stc(tau, k, cd{tr, ti});
+            if (p != k) {
+                const int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
             }
         }
         c.sync();
-        {
-            // the reflector, scaled, into A below the diagonal and into LDS
-            const cd scale = cd{bcast[1], bcast[2]};
-            for (int r = k + 1 + tid; r < n; r += nth) {
-                const cd x = A[r * n + k] * scale;
-                A[r * n + k] = x;
-                stc(vs, r, x);
-            }
-            if (tid == 0) {
-                A[k * n + k] = cd{bcast[3], 0.0};
-                stc(vs, k, cd{1.0, 0.0});
-            }
+        // ---- apply H(k)^H = I - conj(tau) v v^H to columns j > k, new norms and
+        // diagonal candidates (the exchanged column p included)
+        if (k + 1 < n) {
+            const cd ctau = cd{tr, -ti};
+            if (n <= 18 * kGroup) qrp_apply_regs<18, kGroup>(c, A, n, k, vs, nrm, ctau, alpha);
+            else if (n <= 18 * 2 * kGroup) qrp_apply_regs<18, 2 * kGroup>(c, A, n, k, vs, nrm, ctau, alpha);
+            else qrp_apply_loop(c, A, n, k, vs, nrm, ctau, alpha);
+            c.sync();
         }
-        c.sync();
-        // ---- apply H(k)^H = I - conj(tau) v v^H to columns j > k, new norms
-        const cd ctau = conj(ldc(tau, k));
-        if (n <= 18 * kGroup) qrp_apply_regs<18, kGroup>(c, A, n, k, vs, nrm, ctau);
-        else if (n <= 18 * 2 * kGroup) qrp_apply_regs<18, 2 * kGroup>(c, A, n, k, vs, nrm, ctau);
-        else qrp_apply_loop(c, A, n, k, vs, nrm, ctau);
-        c.sync();
     }
     // ---- D and row normalisation of R
     for (int k = tid; k < n; k += nth) D[k] = cabs(A[k * n + k]);
