@@ -118,6 +118,11 @@ struct bafq_handle {
     int path_count[2] = {0, 0};
     int* d_queue = nullptr;                   // one work counter per path
     int grid[2][2] = {{0, 0}, {0, 0}};        // persistent grid per [kernel][path]
+    // plain sweeps of the device-memory path with two workgroups per chain:
+    // npairs pairs (0: one workgroup per chain); d_pairs holds their barrier
+    // words (2 per pair) and mailboxes (1 per pair)
+    int npairs = 0;
+    int* d_pairs = nullptr;
     std::vector<bafq_schedule> schedule;      // in effect, per model
     void* launches = nullptr;                 // back-end launch cache
     // learned global moves: old fields, forward drift, per-chain sums, flags, subset ids
@@ -441,6 +446,25 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
             h->grid[k][p] = std::min(res, h->path_count[p]);
         }
     }
+    // Two workgroups per chain on the device-memory path when every chain then
+    // still has its pair resident at once (or BAFQ_PAIRS=1: as many pairs as
+    // fit), the batch has no LDS-path chains (whose kernel would share the
+    // CUs) and every model uses submatrix updates.  BAFQ_PAIRS=0 turns it off.
+    {
+        const char* env = std::getenv("BAFQ_PAIRS");
+        const int mode = env ? std::atoi(env) : -1;
+        bool ok = mode != 0 && h->path_count[P_GLOBAL] > 0 && h->path_count[P_LDS] == 0;
+        for (int c = 0; ok && c < n_chains; ++c) ok = h->models[chain_model[c]].dk > 0;
+        if (ok) {
+            const int res = rt_resident_groups(K_SWEEP, P_GLOBAL, rt_block_threads(), h->max_lds[P_GLOBAL]);
+            const int cnt = h->path_count[P_GLOBAL];
+            if (res >= 2 && (2 * cnt <= res || mode == 1)) h->npairs = std::min(cnt, res / 2);
+        }
+        if (h->npairs > 0 && rt_malloc((void**)&h->d_pairs, sizeof(int) * 3 * (size_t)h->npairs)) {
+            *out = h;
+            return fail(h, rt_error());
+        }
+    }
     if (rt_sync()) { *out = h; return fail(h, rt_error()); }
     *out = h;
     return 0;
@@ -454,6 +478,7 @@ void bafq_destroy(bafq_handle* h) {
     if (h->gm_state) bafq::rt_free(h->gm_state);
     if (h->d_grad) bafq::rt_free(h->d_grad);
     if (h->tau_ws) bafq::rt_free(h->tau_ws);
+    if (h->d_pairs) bafq::rt_free(h->d_pairs);
     bafq::rt_release_launches(h->launches);
     if (h->dev) bafq::rt_free(h->dev);
     delete h;
@@ -464,6 +489,8 @@ int bafq_get_schedule(const bafq_handle* h, int model, bafq_schedule* out) {
     *out = h->schedule[model];
     return 0;
 }
+
+int bafq_pairs(const bafq_handle* h) { return h ? h->npairs : 0; }
 
 const char* bafq_last_error(const bafq_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
@@ -510,7 +537,13 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure,
     a.queue = nullptr;
     a.grad = h->d_grad;
     a.phi0 = h->d_phi;
+    a.pairs = 0;
+    a.npairs = 0;
+    a.pair_flags = nullptr;
+    a.pair_mail = nullptr;
+    a.pair_wait = 0;
     double total_ms = 0.0;
+    const bool paired = k == K_SWEEP && h->npairs > 0;
     if (k == K_WARMUP) {
         a.ids = h->d_ids;
         a.count = (int)h->chains.size();
@@ -530,8 +563,24 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure,
             l.grid = h->grid[(k == K_GRAD || k == K_MEASURE || k == K_SWEEP_TAU) ? K_SWEEP : k][p];
             l.block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
             l.smem = h->max_lds[p];
+            if (paired && p == P_GLOBAL) {
+                l.a.pairs = 1;
+                l.a.npairs = h->npairs;
+                l.a.pair_flags = h->d_pairs;
+                l.a.pair_mail = h->d_pairs + 2 * h->npairs;
+                l.a.pair_wait = (uint64_t)(20.0 * rt_clock_hz());    // 20 s
+                l.grid = 2 * h->npairs;
+            }
         }
         if (rt_launch_paths(&h->launches, k, pl, n, &total_ms)) return fail(h, rt_error());
+        if (paired) {
+            std::vector<int> fl(2 * (size_t)h->npairs);
+            if (rt_d2h(fl.data(), h->d_pairs, sizeof(int) * fl.size())) return fail(h, rt_error());
+            for (int i = 0; i < h->npairs; ++i)
+                if (fl[2 * i + 1])
+                    return fail(h, "a workgroup waited more than 20 s for its partner (the pair was not resident "
+                                   "at once); the sweep's results are invalid, set BAFQ_PAIRS=0");
+        }
     }
     h->last_ms = total_ms;
     if (rt_sync()) return fail(h, rt_error());

@@ -77,8 +77,12 @@ __global__ void __launch_bounds__(512) k_sweep_tau_lds(LaunchArgs a) {
     body_sweep<true, true>(c, a, (int*)smem);
 }
 
-__global__ void __launch_bounds__(64) k_reset_queue(int* q) {
-    if (threadIdx.x == 0) *q = 0;
+// Before each persistent kernel: its work counter and, for a paired launch,
+// the pairs' barrier words.
+__global__ void __launch_bounds__(64) k_reset_queue(LaunchArgs a) {
+    if (threadIdx.x == 0) *a.queue = 0;
+    if (a.pairs)
+        for (int i = threadIdx.x; i < 2 * a.npairs; i += blockDim.x) a.pair_flags[i] = 0;
 }
 
 __global__ void __launch_bounds__(64) k_warmup(LaunchArgs a) {
@@ -649,7 +653,7 @@ static int build_graph(GraphCache::Entry& e, KernelId k, const PathLaunch* pl, i
                   "hipFuncSetAttribute"))
             return 1;
         hipGraphNode_t reset, kern;
-        void* rargs[] = {(void*)&e.args[i].queue};
+        void* rargs[] = {(void*)&e.args[i]};
         hipKernelNodeParams rp{};
         rp.func = (void*)k_reset_queue;
         rp.gridDim = dim3(1);

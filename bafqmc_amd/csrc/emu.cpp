@@ -170,10 +170,13 @@ void run_one_group(int g, int grid, int block, size_t smem, const std::function<
 
 }  // namespace
 
+// min_threads: OS threads the launch needs at least (2 for a paired launch, whose
+// workgroups wait for their partners).
 template <class F>
-static void run_groups(int grid, int block, size_t smem, F&& fn) {
+static void run_groups(int grid, int block, size_t smem, F&& fn, int min_threads = 1) {
     const std::function<void(EmuCtx&)> body = fn;
-    const int nt = std::max(1, std::min<int>(grid, (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()))));
+    int nt = std::max(1, std::min<int>(grid, (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()))));
+    nt = std::max(nt, std::min(grid, min_threads));
     std::atomic<int> next{0};
     auto worker = [&]() {
         std::vector<std::unique_ptr<char[]>> stacks;
@@ -265,6 +268,8 @@ int rt_launch_paths(void**, KernelId k, const PathLaunch* pl, int n, double* ms)
     for (int i = 0; i < n; ++i) {
         const PathLaunch& l = pl[i];
         *l.a.queue = 0;
+        if (l.a.pairs)
+            for (int j = 0; j < 2 * l.a.npairs; ++j) l.a.pair_flags[j] = 0;
         run_groups(l.grid, l.block, l.smem, [&](EmuCtx& c0) {
             EmuCtx c{c0.g, c0.t, c0.smem + kSlotBytes};
             int* slot = (int*)c0.smem;
@@ -284,7 +289,7 @@ int rt_launch_paths(void**, KernelId k, const PathLaunch* pl, int n, double* ms)
                 if (l.path == P_LDS) body_sweep<true>(c, l.a, slot);
                 else body_sweep<false>(c, l.a, slot);
             }
-        });
+        }, l.a.pairs ? 2 : 1);
     }
     if (g_barrier_mismatch) {
         g_err = "emulator: the threads of a workgroup passed different numbers of barriers";

@@ -76,6 +76,12 @@ DEV void stc(P p, int i, cd v) {
 // GEMM variants.
 constexpr int kPairMinN = 96;
 
+// Share `i` of `n` of a routine's work when several workgroups split one
+// matrix operation (the default: all of it).
+struct Part {
+    int i = 0, n = 1;
+};
+
 // ------------------------------------------------------------ reductions
 DEV double wave_sum(const Ctx& c, double v) {
     for (int m = 32; m >= 1; m >>= 1) v += c.shfl_xor(v, m);
@@ -92,9 +98,9 @@ DEV double group_sum(const Ctx& c, double v, int gs) {
 }
 
 // ------------------------------------------------------------ elementwise
-DEV void mat_copy(const Ctx& c, cd* dst, const cd* src, int n, bool conj_t) {
+DEV void mat_copy(const Ctx& c, cd* dst, const cd* src, int n, bool conj_t, Part pt = Part{}) {
     const int nn = n * n;
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+    for (int e = c.tid() + pt.i * c.nthreads(); e < nn; e += pt.n * c.nthreads()) {
         const int r = e / n, k = e - r * n;
         dst[e] = conj_t ? conj(src[k * n + r]) : src[e];
     }
@@ -255,13 +261,13 @@ DEVFN void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool
 // time that share their E fragments (side 0: one tile row and two tile
 // columns; side 1 the transpose); the loads of eight k-steps of both are
 // issued together before their products (the operands come from L2/HBM).
-DEVFN void gemm_rc(const Ctx& c, cd* C, const double* Ez, const cd* X, int side, int n) {
+DEVFN void gemm_rc(const Ctx& c, cd* C, const double* Ez, const cd* X, int side, int n, Part pt = Part{}) {
     const int nt = (n + 15) >> 4, nk = (n + 3) >> 2, nh = (nt + 1) >> 1;
     const int lane = c.lane();
     const int li = lane & 15, lk = lane >> 4;
     auto Xg = as_global(X);
     auto Cg = as_global(C);
-    for (int t = c.wave(); t < nt * nh; t += c.nwaves()) {
+    for (int t = c.wave() + pt.i * c.nwaves(); t < nt * nh; t += pt.n * c.nwaves()) {
         const int sh = t / nh, p = t - sh * nh;        // shared tile index, pair of the other index
         const bool two = 2 * p + 1 < nt;               // wavefront-uniform
         int ti[2], tj[2];

@@ -32,6 +32,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <thread>
 #include <vector>
 #define DEV inline
 #define DEVFN inline
@@ -82,6 +83,30 @@ struct HipCtx {
     DEV int atomic_add(int* p, int v) const { return atomicAdd(p, v); }
     // constant-rate device clock (hipDeviceAttributeWallClockRate kHz)
     DEV uint64_t clock() const { return wall_clock64(); }
+    // Ordering for workgroups that share data through device memory (agent
+    // scope: also across the XCDs' L2 caches).  stores_done: this wavefront's
+    // stores have reached L2 (the L1 is write-through); arrive: a release
+    // increment (one L2 write-back for the workgroup); a waiter polls with
+    // relaxed loads and takes one acquire fence after the last (an acquire
+    // load per poll would invalidate the caches on every iteration).
+    DEV void stores_done() const { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+    DEV void arrive(int* p) const { __hip_atomic_fetch_add(p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
+    DEV int load_relaxed(int* p) const { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    DEV void acquire_device() const { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+    DEV void store_flag(int* p, int v) const { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    DEV void pause() const { __builtin_amdgcn_s_sleep(4); }
+    // Pair and rank of workgroup b in a grid of g workgroups run as pairs.
+    // Workgroups are dispatched to the 8 XCDs in turn (b % 8), so b and b + 8
+    // of each group of 16 share an XCD and its L2; other grids pair b, b + 1.
+    DEV static void pair_of(int b, int g, int& pair, int& rank) {
+        if (g % 16 == 0) {
+            pair = (b >> 4) * 8 + (b & 7);
+            rank = (b >> 3) & 1;
+        } else {
+            pair = b >> 1;
+            rank = b & 1;
+        }
+    }
 
     // DPP lane permutations (no LDS round trip): quad_perm [1,0,3,2] = 0xB1,
     // quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141, row_mirror = 0x140.
@@ -213,6 +238,18 @@ struct EmuCtx {
     }
     int atomic_add(int* p, int v) const {
         return reinterpret_cast<std::atomic<int>*>(p)->fetch_add(v);
+    }
+    void stores_done() const { std::atomic_thread_fence(std::memory_order_release); }
+    void arrive(int* p) const { reinterpret_cast<std::atomic<int>*>(p)->fetch_add(1, std::memory_order_release); }
+    int load_relaxed(int* p) const { return reinterpret_cast<std::atomic<int>*>(p)->load(std::memory_order_relaxed); }
+    void acquire_device() const { std::atomic_thread_fence(std::memory_order_acquire); }
+    void store_flag(int* p, int v) const { reinterpret_cast<std::atomic<int>*>(p)->store(v, std::memory_order_relaxed); }
+    void pause() const { std::this_thread::yield(); }
+    // Workgroups run on a few OS threads in index order: pairs are b, b + 1,
+    // so a workgroup that waits for its partner never holds back the partner's start.
+    static void pair_of(int b, int, int& pair, int& rank) {
+        pair = b >> 1;
+        rank = b & 1;
     }
     // nanoseconds
     uint64_t clock() const {

@@ -95,6 +95,15 @@ struct LaunchArgs {
     const double* phi0;
     int tau;             // K_SWEEP_TAU: convention of the time-displaced pass (TauMode)
     int tau_record;      // K_SWEEP_TAU: keep the last pass's functions at every slice
+    // K_SWEEP on the device-memory path: 1 = two workgroups per chain (the grid
+    // holds 2 x count workgroups, all resident).  Per pair: pair_flags[2p] its
+    // barrier arrivals, pair_flags[2p + 1] set when a wait timed out (both zeroed
+    // before the kernel), pair_mail[p] the chain index the leader took.
+    int pairs;
+    int npairs;
+    int* pair_flags;
+    int* pair_mail;
+    uint64_t pair_wait;  // longest wait at a pair barrier, in clock() ticks
 };
 
 // Bytes at the start of a workgroup's dynamic LDS that hold the work-queue
@@ -213,13 +222,57 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0, int sb =
 // ---------------------------------------------------------------- timers
 struct Timer {
     uint64_t t0;
+    bool on = true;      // false in the second workgroup of a pair (the leader keeps the times)
     DEV void start(const Ctx& c) { t0 = c.clock(); }
     DEV void stop(const Ctx& c, const ChainDev& ch, int slot) {
         const uint64_t t1 = c.clock();
-        if (c.tid() == 0) ch.stats[slot] += (double)(t1 - t0);
+        if (on && c.tid() == 0) ch.stats[slot] += (double)(t1 - t0);
         t0 = t1;
     }
 };
+
+// Workgroups that advance one chain together on the device-memory path (a
+// pair, or a single workgroup: parts = 1).  Both members run the same control
+// flow: every Metropolis decision is made by both from the same data, the
+// matrix products and rank-k updates are split by 16-row tiles (Part), and
+// the stabilisation and the estimators run in the leader (part 0) alone.
+// team_sync orders every write to device memory before it for both members.
+struct Team {
+    int part = 0, parts = 1;
+    int* flags = nullptr;    // [0] arrivals, [1] timed out (device memory)
+    int* epoch = nullptr;    // barriers passed (LDS, thread 0)
+    uint64_t wait = 0;       // longest wait, clock() ticks
+    DEV bool lead() const { return part == 0; }
+    DEV Part share() const { return Part{part, parts}; }
+};
+
+// Barrier of the team.  A member that waits longer than team.wait sets the
+// timed-out flag, after which no member of that pair waits again: the kernel
+// ends and the launch reports the error (its results are then invalid).
+DEV void team_sync(const Ctx& c, const Team& team) {
+    if (team.parts == 1) {
+        c.sync();
+        return;
+    }
+    c.stores_done();
+    c.sync();
+    if (c.tid() == 0) {
+        const int target = team.parts * ++*team.epoch;
+        c.arrive(team.flags);                    // releases the workgroup's completed stores
+        if (!c.load_relaxed(team.flags + 1)) {
+            const uint64_t t0 = c.clock();
+            while (c.load_relaxed(team.flags) < target) {
+                if (c.clock() - t0 > team.wait) {
+                    c.store_flag(team.flags + 1, 1);
+                    break;
+                }
+                c.pause();
+            }
+        }
+        c.acquire_device();                      // the partner's stores, for the whole workgroup (one L1)
+    }
+    c.sync();
+}
 
 // ---------------------------------------------------------------- helpers
 DEV int nsec(const ModelDev& m) { return m.kind == 1 ? 2 : 1; }
@@ -272,11 +325,11 @@ DEV void hs_factors(const Ctx& c, const ModelDev& m, const Lds& L, const double*
 
 // M <- diag(fl) M diag(fr), fl/fr in LDS (or null); M in LDS or global (pointer type MP).
 template <class MP>
-DEV void scale_mat_t(const Ctx& c, MP M, int n, const cd* fl_, const cd* fr_) {
+DEV void scale_mat_t(const Ctx& c, MP M, int n, const cd* fl_, const cd* fr_, Part pt = Part{}) {
     auto fl = as_lds(fl_);
     auto fr = as_lds(fr_);
     const int nn = n * n;
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+    for (int e = c.tid() + pt.i * c.nthreads(); e < nn; e += pt.n * c.nthreads()) {
         const int r = e / n, k = e - r * n;
         cd x = ldc(M, e);
         if (fl_) x = ldc(fl, r) * x;
@@ -285,7 +338,9 @@ DEV void scale_mat_t(const Ctx& c, MP M, int n, const cd* fl_, const cd* fr_) {
     }
     c.sync();
 }
-DEV void scale_mat(const Ctx& c, cd* M, int n, const cd* fl, const cd* fr) { scale_mat_t(c, M, n, fl, fr); }
+DEV void scale_mat(const Ctx& c, cd* M, int n, const cd* fl, const cd* fr, Part pt = Part{}) {
+    scale_mat_t(c, M, n, fl, fr, pt);
+}
 DEV void scale_mat_lds(const Ctx& c, cd* M, int n, const cd* fl, const cd* fr) {
     scale_mat_t(c, as_lds(M), n, fl, fr);
 }
@@ -481,15 +536,18 @@ DEVFN void site_loop_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 // one wavefront); G in LDS or in device memory, U and W in LDS or device
 // memory; cols <= kMaxCols (16 or 32).  A tile's operands for all its k-steps
 // and its tile of G0 are loaded before the products (one round trip per tile).
+// With a Part, only the row tiles ti = pt.i, pt.i + pt.n, ... (rows of Ut
+// outside them are not read).
 template <int kMaxCols = 16, class GP, class UP>
-DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols) {
+DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols, Part pt = Part{}) {
     constexpr int kQ = kMaxCols / 4;
     n = c.uniform(n);                 // scalar tests of the k-steps taken
     cols = c.uniform(cols);
     const int nt = (n + 15) >> 4;
+    const int nrt = (nt - pt.i + pt.n - 1) / pt.n;    // row tiles of this part
     const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
-    for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
-        const int ti = t / nt, tj = t - ti * nt;
+    for (int t = c.wave(); t < nrt * nt; t += c.nwaves()) {
+        const int ti = pt.i + pt.n * (t / nt), tj = t % nt;
         const int ar = 16 * ti + li, bc = 16 * tj + li;
         cd a[kQ], b[kQ], g[4];
 #pragma unroll
@@ -530,11 +588,12 @@ DEV void delayed_flush_t(const Ctx& c, GP G, UP Ut, UP W, int n, int cols) {
 DEVFN void delayed_flush(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
     delayed_flush_t(c, as_lds(G), as_lds(Ut), as_lds(W), n, cols);
 }
-DEVFN void delayed_flush_glb(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
-    delayed_flush_t(c, as_global(G), as_lds(Ut), as_lds(W), n, cols);
+DEVFN void delayed_flush_glb(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols, Part pt = Part{}) {
+    delayed_flush_t(c, as_global(G), as_lds(Ut), as_lds(W), n, cols, pt);
 }
-DEVFN void delayed_flush_glb_uw(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols) {
-    delayed_flush_t<kMaxDkGlb>(c, as_global(G), as_global(Ut), as_global(W), n, cols);
+DEVFN void delayed_flush_glb_uw(const Ctx& c, cd* G, const cd* Ut, const cd* W, int n, int cols,
+                                Part pt = Part{}) {
+    delayed_flush_t<kMaxDkGlb>(c, as_global(G), as_global(Ut), as_global(W), n, cols, pt);
 }
 
 // Same decisions, RNG use and result as site_loop_t (LocalU_metro + LocalU_prop_L/R).
@@ -739,13 +798,16 @@ DEVFN void site_loop_delayed_glb(const Ctx& c, const ModelDev& m, const ChainDev
 // = sum_a G[x][row_a] X[a][j], x < n, j < na, on FP64 MFMA 16x16x4 (complex by
 // three real products); X (na x na, leading dimension sb) in LDS, the rows
 // row_a = grow(loc[a]) of G gathered from device memory.
+// With a Part, only the row tiles tx = pt.i, pt.i + pt.n, ... of U (those of
+// the flush's share).
 template <class GP, class UP, class RowOf>
 DEV void sub_form_u(const Ctx& c, UP Ut, GP G, LDS_AS const cd* X, int sb, LDS_AS const int* loc, int n, int na,
-                    RowOf grow) {
+                    RowOf grow, Part pt = Part{}) {
     const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
     const int ntx = (n + 15) >> 4, ntj = (na + 15) >> 4, nk = (na + 3) >> 2;
-    for (int t = c.wave(); t < ntx * ntj; t += c.nwaves()) {
-        const int tx = t / ntj, tj = t - tx * ntj;
+    const int nrt = (ntx - pt.i + pt.n - 1) / pt.n;
+    for (int t = c.wave(); t < nrt * ntj; t += c.nwaves()) {
+        const int tx = pt.i + pt.n * (t / ntj), tj = t % ntj;
         const int ar = 16 * tx + li, bc = 16 * tj + li;
         d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ss = rr;
         for (int k0 = 0; k0 < nk; k0 += 4) {
@@ -794,9 +856,15 @@ DEV void sub_form_u(const Ctx& c, UP Ut, GP G, LDS_AS const cd* X, int sb, LDS_A
 //     order, whose pivots are the accepted sites' P (nonzero), U = G0[:,A] X
 //     runs on the matrix cores, then the flush G -= U W with W = (1 - G0)[A,:].
 // The per-site work is O(|R|^2) in LDS instead of O(N |A|) in device memory.
+// In a team (two workgroups on one chain) both members make every decision;
+// U's row tiles and the flush are split between them, W is formed whole by
+// each member when it is in LDS and in halves when in device memory, and
+// team barriers separate the reads of G of a block from the flush and the
+// flush from the next block's reads.  The caller has passed a team barrier
+// since its last write to G.
 template <bool kUW>
 DEV void site_loop_sub_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G_, int t, int nf,
-                         bool ascending, int& seed) {
+                         bool ascending, int& seed, const Team& team) {
     const int n = m.n, ns = m.ns, sb = m.dk;
     const bool pairing = m.kind == 1;
     const int ksec = pairing ? 2 : 1;
@@ -944,32 +1012,36 @@ DEV void site_loop_sub_t(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
         }
         c.sync();
         // ---- U = G0[:,A] X and W = (1 - G0)[A,:] from G (still G0), then the flush
-        sub_form_u(c, Ut, G, B, sb, loc, n, na, grow);
-        for (int e = tid; e < na * n; e += nth) {
+        sub_form_u(c, Ut, G, B, sb, loc, n, na, grow, team.share());
+        const Part wp = kUW ? team.share() : Part{};
+        for (int e = tid + wp.i * nth; e < na * n; e += wp.n * nth) {
             const int j = e / n, x = e - j * n;
             const int row = grow(loc[j]);
             const cd g = ldc(G, row * n + x);
             stc(W, j * n + x, cd{(x == row ? 1.0 : 0.0) - g.r, -g.i});
         }
-        c.sync();
-        if (kUW) delayed_flush_glb_uw(c, G_, du, dw, n, na);
-        else delayed_flush_glb(c, G_, du, dw, n, na);
+        team_sync(c, team);                          // every read of G of this block is done
+        if (kUW) delayed_flush_glb_uw(c, G_, du, dw, n, na, team.share());
+        else delayed_flush_glb(c, G_, du, dw, n, na, team.share());
+        if (team.parts > 1) team_sync(c, team);      // the flush is done before the next block's reads
     }
-    for (int i = tid; i < ns; i += nth) {
-        auto pp = props + i;
-        if (pp->pad != 0.0) phi_t[i * 2 + nf - 1] = pp->phi_new;
+    if (team.lead()) {
+        for (int i = tid; i < ns; i += nth) {
+            auto pp = props + i;
+            if (pp->pad != 0.0) phi_t[i * 2 + nf - 1] = pp->phi_new;
+        }
+        if (tid == 0) {
+            ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
+            ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
+        }
     }
-    if (tid == 0) {
-        ch.stats[nf == 1 ? S_PROP_U1 : S_PROP_U2] += (double)ns;
-        ch.stats[nf == 1 ? S_ACC_U1 : S_ACC_U2] += (double)nacc;
-    }
-    c.sync();
+    team_sync(c, team);                              // fields written, last reads of G done
 }
 
 DEVFN void site_loop_sub_glb(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* G, int t, int nf,
-                             bool ascending, int& seed) {
-    if (m.uw_glb) site_loop_sub_t<true>(c, m, ch, L, G, t, nf, ascending, seed);
-    else site_loop_sub_t<false>(c, m, ch, L, G, t, nf, ascending, seed);
+                             bool ascending, int& seed, const Team& team) {
+    if (m.uw_glb) site_loop_sub_t<true>(c, m, ch, L, G, t, nf, ascending, seed, team);
+    else site_loop_sub_t<false>(c, m, ch, L, G, t, nf, ascending, seed, team);
 }
 
 // ---------------------------------------------------------------- MFMA GEMM in LDS
@@ -1478,26 +1550,37 @@ DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
     c.sync();
 }
 
+// With a team of two workgroups (submatrix updates only, dk > 0; no grad or
+// frozen pass) the products and scalings are split by the team, the wraps and
+// the estimators run in the leader, and a team barrier follows every phase
+// that writes device memory the other member reads next.  A single
+// workgroup computes the same values in the same order.
 DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
-                           bool meas, double* grad = nullptr, bool frozen = false) {
+                           bool meas, double* grad = nullptr, bool frozen = false, const Team& team = Team{}) {
     const int n = m.n;
+    const Part sh = team.share();
     Timer tm;
-    live_identity(c, n, ch.ul, ch.dl, ch.vl);     // Wrap_L(Ltrot): L list slot Nst is the identity
+    tm.on = team.lead();
+    if (team.lead()) live_identity(c, n, ch.ul, ch.dl, ch.vl);   // Wrap_L(Ltrot): L list slot Nst is the identity
+    team_sync(c, team);
     tm.start(c);
     for (int nt = m.ltrot; nt >= 1; --nt) {
         if (nt % m.nwrap == 0 && nt != m.ltrot) {
-            wrap_l_global(c, m, ch, L, nt);
+            if (team.lead()) wrap_l_global(c, m, ch, L, nt);
+            team_sync(c, team);
             tm.stop(c, ch, S_T_WRAP);
         }
         if (meas) {
-            measure(c, m, ch, L, ch.g);
+            if (team.lead()) measure(c, m, ch, L, ch.g);
             tm.stop(c, ch, S_T_MEAS);
         }
         // propT_L: G <- e^{+dtau K} G e^{-dtau K}, UL <- UL e^{-dtau K}
-        gemm_rc(c, ch.w1, m.ez[2], ch.g, 1, n);
-        gemm_rc(c, ch.g, m.ez[1], ch.w1, 0, n);
-        gemm_rc(c, ch.w1, m.ez[2], ch.ul, 1, n);
-        mat_copy(c, ch.ul, ch.w1, n, false);
+        gemm_rc(c, ch.w1, m.ez[2], ch.g, 1, n, sh);
+        gemm_rc(c, ch.w2, m.ez[2], ch.ul, 1, n, sh);
+        team_sync(c, team);
+        gemm_rc(c, ch.g, m.ez[1], ch.w1, 0, n, sh);
+        mat_copy(c, ch.ul, ch.w2, n, false, sh);
+        team_sync(c, team);
         tm.stop(c, ch, S_T_KIN);
         const int t = nt - 1;
         const double* phi_t = ch.phi + (size_t)t * m.ns * 2;
@@ -1505,24 +1588,29 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
             if (!m.active[nf - 1]) continue;
             if (grad) grad_extract(c, m, phi_t, as_global(ch.g), t, nf, grad);
             else if (frozen) {}
-            else if (m.dk > 0) site_loop_sub_glb(c, m, ch, L, ch.g, t, nf, false, seed);
+            else if (m.dk > 0) site_loop_sub_glb(c, m, ch, L, ch.g, t, nf, false, seed, team);
             else site_loop(c, m, ch, L, ch.g, t, nf, false, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
-            scale_mat(c, ch.g, n, L.finv, L.f);      // G <- e^{-V} G e^{V}
-            scale_mat(c, ch.ul, n, nullptr, L.f);    // UL <- UL e^{V}
+            scale_mat(c, ch.g, n, L.finv, L.f, sh);      // G <- e^{-V} G e^{V}
+            scale_mat(c, ch.ul, n, nullptr, L.f, sh);    // UL <- UL e^{V}
+            team_sync(c, team);
             tm.stop(c, ch, S_T_HS);
         }
     }
-    wrap_l_global(c, m, ch, L, 0);
+    if (team.lead()) wrap_l_global(c, m, ch, L, 0);
+    team_sync(c, team);
     tm.stop(c, ch, S_T_WRAP);
 }
 
 DEV void sweep_right_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
-                            bool meas) {
+                            bool meas, const Team& team = Team{}) {
     const int n = m.n;
+    const Part sh = team.share();
     Timer tm;
-    live_identity(c, n, ch.ur, ch.dr, ch.vr);     // Wrap_R(0): R list slot 0 is the identity
+    tm.on = team.lead();
+    if (team.lead()) live_identity(c, n, ch.ur, ch.dr, ch.vr);   // Wrap_R(0): R list slot 0 is the identity
+    team_sync(c, team);
     tm.start(c);
     for (int nt = 1; nt <= m.ltrot; ++nt) {
         const int t = nt - 1;
@@ -1530,27 +1618,33 @@ DEV void sweep_right_global(const Ctx& c, const ModelDev& m, const ChainDev& ch,
         for (int nf = 1; nf <= 2; ++nf) {
             if (!m.active[nf - 1]) continue;
             hs_factors(c, m, L, phi_t, nf);
-            scale_mat(c, ch.g, n, L.f, L.finv);      // G <- e^{V} G e^{-V} (old fields)
+            scale_mat(c, ch.g, n, L.f, L.finv, sh);      // G <- e^{V} G e^{-V} (old fields)
+            team_sync(c, team);
             tm.stop(c, ch, S_T_HS);
-            if (m.dk > 0) site_loop_sub_glb(c, m, ch, L, ch.g, t, nf, true, seed);
+            if (m.dk > 0) site_loop_sub_glb(c, m, ch, L, ch.g, t, nf, true, seed, team);
             else site_loop(c, m, ch, L, ch.g, t, nf, true, seed);
             tm.stop(c, ch, S_T_SITE);
             hs_factors(c, m, L, phi_t, nf);
-            scale_mat(c, ch.ur, n, L.f, nullptr);    // UR <- e^{V} UR (new fields)
+            scale_mat(c, ch.ur, n, L.f, nullptr, sh);    // UR <- e^{V} UR (new fields)
             tm.stop(c, ch, S_T_HS);
         }
+        team_sync(c, team);
         // propT_R: G <- e^{-dtau K} G e^{+dtau K}, UR <- e^{-dtau K} UR
-        gemm_rc(c, ch.w1, m.ez[0], ch.g, 0, n);
-        gemm_rc(c, ch.g, m.ez[3], ch.w1, 1, n);
-        gemm_rc(c, ch.w1, m.ez[0], ch.ur, 0, n);
-        mat_copy(c, ch.ur, ch.w1, n, false);
+        gemm_rc(c, ch.w1, m.ez[0], ch.g, 0, n, sh);
+        gemm_rc(c, ch.w2, m.ez[0], ch.ur, 0, n, sh);
+        team_sync(c, team);
+        gemm_rc(c, ch.g, m.ez[3], ch.w1, 1, n, sh);
+        mat_copy(c, ch.ur, ch.w2, n, false, sh);
+        team_sync(c, team);
         tm.stop(c, ch, S_T_KIN);
         if (nt % m.nwrap == 0) {
-            wrap_r_global(c, m, ch, L, nt);
+            if (team.lead()) wrap_r_global(c, m, ch, L, nt);
+            team_sync(c, team);
             tm.stop(c, ch, S_T_WRAP);
         }
         if (meas) {
-            measure(c, m, ch, L, ch.g);
+            if (team.lead()) measure(c, m, ch, L, ch.g);
+            if (team.parts > 1) team_sync(c, team);     // the next slice's scaling writes G
             tm.stop(c, ch, S_T_MEAS);
         }
     }
@@ -1738,6 +1832,36 @@ DEV int next_item(const Ctx& c, int* queue, int* slot) {
     return q;
 }
 
+// The same for a team: the leader takes the index and leaves it in the pair's
+// mailbox for its partner.  After a timed-out wait both members stop.
+DEV int next_item_team(const Ctx& c, const LaunchArgs& a, int* slot, const Team& team, int* mail) {
+    if (team.parts == 1) return next_item(c, a.queue, slot);
+    if (team.lead() && c.tid() == 0) c.store_flag(mail, c.atomic_add(a.queue, 1));
+    team_sync(c, team);
+    if (c.tid() == 0) *slot = c.load_relaxed(team.flags + 1) ? a.count : c.load_relaxed(mail);
+    c.sync();
+    const int q = *slot;
+    c.sync();
+    return q;
+}
+
+// The team of workgroup c.block() in a launch with a.pairs set.
+DEV Team make_team(const Ctx& c, const LaunchArgs& a, int* slot, int** mail) {
+    Team team;
+    if (!a.pairs) return team;
+    int pair, rank;
+    Ctx::pair_of(c.block(), c.nblocks(), pair, rank);
+    team.part = rank;
+    team.parts = 2;
+    team.flags = a.pair_flags + 2 * pair;
+    team.epoch = slot + 1;
+    team.wait = a.pair_wait;
+    *mail = a.pair_mail + pair;
+    if (c.tid() == 0) *team.epoch = 0;
+    c.sync();
+    return team;
+}
+
 template <bool kLds>
 DEV void body_prepare(const Ctx& c, const LaunchArgs& a, int* slot) {
     for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
@@ -1752,9 +1876,12 @@ DEV void body_prepare(const Ctx& c, const LaunchArgs& a, int* slot) {
 // kTau: Local_sweep with toggle = .true. (local_sweep.f90:119-123): the
 // time-displaced pass between sweep_L and sweep_R (a kernel of its own, so the
 // plain sweep's code is not affected by it).
+// a.pairs (device-memory path, plain sweep only): two workgroups per chain.
 template <bool kLds, bool kTau = false>
 DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
-    for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
+    int* mail = nullptr;
+    const Team team = (!kLds && !kTau) ? make_team(c, a, slot, &mail) : Team{};
+    for (int q = next_item_team(c, a, slot, team, mail); q < a.count; q = next_item_team(c, a, slot, team, mail)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
         const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk, kLds ? 0 : m.dk);
@@ -1762,16 +1889,16 @@ DEV void body_sweep(const Ctx& c, const LaunchArgs& a, int* slot) {
         if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
         for (int s = 0; s < a.nsweep; ++s) {
             if (kLds) sweep_left_lds(c, m, ch, L, seed, a.measure != 0);
-            else sweep_left_global(c, m, ch, L, seed, a.measure != 0);
+            else sweep_left_global(c, m, ch, L, seed, a.measure != 0, nullptr, false, team);
             if (kTau && ch.tau) {
                 if (kLds) global_from_lds(c, ch.g, L.mat, m.n);    // G(0) for the pass; LDS keeps it
                 tau_pass(c, m, ch, L, a.tau, a.tau_record != 0 && s == a.nsweep - 1);
             }
             if (kLds) sweep_right_lds(c, m, ch, L, seed, a.measure != 0);
-            else sweep_right_global(c, m, ch, L, seed, a.measure != 0);
+            else sweep_right_global(c, m, ch, L, seed, a.measure != 0, team);
         }
         if (kLds) global_from_lds(c, ch.g, L.mat, m.n);
-        if (c.tid() == 0) *ch.seed = seed;
+        if (team.lead() && c.tid() == 0) *ch.seed = seed;
         c.sync();
     }
 }

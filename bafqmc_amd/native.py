@@ -46,6 +46,7 @@ SYMBOLS = {
     "bafq_create_scheduled": (C.c_int, [C.POINTER(ModelDesc), C.c_int, c_int32_p, C.c_int, C.POINTER(Schedule),
                                         C.POINTER(C.c_void_p)]),
     "bafq_get_schedule": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Schedule)]),
+    "bafq_pairs": (C.c_int, [C.c_void_p]),
     "bafq_destroy": (None, [C.c_void_p]),
     "bafq_last_error": (C.c_char_p, [C.c_void_p]),
     "bafq_dims": (C.c_int, [C.c_void_p, C.c_int, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),

@@ -116,6 +116,10 @@ class BatchedSolver:
             out.append((sc.path, sc.delayed))
         return out
 
+    def pairs(self):
+        """Pairs of workgroups sharing a chain in the device-memory sweep kernel (0: none)."""
+        return int(self.lib.bafq_pairs(self.h))
+
     # ------------------------------------------------------------------ utils
     def _check(self, rc, what):
         if rc:

@@ -17,7 +17,7 @@ from tests.test_gpu_parity import CASES
 from bafqmc_amd.model import ModelSpec
 from bafqmc_amd.solver import BatchedSolver
 from tests.test_gpu_parity import (check_sweep_vs_oracle, check_golden, check_mfma_layout, check_lane_ops, check_qrp,
-                                   check_qrp_ties,
+                                   check_qrp_ties, check_pairs_bitwise,
                                    check_uneven_batch, check_gaussian, check_blowup, GOLDEN)
 
 
@@ -69,6 +69,14 @@ def test_emu_wide_device_memory_updates(emu_lib, case, width, monkeypatch):
     MFMA k-steps over both."""
     monkeypatch.setattr(BatchedSolver, "default_schedule", (0, width))
     check_sweep_vs_oracle(emu_lib, case)
+
+
+@pytest.mark.parametrize("case,width,nchains", [(3, 8, 3), (4, 16, 2), (5, 32, 1)])
+def test_emu_paired_workgroups_equal_single(emu_lib, case, width, nchains, monkeypatch):
+    """Two workgroups per chain, run as concurrent OS threads; with 3 chains
+    the emulator's 3 resident workgroups hold one pair, which takes the chains
+    one after another from the queue."""
+    check_pairs_bitwise(emu_lib, case, width, nchains, monkeypatch)
 
 
 def test_wide_updates_rejected_on_the_lds_path(emu_lib):

@@ -11,7 +11,7 @@ import pytest
 
 from oracle import bafq_oracle as O
 from bafqmc_amd.model import ModelSpec, conf_set_batch, conf_transfer
-from bafqmc_amd.solver import BatchedSolver
+from bafqmc_amd.solver import BatchedSolver, STATS
 from tests.helpers import (fixed_fields, load_golden, golden_row, golden_info_value, oracle_params, solver_params,
                            reduced, rel_err, assert_obs_close, obs_atol)
 
@@ -337,6 +337,12 @@ def test_wide_device_memory_updates_match_reference(hip_lib, name, width, monkey
     check_golden(hip_lib, name)
 
 
+@pytest.mark.parametrize("case,width,nchains", [(4, 16, 3), (5, 32, 2), (6, 16, 2), (7, 32, 3)])
+def test_paired_workgroups_equal_single(hip_lib, case, width, nchains, monkeypatch):
+    """Two workgroups per chain (device-memory path): the same chain, bit for bit."""
+    check_pairs_bitwise(hip_lib, case, width, nchains, monkeypatch)
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_uneven_batch_equals_single_chains(hip_lib, mixed):
     check_uneven_batch(hip_lib, mixed)
@@ -424,6 +430,40 @@ def test_uneven_workload_batch_equals_single_chains(hip_lib):
         f1, s1d = s1.get_fields()
         assert np.array_equal(fb[c], f1[0]) and sb[c] == s1d[0], (c, m)
         s1.close()
+
+
+def check_pairs_bitwise(lib, case, width, nchains, monkeypatch, sweeps=2):
+    """Two workgroups per chain on the device-memory path (BAFQ_PAIRS=1) give
+    the same chain as one workgroup per chain (BAFQ_PAIRS=0), bit for bit:
+    fields, seeds, G, every estimator and every counter except the timers."""
+    d = CASES[case]
+    ns = d["nlx"] * d["nly"]
+    phis = [fixed_fields(d["ltrot"], ns) * (1.0 + 0.05 * k) for k in range(nchains)]
+    seeds = [24680 + 11 * k for k in range(nchains)]
+    timers = [v for k, v in STATS.items() if k.startswith("t_")]
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("BAFQ_PAIRS", mode)
+        s = BatchedSolver([ModelSpec(solver_params(d))], [0] * nchains, lib=lib, schedule=[(0, width)])
+        if mode == "0":
+            assert s.pairs() == 0
+        else:
+            assert s.pairs() >= 1
+        s.set_fields(phis, seeds)
+        s.prepare()
+        for _ in range(sweeps):
+            s.sweep(1)
+        obs, st = s.read_bin_raw()
+        st = np.delete(np.asarray(st), timers, axis=1)
+        phi, sd = s.get_fields()
+        out[mode] = ([s.green(c) for c in range(nchains)], np.asarray(obs), st, phi, np.asarray(sd))
+        s.close()
+    a, b = out["0"], out["1"]
+    for c in range(nchains):
+        assert np.array_equal(a[0][c], b[0][c]), f"G of chain {c}"
+        assert np.array_equal(a[3][c], b[3][c]), f"fields of chain {c}"
+    for k in (1, 2, 4):
+        assert np.array_equal(a[k], b[k])
 
 
 def check_uneven_batch(hip_lib, mixed=False, light=False):
