@@ -148,12 +148,12 @@ DEV cd load_op(const cd* M, bool conj_t, int row, int col, int n) {
 // loaders la(r, k), lb(k, j) apply any transposition, conjugation, masking or
 // permutation; st(r, j, v) stores one result.
 template <class LA, class LB, class ST>
-DEV void mfma_gemm(const Ctx& c, int n, LA la, LB lb, ST st) {
+DEV void mfma_gemm(const Ctx& c, int n, LA la, LB lb, ST st, Part pt = Part{}) {
     constexpr int kChunk = 4;
     const int nt = (n + 15) >> 4, nk = (n + 3) >> 2;
     const int lane = c.lane();
     const int li = lane & 15, lk = lane >> 4;
-    for (int t = c.wave(); t < nt * nt; t += c.nwaves()) {
+    for (int t = c.wave() + pt.i * c.nwaves(); t < nt * nt; t += pt.n * c.nwaves()) {
         const int ti = t / nt, tj = t - ti * nt;
         const int ar = 16 * ti + li, bc = 16 * tj + li;
         // three real products per complex one (Gauss): Re = sum ar br - sum ai bi,
@@ -190,12 +190,12 @@ DEV void mfma_gemm(const Ctx& c, int n, LA la, LB lb, ST st) {
 // their A fragments, the loads of both issued together (for operands in device
 // memory); the same products per tile.
 template <class LA, class LB, class ST>
-DEV void mfma_gemm2(const Ctx& c, int n, LA la, LB lb, ST st) {
+DEV void mfma_gemm2(const Ctx& c, int n, LA la, LB lb, ST st, Part pt = Part{}) {
     constexpr int kChunk = 4, kT = 2;
     const int nt = (n + 15) >> 4, nk = (n + 3) >> 2, nh = (nt + kT - 1) / kT;
     const int lane = c.lane();
     const int li = lane & 15, lk = lane >> 4;
-    for (int t = c.wave(); t < nt * nh; t += c.nwaves()) {
+    for (int t = c.wave() + pt.i * c.nwaves(); t < nt * nh; t += pt.n * c.nwaves()) {
         const int ti = t / nh, tj0 = kT * (t - ti * nh);
         const int ar = 16 * ti + li;
         // three real products per complex one (Gauss): Re = sum ar br - sum ai bi,
@@ -246,12 +246,12 @@ DEV void mfma_gemm2(const Ctx& c, int n, LA la, LB lb, ST st) {
     c.sync();
 }
 
-DEVFN void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool opB, int n) {
+DEVFN void gemm_cc(const Ctx& c, cd* C, const cd* A, bool opA, const cd* B, bool opB, int n, Part pt = Part{}) {
     auto la = [&](int r, int k) { return load_op(A, opA, r, k, n); };
     auto lb = [&](int k, int j) { return load_op(B, opB, k, j, n); };
     auto st = [&](int r, int j, cd v) { C[r * n + j] = v; };
-    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st);
-    else mfma_gemm(c, n, la, lb, st);
+    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st, pt);
+    else mfma_gemm(c, n, la, lb, st, pt);
 }
 
 // C = E X (side 0) or X E (side 1) on FP64 MFMA 16x16x4, E real N x N in MFMA
@@ -597,6 +597,11 @@ DEV void refl_apply_glb(const Ctx& c, cd* X, const cd* A, int n, int k, cd t, in
 
 // ------------------------- blocked reflector application and substitution (device memory)
 // For n >= kBlkMinN and with a work buffer of 32 n + 512 complex, Q formation,
+// With a Part, the blocked routines below touch only the 16-column tiles
+// j / 16 = pt.i (mod pt.n) of the matrix they update (their columns are
+// independent); each workgroup then needs a work buffer of its own.
+DEV bool own_col(Part pt, int j) { return ((j >> 4) % pt.n) == pt.i; }
+
 // Q^H X and the triangular solve run on blocks of kBlk reflectors or rows: a
 // block of reflectors H(kb) ... H(kb + nb - 1) = I - V T V^H (ZLARFT, forward,
 // columnwise) is applied as X <- X - V (op(T) (V^H X)), two FP64 MFMA
@@ -626,10 +631,13 @@ DEV void blk_stage_reflectors(const Ctx& c, cd* Vt, const cd* A, int n, int kb, 
 // leading dimension ldc_), on FP64 MFMA 16x16x4: output tiles of 16 columns,
 // one per wavefront in turn.
 template <class LB>
-DEV void blk_vh_times(const Ctx& c, cd* C, int ldc_, const cd* Vt, int n, int r0, int c0, int m, LB lb) {
+DEV void blk_vh_times(const Ctx& c, cd* C, int ldc_, const cd* Vt, int n, int r0, int c0, int m, LB lb,
+                      Part pt = Part{}) {
     const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
     const int ntj = (m + 15) >> 4;
-    for (int tj = c.wave(); tj < ntj; tj += c.nwaves()) {
+    // c0 is a multiple of 16: tile tj holds the columns of tile c0 / 16 + tj
+    const int t0 = ((pt.i - (c0 >> 4)) % pt.n + pt.n) % pt.n;
+    for (int tj = t0 + pt.n * c.wave(); tj < ntj; tj += pt.n * c.nwaves()) {
         const int col = c0 + 16 * tj + li;
         const bool cin = 16 * tj + li < m;
         d4 rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ri = rr, ir = rr;
@@ -688,8 +696,9 @@ DEV void blk_build_t(const Ctx& c, cd* T, const cd* Gm, const cd* tau, int kb, i
 }
 
 // W[:, c0 : c0 + m] <- op(T) W, op(T) = T (herm false) or T^H; W with leading dimension n.
-DEV void blk_t_times(const Ctx& c, cd* W, const cd* T, int n, int c0, int m, bool herm) {
+DEV void blk_t_times(const Ctx& c, cd* W, const cd* T, int n, int c0, int m, bool herm, Part pt = Part{}) {
     for (int j = c.tid(); j < m; j += c.nthreads()) {
+        if (!own_col(pt, c0 + j)) continue;
         cd w[kBlk], o[kBlk];
 #pragma unroll
         for (int b = 0; b < kBlk; ++b) w[b] = W[b * n + c0 + j];
@@ -714,12 +723,14 @@ DEV void blk_t_times(const Ctx& c, cd* W, const cd* T, int n, int c0, int m, boo
 // columns col in [c_lo, n) (16-aligned tiles; Ut and Wr with leading dimension
 // n, cols <= 16), on FP64 MFMA: the shape of the delayed-update flush.
 DEV void blk_rank_update(const Ctx& c, cd* X, const cd* Ut, const cd* Wr, int n, int cols, int r_lo, int r_hi,
-                         int c_lo) {
+                         int c_lo, Part pt = Part{}) {
     const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
-    const int ti0 = r_lo >> 4, ti1 = (r_hi + 15) >> 4, tj0 = c_lo >> 4, tj1 = (n + 15) >> 4;
-    const int nti = ti1 - ti0, ntj = tj1 - tj0;
+    const int ti0 = r_lo >> 4, ti1 = (r_hi + 15) >> 4, tj1 = (n + 15) >> 4;
+    // own column tiles tj = tjs, tjs + pt.n, ... (tjs the first >= c_lo / 16)
+    const int tjs = (c_lo >> 4) + (((pt.i - (c_lo >> 4)) % pt.n + pt.n) % pt.n);
+    const int nti = ti1 - ti0, ntj = tjs < tj1 ? (tj1 - tjs + pt.n - 1) / pt.n : 0;
     for (int t = c.wave(); t < nti * ntj; t += c.nwaves()) {
-        const int ti = ti0 + t / ntj, tj = tj0 + t % ntj;
+        const int ti = ti0 + t / ntj, tj = tjs + pt.n * (t % ntj);
         const int ar = 16 * ti + li, bc = 16 * tj + li;
         cd a[4], b[4], g[4];
 #pragma unroll
@@ -754,7 +765,8 @@ DEV void blk_rank_update(const Ctx& c, cd* X, const cd* Ut, const cd* Wr, int n,
 
 // One block of reflectors applied from the left to columns [c0, n) of X, rows
 // [kb, n): X <- (I - V op(T) V^H) X with op(T) = T^H (herm, for Q^H) or T.
-DEV void blk_apply(const Ctx& c, cd* X, const cd* A, const cd* tau, int n, int kb, int c0, bool herm, cd* work) {
+DEV void blk_apply(const Ctx& c, cd* X, const cd* A, const cd* tau, int n, int kb, int c0, bool herm, cd* work,
+                   Part pt = Part{}) {
     const int nb = n - kb < kBlk ? n - kb : kBlk;
     cd* Vt = work;
     cd* W = work + (size_t)kBlk * n;
@@ -763,21 +775,28 @@ DEV void blk_apply(const Ctx& c, cd* X, const cd* A, const cd* tau, int n, int k
     blk_stage_reflectors(c, Vt, A, n, kb, nb);
     blk_vh_times(c, Gm, kBlk, Vt, n, kb, 0, kBlk, [&](int r, int j) { return Vt[j * n + r]; });
     blk_build_t(c, T, Gm, tau, kb, nb);
-    blk_vh_times(c, W, n, Vt, n, kb, c0, n - c0, [&](int r, int j) { return X[r * n + j]; });
-    blk_t_times(c, W, T, n, c0, n - c0, herm);
-    blk_rank_update(c, X, Vt, W, n, nb, kb, n, c0);
+    blk_vh_times(c, W, n, Vt, n, kb, c0, n - c0, [&](int r, int j) { return X[r * n + j]; }, pt);
+    blk_t_times(c, W, T, n, c0, n - c0, herm, pt);
+    blk_rank_update(c, X, Vt, W, n, nb, kb, n, c0, pt);
 }
 
 // Q = H(0) ... H(n-1) formed explicitly (ZUNGQR); blocked with a work buffer
 // of blk_work_elems(n) complex for n >= kBlkMinN (Q <- Q_b Q from the last
 // block to the first; block kb touches columns >= kb only, the others are
 // still those of the identity there).
-DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n, cd* work = nullptr) {
-    mat_identity(c, Q, n);
+// A Part (blocked form only) forms the part's column tiles of Q.
+DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n, cd* work = nullptr, Part pt = Part{}) {
     if (work && n >= kBlkMinN) {
-        for (int kb = ((n - 1) / kBlk) * kBlk; kb >= 0; kb -= kBlk) blk_apply(c, Q, A, tau, n, kb, kb, false, work);
+        for (int e = c.tid(); e < n * n; e += c.nthreads()) {
+            const int r = e / n, k = e - r * n;
+            if (own_col(pt, k)) Q[e] = cd{r == k ? 1.0 : 0.0, 0.0};
+        }
+        c.sync();
+        for (int kb = ((n - 1) / kBlk) * kBlk; kb >= 0; kb -= kBlk)
+            blk_apply(c, Q, A, tau, n, kb, kb, false, work, pt);
         return;
     }
+    mat_identity(c, Q, n);
     const int tid = c.tid(), nth = c.nthreads();
     const int grp = tid / kGroup, gl = tid % kGroup, ngrp = nth / kGroup;
     for (int k = n - 1; k >= 0; --k) {
@@ -813,9 +832,11 @@ DEVFN void form_q(const Ctx& c, cd* Q, const cd* A, const cd* tau, int n, cd* wo
 }
 
 // X <- Q^H X (ZUNMQR 'L','C'); blocked as form_q with a work buffer.
-DEVFN void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n, cd* work = nullptr) {
+// A Part (blocked form only) updates the part's column tiles of X.
+DEVFN void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n, cd* work = nullptr,
+                         Part pt = Part{}) {
     if (work && n >= kBlkMinN) {
-        for (int kb = 0; kb < n; kb += kBlk) blk_apply(c, X, A, tau, n, kb, 0, true, work);
+        for (int kb = 0; kb < n; kb += kBlk) blk_apply(c, X, A, tau, n, kb, 0, true, work, pt);
         return;
     }
     const int tid = c.tid(), nth = c.nthreads();
@@ -856,11 +877,13 @@ DEVFN void apply_qh_left(const Ctx& c, cd* X, const cd* A, const cd* tau, int n,
 // work buffer (kBlk n complex) and n >= kBlkMinN: row blocks of kBlk from the
 // bottom, each solved in registers (one thread per column of X) and then
 // removed from the rows above by one MFMA product.
-DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n, cd* work = nullptr) {
+// A Part (blocked form only) solves for the part's column tiles of X.
+DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n, cd* work = nullptr, Part pt = Part{}) {
     if (work && n >= kBlkMinN) {
         for (int r0 = ((n - 1) / kBlk) * kBlk; r0 >= 0; r0 -= kBlk) {
             const int nb = n - r0 < kBlk ? n - r0 : kBlk;
             for (int j = c.tid(); j < n; j += c.nthreads()) {
+                if (!own_col(pt, j)) continue;
                 cd x[kBlk];
 #pragma unroll
                 for (int a = 0; a < kBlk; ++a) x[a] = a < nb ? X[(r0 + a) * n + j] : cd{0.0, 0.0};
@@ -884,7 +907,7 @@ DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n, cd* work = n
                 work[e] = (k < nb && r < r0) ? A[r * n + r0 + k] : cd{0.0, 0.0};
             }
             c.sync();
-            blk_rank_update(c, X, work, X + (size_t)r0 * n, n, nb, 0, r0, 0);
+            blk_rank_update(c, X, work, X + (size_t)r0 * n, n, nb, 0, r0, 0, pt);
         }
         return;
     }
@@ -903,21 +926,22 @@ DEVFN void trsm_upper_left(const Ctx& c, cd* X, const cd* A, int n, cd* work = n
 }
 
 // Y = V * X[perm, :]   (ZLAPMR forward + ZTRMM 'L','U','N'); V = upper triangle of A.
-DEVFN void trmm_perm_rows(const Ctx& c, cd* Y, const cd* A, const cd* X, const int* perm, int n) {
+DEVFN void trmm_perm_rows(const Ctx& c, cd* Y, const cd* A, const cd* X, const int* perm, int n, Part pt = Part{}) {
     auto la = [&](int r, int k) { return csel(k >= r, A[r * n + k], cd{0.0, 0.0}); };
     auto lb = [&](int k, int j) { return X[perm[k] * n + j]; };
     auto st = [&](int r, int j, cd v) { Y[r * n + j] = v; };
-    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st);
-    else mfma_gemm(c, n, la, lb, st);
+    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st, pt);
+    else mfma_gemm(c, n, la, lb, st, pt);
 }
 
 // Y = X[:, perm] * V^H  (ZLAPMT forward + ZTRMM 'R','U','C').
-DEVFN void perm_cols_times_vh(const Ctx& c, cd* Y, const cd* X, const cd* A, const int* perm, int n) {
+DEVFN void perm_cols_times_vh(const Ctx& c, cd* Y, const cd* X, const cd* A, const int* perm, int n,
+                              Part pt = Part{}) {
     auto la = [&](int r, int k) { return X[r * n + perm[k]]; };
     auto lb = [&](int k, int j) { return csel(k >= j, conj(A[j * n + k]), cd{0.0, 0.0}); };
     auto st = [&](int r, int j, cd v) { Y[r * n + j] = v; };
-    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st);
-    else mfma_gemm(c, n, la, lb, st);
+    if (n > kPairMinN) mfma_gemm2(c, n, la, lb, st, pt);
+    else mfma_gemm(c, n, la, lb, st, pt);
 }
 
 // ------------------------------------ column-parallel QR family (LDS, n <= 96)

@@ -1118,6 +1118,68 @@ DEVFN void lds_gemm(const Ctx& c, cd* M_, const double* Ez, int n, bool left) {
 // ---------------------------------------------------------------- stabilization
 // QRP of A (already holding U D or U^H D) and the new factors; A may be LDS.
 // stab_UR (src/number_conserving/src/stabilization.f90:69-93).
+// ---- the stabilisation on a team (device-memory path): the pivoted QR runs
+// in the leader, which leaves tau, D and the pivots (4n doubles) in the
+// chain's work matrix w2 for its partner (w2 is free from the QR until the
+// partner has read them); the products, the reflector blocks and the
+// triangular solve are split by tiles or column tiles.  Each member's blocked
+// routines use a work buffer of their own in w3 (blk_work_elems each), so the
+// split form needs the blocked routines and both buffers in w3.
+DEV bool team_wrap_fits(int n) { return n >= kBlkMinN && 2 * blk_work_elems(n) <= (size_t)n * n; }
+
+DEV cd* team_work(const ChainDev& ch, const Team& team, int n) {
+    return ch.w3 + (size_t)team.part * blk_work_elems(n);
+}
+
+DEV void qrp_team(const Ctx& c, const ChainDev& ch, const Lds& L, cd* A, int n, const Team& team) {
+    double* pub = (double*)ch.w2;
+    if (team.lead()) {
+        qrp(c, A, n, L.qr, L.dtmp);
+        for (int k = c.tid(); k < n; k += c.nthreads()) {
+            pub[2 * k] = L.qr.tau[k].r;
+            pub[2 * k + 1] = L.qr.tau[k].i;
+            pub[2 * n + k] = L.dtmp[k];
+            pub[3 * n + k] = (double)L.qr.perm[k];
+        }
+    }
+    team_sync(c, team);
+    if (!team.lead()) {
+        for (int k = c.tid(); k < n; k += c.nthreads()) {
+            L.qr.tau[k] = cd{pub[2 * k], pub[2 * k + 1]};
+            L.dtmp[k] = pub[2 * n + k];
+            L.qr.perm[k] = (int)pub[3 * n + k];
+        }
+    }
+    c.sync();
+}
+
+DEVFN void stab_ur_team(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* A, const Team& team) {
+    const int n = m.n;
+    const Part sh = team.share();
+    qrp_team(c, ch, L, A, n, team);
+    trmm_perm_rows(c, ch.w1, A, ch.vr, L.qr.perm, n, sh);
+    team_sync(c, team);
+    mat_copy(c, ch.vr, ch.w1, n, false, sh);
+    form_q(c, ch.ur, A, L.qr.tau, n, team_work(ch, team, n), sh);
+    if (team.lead()) vec_copy(c, ch.dr, L.dtmp, n);
+    team_sync(c, team);
+}
+
+DEVFN void stab_ul_team(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* A, const Team& team) {
+    const int n = m.n;
+    const Part sh = team.share();
+    qrp_team(c, ch, L, A, n, team);
+    perm_cols_times_vh(c, ch.w1, ch.vl, A, L.qr.perm, n, sh);
+    team_sync(c, team);
+    mat_copy(c, ch.vl, ch.w1, n, false, sh);
+    team_sync(c, team);
+    form_q(c, ch.w1, A, L.qr.tau, n, team_work(ch, team, n), sh);
+    team_sync(c, team);
+    mat_copy(c, ch.ul, ch.w1, n, true, sh);
+    if (team.lead()) vec_copy(c, ch.dl, L.dtmp, n);
+    team_sync(c, team);
+}
+
 DEVFN void stab_ur_from(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, cd* A) {
     const int n = m.n;
     const bool cols = col_path(c, n, A == L.mat);
@@ -1148,9 +1210,9 @@ DEVFN void stab_ul_from(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
 }
 
 // A <- U diag(D) (conj_t = false) or U^H diag(D) (conj_t = true).
-DEV void load_ud(const Ctx& c, cd* A, const cd* U, const double* D, int n, bool conj_t) {
+DEV void load_ud(const Ctx& c, cd* A, const cd* U, const double* D, int n, bool conj_t, Part pt = Part{}) {
     const int nn = n * n;
-    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+    for (int e = c.tid() + pt.i * c.nthreads(); e < nn; e += pt.n * c.nthreads()) {
         const int r = e / n, k = e - r * n;
         const cd u = conj_t ? conj(U[k * n + r]) : U[e];
         A[e] = u * cd{D[k], 0.0};
@@ -1213,6 +1275,56 @@ DEVFN double stab_green(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
     return -s;
 }
 
+// stab_green on a team: the same steps as stab_green's device-memory form.
+DEVFN double stab_green_team(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt,
+                             const cd* ur, const double* dr, const cd* vr, const cd* ul, const double* dl,
+                             const cd* vl, cd* A, cd* Gout, const Team& team) {
+    const int n = m.n, nn = n * n;
+    const Part sh = team.share();
+    const bool upper = nt > m.ltrot / 2;
+    gemm_cc(c, ch.w1, vr, false, vl, false, n, sh);
+    gemm_cc(c, ch.w2, ur, true, ul, true, n, sh);
+    team_sync(c, team);
+    for (int e = c.tid() + sh.i * c.nthreads(); e < nn; e += sh.n * c.nthreads()) {
+        const int r = e / n, k = e - r * n;
+        if (!upper) {
+            A[e] = ch.w2[e] - cd{dr[r] * dl[k], 0.0} * ch.w1[e];
+        } else {
+            const int t = k * n + r;
+            A[e] = conj(ch.w2[t] - cd{dr[k] * dl[r], 0.0} * ch.w1[t]);
+        }
+    }
+    team_sync(c, team);
+    qrp_team(c, ch, L, A, n, team);
+    mat_copy(c, ch.w1, upper ? ul : ur, n, !upper, sh);
+    team_sync(c, team);
+    cd* const work = team_work(ch, team, n);
+    apply_qh_left(c, ch.w1, A, L.qr.tau, n, work, sh);
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int r = e / n, k = e - r * n;
+        if (own_col(sh, k)) ch.w1[e] = ch.w1[e] * cd{1.0 / L.dtmp[r], 0.0};
+    }
+    c.sync();
+    trsm_upper_left(c, ch.w1, A, n, work, sh);
+    for (int e = c.tid(); e < nn; e += c.nthreads()) {
+        const int r = e / n, k = e - r * n;
+        if (own_col(sh, k)) ch.w2[L.qr.perm[r] * n + k] = ch.w1[e];
+    }
+    team_sync(c, team);
+    if (!upper) {
+        gemm_cc(c, Gout, ul, true, ch.w2, false, n, sh);
+    } else {
+        gemm_cc(c, ch.w3, ur, false, ch.w2, false, n, sh);
+        team_sync(c, team);
+        mat_copy(c, Gout, ch.w3, n, true, sh);
+    }
+    team_sync(c, team);
+    double s = 0.0;
+    for (int k = c.lane(); k < n; k += 64) s += log(L.dtmp[k]);
+    s = wave_sum(c, s);
+    return -s;
+}
+
 // Drift of the wrapped G against the recomputed one (compare_mat,
 // stabilization.f90:256-266) and the last log|det G|.
 DEV void store_drift_and_logdet(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L,
@@ -1246,18 +1358,18 @@ DEV void live_identity(const Ctx& c, int n, cd* u, double* d, cd* v) {
     vec_fill(c, d, n, 1.0);
 }
 
-DEV void store_right(const Ctx& c, const ModelDev& m, const ChainDev& ch, int st) {
+DEV void store_right(const Ctx& c, const ModelDev& m, const ChainDev& ch, int st, Part pt = Part{}) {
     const int n = m.n;
-    mat_copy(c, list_mat(ch.urlist, st, n), ch.ur, n, false);
-    mat_copy(c, list_mat(ch.vrlist, st, n), ch.vr, n, false);
-    vec_copy(c, list_vec(ch.drlist, st, n), ch.dr, n);
+    mat_copy(c, list_mat(ch.urlist, st, n), ch.ur, n, false, pt);
+    mat_copy(c, list_mat(ch.vrlist, st, n), ch.vr, n, false, pt);
+    if (pt.i == 0) vec_copy(c, list_vec(ch.drlist, st, n), ch.dr, n);
 }
 
-DEV void store_left(const Ctx& c, const ModelDev& m, const ChainDev& ch, int st) {
+DEV void store_left(const Ctx& c, const ModelDev& m, const ChainDev& ch, int st, Part pt = Part{}) {
     const int n = m.n;
-    mat_copy(c, list_mat(ch.ullist, st, n), ch.ul, n, false);
-    mat_copy(c, list_mat(ch.vllist, st, n), ch.vl, n, false);
-    vec_copy(c, list_vec(ch.dllist, st, n), ch.dl, n);
+    mat_copy(c, list_mat(ch.ullist, st, n), ch.ul, n, false, pt);
+    mat_copy(c, list_mat(ch.vllist, st, n), ch.vl, n, false, pt);
+    if (pt.i == 0) vec_copy(c, list_vec(ch.dllist, st, n), ch.dl, n);
 }
 
 // ---------------------------------------------------------------- measurement
@@ -1493,9 +1605,29 @@ DEVFN void measure_lds(const Ctx& c, const ModelDev& m, const ChainDev& ch, cons
 
 // ====================================================================== global path
 // Wrap_R (stabilization.f90:330-366), nt % Nwrap == 0, nt != 0.
-DEVFN void wrap_r_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
+DEVFN void wrap_r_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt,
+                         const Team& team = Team{}) {
     const int n = m.n, st = nt / m.nwrap;
     Timer tm;
+    if (team.parts > 1) {
+        const Part sh = team.share();
+        tm.on = team.lead();
+        tm.start(c);
+        load_ud(c, ch.wa, ch.ur, ch.dr, n, false, sh);
+        team_sync(c, team);
+        stab_ur_team(c, m, ch, L, ch.wa, team);
+        tm.stop(c, ch, S_T_WQR);
+        const double ld = stab_green_team(c, m, ch, L, nt, ch.ur, ch.dr, ch.vr, list_mat(ch.ullist, st, n),
+                                          list_vec(ch.dllist, st, n), list_mat(ch.vllist, st, n), ch.wa, ch.wa,
+                                          team);
+        tm.stop(c, ch, S_T_WGREEN);
+        if (team.lead()) store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
+        team_sync(c, team);
+        mat_copy(c, ch.g, ch.wa, n, false, sh);
+        store_right(c, m, ch, st, sh);
+        team_sync(c, team);
+        return;
+    }
     tm.start(c);
     load_ud(c, ch.wa, ch.ur, ch.dr, n, false);
     stab_ur_from(c, m, ch, L, ch.wa);
@@ -1509,9 +1641,28 @@ DEVFN void wrap_r_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
 }
 
 // Wrap_L (stabilization.f90:292-328), nt % Nwrap == 0, nt != Ltrot.
-DEVFN void wrap_l_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt) {
+DEVFN void wrap_l_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt,
+                         const Team& team = Team{}) {
     const int n = m.n, st = nt / m.nwrap;
     Timer tm;
+    if (team.parts > 1) {
+        const Part sh = team.share();
+        tm.on = team.lead();
+        tm.start(c);
+        load_ud(c, ch.wa, ch.ul, ch.dl, n, true, sh);
+        team_sync(c, team);
+        stab_ul_team(c, m, ch, L, ch.wa, team);
+        tm.stop(c, ch, S_T_WQR);
+        const double ld = stab_green_team(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
+                                          list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, ch.wa, ch.wa, team);
+        tm.stop(c, ch, S_T_WGREEN);
+        if (team.lead()) store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
+        team_sync(c, team);
+        mat_copy(c, ch.g, ch.wa, n, false, sh);
+        store_left(c, m, ch, st, sh);
+        team_sync(c, team);
+        return;
+    }
     tm.start(c);
     load_ud(c, ch.wa, ch.ul, ch.dl, n, true);
     stab_ul_from(c, m, ch, L, ch.wa);
@@ -1522,6 +1673,20 @@ DEVFN void wrap_l_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, co
     store_drift_and_logdet(c, m, ch, L, ch.wa, ch.g, ld);
     mat_copy(c, ch.g, ch.wa, n, false);
     store_left(c, m, ch, st);
+}
+
+// A wrap on a team: split where team_wrap_fits, in the leader alone
+// otherwise.  The caller's team barrier follows.
+DEV void wrap_global_team(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int nt, bool right,
+                          const Team& team) {
+    if (team.parts > 1 && !team_wrap_fits(m.n)) {
+        if (!team.lead()) return;
+        if (right) wrap_r_global(c, m, ch, L, nt);
+        else wrap_l_global(c, m, ch, L, nt);
+        return;
+    }
+    if (right) wrap_r_global(c, m, ch, L, nt, team);
+    else wrap_l_global(c, m, ch, L, nt, team);
 }
 
 DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
@@ -1551,9 +1716,10 @@ DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
 }
 
 // With a team of two workgroups (submatrix updates only, dk > 0; no grad or
-// frozen pass) the products and scalings are split by the team, the wraps and
-// the estimators run in the leader, and a team barrier follows every phase
-// that writes device memory the other member reads next.  A single
+// frozen pass) the products and scalings are split by the team, the wraps
+// split all but their pivoted QRs, the estimators run in the leader, and a
+// team barrier follows every phase that writes device memory the other
+// member reads next.  A single
 // workgroup computes the same values in the same order.
 DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, int& seed,
                            bool meas, double* grad = nullptr, bool frozen = false, const Team& team = Team{}) {
@@ -1566,7 +1732,7 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
     tm.start(c);
     for (int nt = m.ltrot; nt >= 1; --nt) {
         if (nt % m.nwrap == 0 && nt != m.ltrot) {
-            if (team.lead()) wrap_l_global(c, m, ch, L, nt);
+            wrap_global_team(c, m, ch, L, nt, false, team);
             team_sync(c, team);
             tm.stop(c, ch, S_T_WRAP);
         }
@@ -1598,7 +1764,7 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
             tm.stop(c, ch, S_T_HS);
         }
     }
-    if (team.lead()) wrap_l_global(c, m, ch, L, 0);
+    wrap_global_team(c, m, ch, L, 0, false, team);
     team_sync(c, team);
     tm.stop(c, ch, S_T_WRAP);
 }
@@ -1638,7 +1804,7 @@ DEV void sweep_right_global(const Ctx& c, const ModelDev& m, const ChainDev& ch,
         team_sync(c, team);
         tm.stop(c, ch, S_T_KIN);
         if (nt % m.nwrap == 0) {
-            if (team.lead()) wrap_r_global(c, m, ch, L, nt);
+            wrap_global_team(c, m, ch, L, nt, true, team);
             team_sync(c, team);
             tm.stop(c, ch, S_T_WRAP);
         }

@@ -79,6 +79,14 @@ def test_emu_paired_workgroups_equal_single(emu_lib, case, width, nchains, monke
     check_pairs_bitwise(emu_lib, case, width, nchains, monkeypatch)
 
 
+def test_emu_paired_workgroups_split_stabilisation(emu_lib, monkeypatch):
+    """N = 81: large enough for the pair to split the stabilisation (blocked
+    reflectors, both members' work buffers), the QR factors passed from the
+    leader to its partner through device memory."""
+    d = dict(pairing=False, u1=1.0, u2=-0.5, mu=-5.0, nlx=9, nly=9, ltrot=4, beta=0.4, nwrap=2, shift_loc=1.5)
+    check_pairs_bitwise(emu_lib, d, 16, 2, monkeypatch)
+
+
 def test_wide_updates_rejected_on_the_lds_path(emu_lib):
     with pytest.raises(RuntimeError, match="above 16 need the device-memory path"):
         BatchedSolver([ModelSpec(solver_params(CASES[4]))], [0], lib=emu_lib, schedule=(1, 32))

@@ -436,7 +436,7 @@ def check_pairs_bitwise(lib, case, width, nchains, monkeypatch, sweeps=2):
     """Two workgroups per chain on the device-memory path (BAFQ_PAIRS=1) give
     the same chain as one workgroup per chain (BAFQ_PAIRS=0), bit for bit:
     fields, seeds, G, every estimator and every counter except the timers."""
-    d = CASES[case]
+    d = CASES[case] if isinstance(case, int) else case
     ns = d["nlx"] * d["nly"]
     phis = [fixed_fields(d["ltrot"], ns) * (1.0 + 0.05 * k) for k in range(nchains)]
     seeds = [24680 + 11 * k for k in range(nchains)]
