@@ -456,7 +456,10 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
         bool ok = mode != 0 && h->path_count[P_GLOBAL] > 0 && h->path_count[P_LDS] == 0;
         for (int c = 0; ok && c < n_chains; ++c) ok = h->models[chain_model[c]].dk > 0;
         if (ok) {
-            const int res = rt_resident_groups(K_SWEEP, P_GLOBAL, rt_block_threads(), h->max_lds[P_GLOBAL]);
+            // the paired kernels: sweep, rebuild, measurement pass
+            int res = rt_resident_groups(K_SWEEP, P_GLOBAL, rt_block_threads(), h->max_lds[P_GLOBAL]);
+            res = std::min(res, rt_resident_groups(K_PREPARE, P_GLOBAL, rt_block_threads(), h->max_lds[P_GLOBAL]));
+            res = std::min(res, rt_resident_groups(K_MEASURE, P_GLOBAL, rt_block_threads(), h->max_lds[P_GLOBAL]));
             const int cnt = h->path_count[P_GLOBAL];
             if (res >= 2 && (2 * cnt <= res || mode == 1)) h->npairs = std::min(cnt, res / 2);
         }
@@ -543,7 +546,7 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure,
     a.pair_mail = nullptr;
     a.pair_wait = 0;
     double total_ms = 0.0;
-    const bool paired = k == K_SWEEP && h->npairs > 0;
+    const bool paired = (k == K_SWEEP || k == K_PREPARE || k == K_MEASURE) && h->npairs > 0;
     if (k == K_WARMUP) {
         a.ids = h->d_ids;
         a.count = (int)h->chains.size();

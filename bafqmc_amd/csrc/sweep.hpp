@@ -1689,6 +1689,45 @@ DEV void wrap_global_team(const Ctx& c, const ModelDev& m, const ChainDev& ch, c
     else wrap_l_global(c, m, ch, L, nt, team);
 }
 
+// The G rebuild on a team (team_wrap_fits(n)): the slice products and
+// scalings split by tiles and elements, the stabilisations as in the wraps.
+// mat_copy, scale_mat and load_ud share one element split, so a member
+// reads only its own elements between them.
+DEV void prepare_global_team(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L, const Team& team) {
+    const int n = m.n;
+    const Part sh = team.share();
+    if (team.lead()) {
+        live_identity(c, n, ch.ur, ch.dr, ch.vr);
+        live_identity(c, n, ch.ul, ch.dl, ch.vl);
+        store_right(c, m, ch, 0);
+        store_left(c, m, ch, m.nst);
+    }
+    team_sync(c, team);
+    for (int nt = 1; nt <= m.ltrot; ++nt) {
+        const double* phi_t = ch.phi + (size_t)(nt - 1) * m.ns * 2;
+        for (int nf = 1; nf <= 2; ++nf) {
+            if (!m.active[nf - 1]) continue;
+            hs_factors(c, m, L, phi_t, nf);
+            scale_mat(c, ch.ur, n, L.f, nullptr, sh);
+        }
+        team_sync(c, team);
+        gemm_rc(c, ch.w1, m.ez[0], ch.ur, 0, n, sh);
+        team_sync(c, team);
+        mat_copy(c, ch.ur, ch.w1, n, false, sh);
+        if (nt % m.nwrap == 0) {
+            load_ud(c, ch.wa, ch.ur, ch.dr, n, false, sh);
+            team_sync(c, team);
+            stab_ur_team(c, m, ch, L, ch.wa, team);
+            store_right(c, m, ch, nt / m.nwrap, sh);
+        }
+    }
+    team_sync(c, team);
+    const double ld = stab_green_team(c, m, ch, L, m.ltrot, ch.ur, ch.dr, ch.vr, ch.ul, ch.dl, ch.vl, ch.wa, ch.g,
+                                      team);
+    if (team.lead() && c.tid() == 0) ch.stats[S_LOGDET] = ld;
+    team_sync(c, team);
+}
+
 DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L) {
     const int n = m.n;
     live_identity(c, n, ch.ur, ch.dr, ch.vr);
@@ -2028,14 +2067,19 @@ DEV Team make_team(const Ctx& c, const LaunchArgs& a, int* slot, int** mail) {
     return team;
 }
 
+// a.pairs (device-memory path): two workgroups per chain, splitting the work
+// where team_wrap_fits(n), the leader alone otherwise.
 template <bool kLds>
 DEV void body_prepare(const Ctx& c, const LaunchArgs& a, int* slot) {
-    for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
+    int* mail = nullptr;
+    const Team team = !kLds ? make_team(c, a, slot, &mail) : Team{};
+    for (int q = next_item_team(c, a, slot, team, mail); q < a.count; q = next_item_team(c, a, slot, team, mail)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
         const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk, kLds ? 0 : m.dk);
         if (kLds) prepare_lds(c, m, ch, L);
-        else prepare_global(c, m, ch, L);
+        else if (team.parts > 1 && team_wrap_fits(m.n)) prepare_global_team(c, m, ch, L, team);
+        else if (team.lead()) prepare_global(c, m, ch, L);
     }
 }
 
@@ -2144,6 +2188,44 @@ DEV void measure_checkpoints(const Ctx& c, const ModelDev& m, const ChainDev& ch
     }
 }
 
+// measure_checkpoints on a team (device-memory path, team_wrap_fits(n)): the
+// estimators in the leader, everything else split as in prepare_global_team.
+DEV void measure_checkpoints_team(const Ctx& c, const ModelDev& m, const ChainDev& ch, const Lds& L,
+                                  const Team& team) {
+    const int n = m.n;
+    const Part sh = team.share();
+    if (team.lead()) {
+        measure(c, m, ch, L, ch.g);
+        live_identity(c, n, ch.ul, ch.dl, ch.vl);
+    }
+    team_sync(c, team);
+    for (int st = m.nst - 1; st >= 0; --st) {
+        const int nt = st * m.nwrap;
+        for (int t = nt + m.nwrap; t > nt; --t) {
+            const double* phi_t = ch.phi + (size_t)(t - 1) * m.ns * 2;
+            gemm_rc(c, ch.w1, m.ez[2], ch.ul, 1, n, sh);          // UL <- UL e^{-dtau K}
+            team_sync(c, team);
+            mat_copy(c, ch.ul, ch.w1, n, false, sh);
+            for (int nf = 2; nf >= 1; --nf) {
+                if (!m.active[nf - 1]) continue;
+                hs_factors(c, m, L, phi_t, nf);
+                scale_mat(c, ch.ul, n, nullptr, L.f, sh);         // UL <- UL e^{V}
+            }
+            team_sync(c, team);
+        }
+        load_ud(c, ch.wa, ch.ul, ch.dl, n, true, sh);
+        team_sync(c, team);
+        stab_ul_team(c, m, ch, L, ch.wa, team);
+        const double ld = stab_green_team(c, m, ch, L, nt, list_mat(ch.urlist, st, n), list_vec(ch.drlist, st, n),
+                                          list_mat(ch.vrlist, st, n), ch.ul, ch.dl, ch.vl, ch.wa, ch.g, team);
+        store_left(c, m, ch, st, sh);
+        if (team.lead() && c.tid() == 0) ch.stats[S_LOGDET] = ld;
+        team_sync(c, team);
+        if (st > 0 && team.lead()) measure(c, m, ch, L, ch.g);
+    }
+    team_sync(c, team);
+}
+
 // Measurement pass: one left sweep with the fields frozen (no proposals, the
 // RNG stream untouched) measuring the equal-time estimators at every slice,
 // as sweep_L of Local_sweep measures them (local_sweep.f90:96-115) -- the
@@ -2152,9 +2234,12 @@ DEV void measure_checkpoints(const Ctx& c, const ModelDev& m, const ChainDev& ch
 // a.measure == 3 measures the current Green's function once (no pass: the
 // state is left as it is).  Otherwise G, the UDV factors and the checkpoint
 // lists end in the state sweep_L leaves them in.
+// a.pairs (device-memory path): two workgroups per chain as in body_prepare.
 template <bool kLds>
 DEV void body_measure(const Ctx& c, const LaunchArgs& a, int* slot) {
-    for (int q = next_item(c, a.queue, slot); q < a.count; q = next_item(c, a.queue, slot)) {
+    int* mail = nullptr;
+    const Team team = !kLds ? make_team(c, a, slot, &mail) : Team{};
+    for (int q = next_item_team(c, a, slot, team, mail); q < a.count; q = next_item_team(c, a, slot, team, mail)) {
         const ChainDev& ch = a.chains[a.ids[q]];
         const ModelDev& m = a.models[ch.model];
         const Lds L = carve(c.smem, m.n, c.nwaves(), kLds, m.uw_glb ? 0 : m.dk, kLds ? 0 : m.dk);
@@ -2164,20 +2249,22 @@ DEV void body_measure(const Ctx& c, const LaunchArgs& a, int* slot) {
             if (kLds) {
                 lds_from_global(c, L.mat, ch.g, m.n);
                 measure_lds(c, m, ch, L, L.mat);
-            } else {
+            } else if (team.lead()) {
                 measure(c, m, ch, L, ch.g);
             }
             c.sync();
             continue;
         }
+        const bool split = team.parts > 1 && team_wrap_fits(m.n);
         for (int s = 0; s < a.nsweep; ++s) {
             if (kLds) lds_from_global(c, L.mat, ch.g, m.n);
             if (a.measure == 2) {
-                measure_checkpoints<kLds>(c, m, ch, L);
+                if (split) measure_checkpoints_team(c, m, ch, L, team);
+                else if (team.lead()) measure_checkpoints<kLds>(c, m, ch, L);
             } else if (kLds) {
                 sweep_left_lds(c, m, ch, L, seed, true, nullptr, true);
             } else {
-                sweep_left_global(c, m, ch, L, seed, true, nullptr, true);
+                sweep_left_global(c, m, ch, L, seed, true, nullptr, true, team);
             }
             if (kLds) global_from_lds(c, ch.g, L.mat, m.n);
         }

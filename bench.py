@@ -138,11 +138,11 @@ def net_flops(desc, batch):
     return tot * batch
 
 
-def measured_traffic(cfg, kernel, nwrap, chains=None, schedule=None):
+def measured_traffic(cfg, kernel, nwrap, chains=None, schedule=None, pairs=None):
     """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (copy-calibrated) of this workload at this Nwrap (and,
-    where the summary records them, this chain count and schedule), when there
-    is one."""
+    where the summary records them, this chain count, schedule and number of
+    workgroup pairs), when there is one."""
     if "slug" not in cfg:
         return None
     f = ROOT / "profiles" / f"{kernel}_{cfg['slug']}_hbm.json"
@@ -154,6 +154,8 @@ def measured_traffic(cfg, kernel, nwrap, chains=None, schedule=None):
     if chains is not None and rec.get("chains", chains) != chains:
         return None
     if schedule is not None and "schedule" in rec and list(rec["schedule"]) != list(schedule):
+        return None
+    if pairs is not None and rec.get("pairs", 0) != pairs:
         return None
     return rec["traffic_bytes_per_launch"]
 
@@ -469,7 +471,7 @@ def run_plain(args, cfg, p, spec, nchains):
     a = statistics(sf, wall, args.steps, "plain")
     kern = "k_sweep_lds" if s.schedule()[0][0] == 1 else "k_sweep"
     rf = roofline(kern, flops_per_chain_sweep(p.n, p.ns, p.ltrot, p.nwrap, nch_of(p)) * nchains, kms)
-    rf["traffic"] = measured_traffic(cfg, kern, p.nwrap, nchains, s.schedule()[0])
+    rf["traffic"] = measured_traffic(cfg, kern, p.nwrap, nchains, s.schedule()[0], s.pairs())
     out = {
         "metric": METRIC, "value": a["eff_samples_per_s"], "unit": "effective S_SF(K) samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": a["ms_per_step"],
@@ -480,6 +482,7 @@ def run_plain(args, cfg, p, spec, nchains):
                    "unstable_chains": int(np.sum(st[:, 4] >= UNSTABLE_DRIFT)), "kernel": kern,
                    "phase_ms_per_chain_sweep": phases,
                    "schedule": {"path": s.schedule()[0][0], "delayed": s.schedule()[0][1],
+                                "workgroup_pairs": s.pairs(),
                                 "autotune_ms_per_sweep": tuned.table if tuned else None}},
         "roofline": rf,
     }

@@ -73,14 +73,15 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
                           int n_chains, const bafq_schedule* schedule, bafq_handle** out);
 /* The schedule in effect for a model. */
 int bafq_get_schedule(const bafq_handle* h, int model, bafq_schedule* out);
-/* Pairs of workgroups that share a chain in bafq_sweep's device-memory-path
- * kernel (0: one workgroup per chain).  No counterpart in the reference (one
- * chain per MPI rank).  bafq_create pairs the workgroups when every
- * device-memory-path chain then still has its own resident pair, the batch
- * has no LDS-path chains and every model uses submatrix updates (delayed > 0);
- * the environment variable BAFQ_PAIRS=0 turns this off, BAFQ_PAIRS=1 pairs
- * as many workgroups as are resident at once.  Results are identical to
- * the unpaired kernel's, bit for bit. */
+/* Pairs of workgroups that share a chain in the device-memory-path kernels
+ * of bafq_sweep, bafq_prepare and bafq_measure_pass (0: one workgroup per
+ * chain).  No counterpart in the reference (one chain per MPI rank).
+ * bafq_create pairs the workgroups when every device-memory-path chain then
+ * still has its own resident pair, the batch has no LDS-path chains and
+ * every model uses submatrix updates (delayed > 0); the environment variable
+ * BAFQ_PAIRS=0 turns this off, BAFQ_PAIRS=1 pairs as many workgroups as are
+ * resident at once.  Results are identical to the unpaired kernels', bit for
+ * bit. */
 int bafq_pairs(const bafq_handle* h);
 
 /* Sizes: matrix dimension N of a chain, doubles per chain in bafq_read_bin's

@@ -435,7 +435,8 @@ def test_uneven_workload_batch_equals_single_chains(hip_lib):
 def check_pairs_bitwise(lib, case, width, nchains, monkeypatch, sweeps=2):
     """Two workgroups per chain on the device-memory path (BAFQ_PAIRS=1) give
     the same chain as one workgroup per chain (BAFQ_PAIRS=0), bit for bit:
-    fields, seeds, G, every estimator and every counter except the timers."""
+    fields, seeds, G, every estimator and every counter except the timers,
+    over the rebuild, the sweeps and the three measurement passes."""
     d = CASES[case] if isinstance(case, int) else case
     ns = d["nlx"] * d["nly"]
     phis = [fixed_fields(d["ltrot"], ns) * (1.0 + 0.05 * k) for k in range(nchains)]
@@ -453,6 +454,8 @@ def check_pairs_bitwise(lib, case, width, nchains, monkeypatch, sweeps=2):
         s.prepare()
         for _ in range(sweeps):
             s.sweep(1)
+        for pm in ("checkpoints", "all", "current"):
+            s.measure_pass(1, mode=pm)
         obs, st = s.read_bin_raw()
         st = np.delete(np.asarray(st), timers, axis=1)
         phi, sd = s.get_fields()
