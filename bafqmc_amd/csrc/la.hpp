@@ -329,6 +329,86 @@ DEVFN void gemm_rc(const Ctx& c, cd* C, const double* Ez, const cd* X, int side,
     c.sync();
 }
 
+// gemm_rc with X staged in LDS, blocks of 16 columns (side 0) or 16 rows
+// (side 1) of X in `panel` (cap complex; two blocks at a time where they fit,
+// each 4 ceil(n/4) x 16 complex, k-major),
+// the wavefronts sharing out the output tiles of both blocks: each element of
+// X is read from device memory once per product instead of once per output
+// tile (kSt loads per thread issued together), E (shared by every chain of the
+// model) comes from L2.  With a Part, the
+// blocks b = pt.i (mod pt.n).  The same MFMA operands in the same order as
+// gemm_rc, so the same bits.
+DEVFN void gemm_rc_panel(const Ctx& c, cd* C, const double* Ez, const cd* X, int side, int n, cd* panel_, int cap,
+                         Part pt = Part{}) {
+    const int nt = (n + 15) >> 4, nk = (n + 3) >> 2, kp = 4 * nk;
+    const int per = cap >= 2 * kp * 16 ? 2 : 1;       // blocks per round
+    const int lane = c.lane(), li = lane & 15, lk = lane >> 4;
+    auto Xg = as_global(X);
+    auto Cg = as_global(C);
+    LDS_AS cd* const panel = as_lds(panel_);
+    const int nb = (nt - pt.i + pt.n - 1) / pt.n;     // this part's blocks
+    for (int b0 = 0; b0 < nb; b0 += per) {
+        const int nbb = nb - b0 < per ? nb - b0 : per; // blocks in this round
+        // panel[h][k][j] = X[k][16 b + j] (side 0) or X[16 b + j][k] (side 1), zero beyond n
+        constexpr int kSt = 10;
+        const int tot = nbb * kp * 16, nth = c.nthreads();
+        for (int e0 = 0; e0 < tot; e0 += kSt * nth) {
+            cd v[kSt];
+#pragma unroll
+            for (int i = 0; i < kSt; ++i) {
+                const int e = e0 + c.tid() + i * nth;
+                const int h = e / (kp * 16), r = e - h * kp * 16;
+                const int b = pt.i + pt.n * (b0 + h);
+                const int k = r >> 4, g = 16 * b + (r & 15);
+                v[i] = cd{0.0, 0.0};
+                if (e < tot && k < n && g < n) v[i] = side == 0 ? ldc(Xg, k * n + g) : ldc(Xg, g * n + k);
+            }
+#pragma unroll
+            for (int i = 0; i < kSt; ++i) {
+                const int e = e0 + c.tid() + i * nth;
+                if (e < tot) stc(panel, e, v[i]);
+            }
+        }
+        c.sync();
+        for (int u = c.wave(); u < nbb * nt; u += c.nwaves()) {
+            const int h = u / nt, t = u - h * nt;
+            const int b = pt.i + pt.n * (b0 + h);
+            LDS_AS const cd* const pb = panel + (size_t)h * kp * 16;
+            // output tile (t, b) (side 0) or (b, t) (side 1); E's tile t in operand order
+            auto ez = as_global(Ez + (size_t)t * nk * 64 + lane);
+            d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = accr;
+            for (int kk0 = 0; kk0 < nk; kk0 += 8) {
+                double e[8];
+                cd x[8];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int kk = kk0 + v;
+                    e[v] = kk < nk ? ez[(size_t)kk * 64] : 0.0;
+                    x[v] = kk < nk ? ldc(pb, (4 * kk + lk) * 16 + li) : cd{0.0, 0.0};
+                }
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    if (kk0 + v < nk) {
+                        if (side == 0) {
+                            accr = c.mfma(e[v], x[v].r, accr);
+                            acci = c.mfma(e[v], x[v].i, acci);
+                        } else {
+                            accr = c.mfma(x[v].r, e[v], accr);
+                            acci = c.mfma(x[v].i, e[v], acci);
+                        }
+                    }
+                }
+            }
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * (side == 0 ? t : b) + mfma_row(lane, r);
+                const int col = 16 * (side == 0 ? b : t) + li;
+                if (row < n && col < n) stc(Cg, row * n + col, cd{accr[r], acci[r]});
+            }
+        }
+        c.sync();                                // the panel is read before the next round
+    }
+}
+
 // ------------------------------------------------------ pivoted QR (ZGEQP3)
 // Scratch in LDS: tau[n] (cd), perm[n] (int), nrm[n] (double), bcast (>= 8
 // doubles), v[2n] (cd: the current reflector, then the device-memory QR's
@@ -343,8 +423,9 @@ struct QrScratch {
 
 constexpr int kGroup = 8;   // lanes cooperating on one column
 
-// H(k)^H = I - conj(tau) v v^H applied to columns j > k of A (device memory)
-// and their new squared norms below row k.  A group of NG lanes owns a column;
+// H(k)^H = I - conj(tau) v v^H applied to columns j > k of A (device memory),
+// their new squared norms below row k (nrm) and their entries in row k + 1
+// (alpha: the next step's diagonal candidates).  A group of NG lanes owns a column;
 // its rows k + gl + NG i are held in registers (slot i, R slots, n <= NG R)
 // between the dot product and the update, so the column is read and written
 // once; v comes from LDS.  With NG = kGroup the arithmetic and its order are
@@ -435,10 +516,10 @@ DEV void qrp_apply_loop(const Ctx& c, cd* A, int n, int k, LDS_AS const cd* v, L
 // Column norms are recomputed exactly after every step (no downdating).
 // Two barriers per step: every wavefront finds the pivot itself from the
 // norms in LDS, and every thread computes the step's ZLARFG scalars itself
-// from the pivot column's norm and its diagonal entry, which the previous
-// step's update left in LDS (alpha); the column swap, the reflector's scaling
-// and its copy to LDS are one pass over the rows.  The scratch's reflector
-// buffer v holds 2n entries: v[0, n) the reflector, v[n, 2n) alpha.
+// from the pivot column's norm from row k down (nrm) and its entry in row k
+// (alpha), both left in LDS by the previous step's update; the column swap,
+// the reflector's scaling and its copy to LDS are one pass over the rows.  The scratch's reflector buffer v holds 2n
+// entries: v[0, n) the reflector, v[n, 2n) alpha.
 DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
     // the scratch arrays are in LDS: typed pointers held in registers, so that
     // stores through A (which may alias any generic address) do not force
@@ -483,7 +564,7 @@ DEVFN void qrp(const Ctx& c, cd* A, int n, const QrScratch& s, double* D) {
         }
         const int p = bi;
         // ---- Householder reflector for the pivot column (ZLARFG conventions):
-        // beta = -sign(Re alpha) |x|, |x|^2 = the column's norm below row k - 1
+        // beta = -sign(Re alpha) sqrt(|alpha|^2 + |x|^2), |x|^2 = the norm below row k
         const cd al = ldc(alpha, p);
         const double a2 = al.r * al.r + al.i * al.i;
         const double nr2 = nrm[p];

@@ -149,6 +149,12 @@ struct Lds {
     cd* finv;          // n
     cd* col;           // 2 slots x 2 sectors x n
     cd* row;           // 2 slots x 2 sectors x n
+    // the device-memory path's kinetic products stage X here (gemm_rc_panel)
+    // over col, row, red, site, prop and the delayed-update buffers to the end
+    // of the layout (panel_cap complex, at least 18 n + 256 >= 16 (n + 3)),
+    // none of which is live during those products
+    cd* panel;
+    int panel_cap;
     double* red;       // nwaves x 64
     cd* site;          // 6 x ns (pairing measurement)
     Proposal* prop;    // ns
@@ -206,9 +212,11 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0, int sb =
     L.col = (cd*)take(4 * (size_t)n * sizeof(cd));
     L.qr.v = L.col;                              // free while a QR runs
     L.row = (cd*)take(4 * (size_t)n * sizeof(cd));
+    L.panel = L.col;
     L.red = (double*)take((size_t)nwaves * 64 * sizeof(double));
     L.site = (cd*)take(6 * (size_t)n * sizeof(cd));
     L.prop = (Proposal*)take((size_t)n * sizeof(Proposal));
+    L.panel_cap = (int)((p - (char*)L.panel) / sizeof(cd));
     L.mat = with_mat ? (cd*)take((size_t)n * n * sizeof(cd)) : nullptr;
     L.du = dk > 0 ? (cd*)take((size_t)n * dk * sizeof(cd)) : nullptr;
     L.dw = dk > 0 ? (cd*)take((size_t)n * dk * sizeof(cd)) : nullptr;
@@ -216,6 +224,7 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0, int sb =
     L.sg0 = sb > 0 ? (cd*)take((size_t)sb * sb * sizeof(cd)) : nullptr;
     L.sloc = sb > 0 ? (int*)take((size_t)sb * sizeof(int)) : nullptr;
     L.sd = sb > 0 ? (cd*)take((size_t)sb * sizeof(cd)) : nullptr;
+    if (!with_mat) L.panel_cap = (int)((p - (char*)L.panel) / sizeof(cd));   // through the end
     return L;
 }
 
@@ -1711,7 +1720,7 @@ DEV void prepare_global_team(const Ctx& c, const ModelDev& m, const ChainDev& ch
             scale_mat(c, ch.ur, n, L.f, nullptr, sh);
         }
         team_sync(c, team);
-        gemm_rc(c, ch.w1, m.ez[0], ch.ur, 0, n, sh);
+        gemm_rc_panel(c, ch.w1, m.ez[0], ch.ur, 0, n, L.panel, L.panel_cap, sh);
         team_sync(c, team);
         mat_copy(c, ch.ur, ch.w1, n, false, sh);
         if (nt % m.nwrap == 0) {
@@ -1741,7 +1750,7 @@ DEV void prepare_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, con
             hs_factors(c, m, L, phi_t, nf);
             scale_mat(c, ch.ur, n, L.f, nullptr);
         }
-        gemm_rc(c, ch.w1, m.ez[0], ch.ur, 0, n);
+        gemm_rc_panel(c, ch.w1, m.ez[0], ch.ur, 0, n, L.panel, L.panel_cap);
         mat_copy(c, ch.ur, ch.w1, n, false);
         if (nt % m.nwrap == 0) {
             load_ud(c, ch.wa, ch.ur, ch.dr, n, false);
@@ -1780,10 +1789,10 @@ DEV void sweep_left_global(const Ctx& c, const ModelDev& m, const ChainDev& ch, 
             tm.stop(c, ch, S_T_MEAS);
         }
         // propT_L: G <- e^{+dtau K} G e^{-dtau K}, UL <- UL e^{-dtau K}
-        gemm_rc(c, ch.w1, m.ez[2], ch.g, 1, n, sh);
-        gemm_rc(c, ch.w2, m.ez[2], ch.ul, 1, n, sh);
+        gemm_rc_panel(c, ch.w1, m.ez[2], ch.g, 1, n, L.panel, L.panel_cap, sh);
+        gemm_rc_panel(c, ch.w2, m.ez[2], ch.ul, 1, n, L.panel, L.panel_cap, sh);
         team_sync(c, team);
-        gemm_rc(c, ch.g, m.ez[1], ch.w1, 0, n, sh);
+        gemm_rc_panel(c, ch.g, m.ez[1], ch.w1, 0, n, L.panel, L.panel_cap, sh);
         mat_copy(c, ch.ul, ch.w2, n, false, sh);
         team_sync(c, team);
         tm.stop(c, ch, S_T_KIN);
@@ -1835,10 +1844,10 @@ DEV void sweep_right_global(const Ctx& c, const ModelDev& m, const ChainDev& ch,
         }
         team_sync(c, team);
         // propT_R: G <- e^{-dtau K} G e^{+dtau K}, UR <- e^{-dtau K} UR
-        gemm_rc(c, ch.w1, m.ez[0], ch.g, 0, n, sh);
-        gemm_rc(c, ch.w2, m.ez[0], ch.ur, 0, n, sh);
+        gemm_rc_panel(c, ch.w1, m.ez[0], ch.g, 0, n, L.panel, L.panel_cap, sh);
+        gemm_rc_panel(c, ch.w2, m.ez[0], ch.ur, 0, n, L.panel, L.panel_cap, sh);
         team_sync(c, team);
-        gemm_rc(c, ch.g, m.ez[3], ch.w1, 1, n, sh);
+        gemm_rc_panel(c, ch.g, m.ez[3], ch.w1, 1, n, L.panel, L.panel_cap, sh);
         mat_copy(c, ch.ur, ch.w2, n, false, sh);
         team_sync(c, team);
         tm.stop(c, ch, S_T_KIN);
@@ -2165,7 +2174,7 @@ DEV void measure_checkpoints(const Ctx& c, const ModelDev& m, const ChainDev& ch
         } else {
             for (int t = nt + m.nwrap; t > nt; --t) {
                 const double* phi_t = ch.phi + (size_t)(t - 1) * m.ns * 2;
-                gemm_rc(c, ch.w1, m.ez[2], ch.ul, 1, n);          // UL <- UL e^{-dtau K}
+                gemm_rc_panel(c, ch.w1, m.ez[2], ch.ul, 1, n, L.panel, L.panel_cap);   // UL <- UL e^{-dtau K}
                 mat_copy(c, ch.ul, ch.w1, n, false);
                 for (int nf = 2; nf >= 1; --nf) {
                     if (!m.active[nf - 1]) continue;
@@ -2203,7 +2212,7 @@ DEV void measure_checkpoints_team(const Ctx& c, const ModelDev& m, const ChainDe
         const int nt = st * m.nwrap;
         for (int t = nt + m.nwrap; t > nt; --t) {
             const double* phi_t = ch.phi + (size_t)(t - 1) * m.ns * 2;
-            gemm_rc(c, ch.w1, m.ez[2], ch.ul, 1, n, sh);          // UL <- UL e^{-dtau K}
+            gemm_rc_panel(c, ch.w1, m.ez[2], ch.ul, 1, n, L.panel, L.panel_cap, sh);   // UL <- UL e^{-dtau K}
             team_sync(c, team);
             mat_copy(c, ch.ul, ch.w1, n, false, sh);
             for (int nf = 2; nf >= 1; --nf) {
