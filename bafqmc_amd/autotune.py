@@ -15,12 +15,18 @@ src/number_conserving/src/stabilization.f90:256-266) stays below `drift_tol`
 Changing Nwrap changes results at the level of that drift only.
 
 Timings are device times of the sweep launch (HIP events).  Results can be
-kept in a JSON cache keyed by model, sample size and device.
+kept in a JSON cache keyed by model, sample size and device.  A sample that
+is not the whole batch is timed with one workgroup per chain (BAFQ_PAIRS=0):
+the library pairs the workgroups of device-memory chains only when the whole
+batch leaves CUs idle, which a small sample of a large or mixed batch would
+and the batch would not.
 """
 from __future__ import annotations
 
+import contextlib
 import dataclasses
 import json
+import os
 from pathlib import Path
 
 import numpy as np
@@ -40,9 +46,27 @@ class Choice:
     table: dict          # "(path, delayed, nwrap)" -> ms per sweep (None: rejected)
 
 
-def _time(lib, p, schedule, chains, sweeps, seed):
+@contextlib.contextmanager
+def _unpaired(active):
+    """BAFQ_PAIRS=0 while a handle is created (bafq_create reads it)."""
+    if not active:
+        yield
+        return
+    old = os.environ.get("BAFQ_PAIRS")
+    os.environ["BAFQ_PAIRS"] = "0"
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["BAFQ_PAIRS"]
+        else:
+            os.environ["BAFQ_PAIRS"] = old
+
+
+def _time(lib, p, schedule, chains, sweeps, seed, whole=True):
     phis, seeds = conf_set_batch(p, chain_seeds(seed, chains))
-    s = BatchedSolver([ModelSpec(p)], [0] * chains, lib=lib, schedule=[schedule])
+    with _unpaired(not whole):
+        s = BatchedSolver([ModelSpec(p)], [0] * chains, lib=lib, schedule=[schedule])
     try:
         s.set_fields(list(phis), seeds)
         if p.nwarm:
@@ -66,13 +90,14 @@ def _nwrap_candidates(p):
     return sorted(set(near) | {p.nwrap})
 
 
-def tune_model(lib, params, chains=64, sweeps=2, tune_nwrap=False, drift_tol=1e-7, seed=97):
-    """Fastest (schedule, Nwrap) for one model; params is a bafqmc_amd.model.Params."""
+def tune_model(lib, params, chains=64, sweeps=2, tune_nwrap=False, drift_tol=1e-7, seed=97, whole=True):
+    """Fastest (schedule, Nwrap) for one model; params is a bafqmc_amd.model.Params.
+    whole: the chains timed are the whole batch (else timed unpaired)."""
     table = {}
     best = None
     for sched in SCHEDULES:
         try:
-            ms, drift, eff = _time(lib, params, sched, chains, sweeps, seed)
+            ms, drift, eff = _time(lib, params, sched, chains, sweeps, seed, whole)
         except SolverError:
             table[str((*sched, params.nwrap))] = None
             continue
@@ -88,7 +113,7 @@ def tune_model(lib, params, chains=64, sweeps=2, tune_nwrap=False, drift_tol=1e-
             if nw == params.nwrap:
                 continue
             q = dataclasses.replace(params, nwrap=nw)
-            ms, drift, _ = _time(lib, q, best[1], chains, sweeps, seed)
+            ms, drift, _ = _time(lib, q, best[1], chains, sweeps, seed, whole)
             ok = drift < drift_tol
             table[str((*best[1], nw))] = ms if ok else None
             if ok and ms < best[0]:
@@ -115,12 +140,13 @@ def tune(lib, models, chain_model, sweeps=2, tune_nwrap=False, drift_tol=1e-7, c
     for k, m in enumerate(models):
         p = m.params if isinstance(m, ModelSpec) else m
         n = int(max(1, min(counts[k], max_sample)))
-        key = _key(p, n, device) + (":nwrap" if tune_nwrap else "")
+        whole = len(models) == 1 and n == counts[k]
+        key = _key(p, n, device) + (":nwrap" if tune_nwrap else "") + ("" if whole else ":sample")
         if key in stored:
             c = stored[key]
             out.append(Choice(tuple(c["schedule"]), c["nwrap"], c["ms_per_sweep"], c["table"]))
             continue
-        ch = tune_model(lib, p, n, sweeps, tune_nwrap, drift_tol)
+        ch = tune_model(lib, p, n, sweeps, tune_nwrap, drift_tol, whole=whole)
         stored[key] = dataclasses.asdict(ch)
         out.append(ch)
     if cache:
