@@ -149,10 +149,9 @@ struct Lds {
     cd* finv;          // n
     cd* col;           // 2 slots x 2 sectors x n
     cd* row;           // 2 slots x 2 sectors x n
-    // the device-memory path's kinetic products stage X here (gemm_rc_panel)
-    // over col, row, red, site, prop and the delayed-update buffers to the end
-    // of the layout (panel_cap complex, at least 18 n + 256 >= 16 (n + 3)),
-    // none of which is live during those products
+    // the device-memory path's kinetic products stage X here (gemm_rc_panel):
+    // the whole layout (panel_cap complex), none of which is live during those
+    // products; with G in LDS only col .. prop (18 n + 256 >= 16 (n + 3))
     cd* panel;
     int panel_cap;
     double* red;       // nwaves x 64
@@ -224,7 +223,10 @@ DEV Lds carve(char* base, int n, int nwaves, bool with_mat, int dk = 0, int sb =
     L.sg0 = sb > 0 ? (cd*)take((size_t)sb * sb * sizeof(cd)) : nullptr;
     L.sloc = sb > 0 ? (int*)take((size_t)sb * sizeof(int)) : nullptr;
     L.sd = sb > 0 ? (cd*)take((size_t)sb * sizeof(cd)) : nullptr;
-    if (!with_mat) L.panel_cap = (int)((p - (char*)L.panel) / sizeof(cd));   // through the end
+    if (!with_mat) {                             // the whole layout
+        L.panel = (cd*)base;
+        L.panel_cap = (int)((p - base) / sizeof(cd));
+    }
     return L;
 }
 
