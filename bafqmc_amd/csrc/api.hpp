@@ -123,6 +123,7 @@ struct bafq_handle {
     // words (2 per pair) and mailboxes (1 per pair)
     int npairs = 0;
     int* d_pairs = nullptr;
+    double pair_wait_s = 20.0;                // longest wait at a pair barrier (BAFQ_PAIR_WAIT_MS)
     std::vector<bafq_schedule> schedule;      // in effect, per model
     void* launches = nullptr;                 // back-end launch cache
     // learned global moves: old fields, forward drift, per-chain sums, flags, subset ids
@@ -449,8 +450,10 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
     // Two workgroups per chain on the device-memory path when every chain then
     // still has its pair resident at once (or BAFQ_PAIRS=1: as many pairs as
     // fit), the batch has no LDS-path chains (whose kernel would share the
-    // CUs) and every model uses submatrix updates.  BAFQ_PAIRS=0 turns it off.
+    // CUs) and every model uses submatrix updates.  BAFQ_PAIRS=0 turns it off;
+    // BAFQ_PAIR_WAIT_MS sets the longest wait at a pair barrier (20000).
     {
+        if (const char* w = std::getenv("BAFQ_PAIR_WAIT_MS")) h->pair_wait_s = std::max(0.0, std::atof(w) / 1000.0);
         const char* env = std::getenv("BAFQ_PAIRS");
         const int mode = env ? std::atoi(env) : -1;
         bool ok = mode != 0 && h->path_count[P_GLOBAL] > 0 && h->path_count[P_LDS] == 0;
@@ -571,7 +574,7 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure,
                 l.a.npairs = h->npairs;
                 l.a.pair_flags = h->d_pairs;
                 l.a.pair_mail = h->d_pairs + 2 * h->npairs;
-                l.a.pair_wait = (uint64_t)(20.0 * rt_clock_hz());    // 20 s
+                l.a.pair_wait = (uint64_t)(h->pair_wait_s * rt_clock_hz());
                 l.grid = 2 * h->npairs;
             }
         }
@@ -581,8 +584,9 @@ static int launch_all(bafq_handle* h, bafq::KernelId k, int nsweep, int measure,
             if (rt_d2h(fl.data(), h->d_pairs, sizeof(int) * fl.size())) return fail(h, rt_error());
             for (int i = 0; i < h->npairs; ++i)
                 if (fl[2 * i + 1])
-                    return fail(h, "a workgroup waited more than 20 s for its partner (the pair was not resident "
-                                   "at once); the sweep's results are invalid, set BAFQ_PAIRS=0");
+                    return fail(h, "a workgroup waited longer than BAFQ_PAIR_WAIT_MS (default 20 s) for its partner "
+                                   "(the pair was not resident at once); the launch's results are invalid, set "
+                                   "BAFQ_PAIRS=0");
         }
     }
     h->last_ms = total_ms;

@@ -87,6 +87,25 @@ def test_emu_paired_workgroups_split_stabilisation(emu_lib, monkeypatch):
     check_pairs_bitwise(emu_lib, d, 16, 2, monkeypatch)
 
 
+def test_emu_pair_wait_timeout_is_an_error(emu_lib, monkeypatch):
+    """A pair barrier's bounded wait: with a zero limit the first wait that is
+    not met at once sets the pair's flag, both members stop waiting, the kernel
+    ends and the sweep returns an error instead of hanging."""
+    from bafqmc_amd.solver import SolverError
+    monkeypatch.setenv("BAFQ_PAIRS", "1")
+    monkeypatch.setenv("BAFQ_PAIR_WAIT_MS", "0")
+    d = CASES[5]
+    s = BatchedSolver([ModelSpec(solver_params(d))], [0], lib=emu_lib, schedule=[(0, 32)])
+    try:
+        assert s.pairs() == 1
+        s.set_fields([fixed_fields(d["ltrot"], d["nlx"] * d["nly"])], [4242])
+        with pytest.raises(SolverError, match="BAFQ_PAIR_WAIT_MS"):
+            s.prepare()
+            s.sweep(2)
+    finally:
+        s.close()
+
+
 def test_wide_updates_rejected_on_the_lds_path(emu_lib):
     with pytest.raises(RuntimeError, match="above 16 need the device-memory path"):
         BatchedSolver([ModelSpec(solver_params(CASES[4]))], [0], lib=emu_lib, schedule=(1, 32))
