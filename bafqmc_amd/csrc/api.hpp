@@ -449,22 +449,36 @@ int bafq_create_scheduled(const bafq_model_desc* models, int n_models, const int
     }
     // Two workgroups per chain on the device-memory path when every chain then
     // still has its pair resident at once (or BAFQ_PAIRS=1: as many pairs as
-    // fit), the batch has no LDS-path chains (whose kernel would share the
-    // CUs) and every model uses submatrix updates.  BAFQ_PAIRS=0 turns it off;
-    // BAFQ_PAIR_WAIT_MS sets the longest wait at a pair barrier (20000).
+    // fit) and every model on that path uses submatrix updates.  LDS-path
+    // chains of the batch run in a kernel beside the paired ones: there the
+    // two kernels must have the same resident capacity, and the LDS-path grid
+    // is cut to what the pairs leave free, so every workgroup of both fits at
+    // once.  BAFQ_PAIRS=0 turns it off; BAFQ_PAIR_WAIT_MS sets the longest
+    // wait at a pair barrier (20000).
     {
         if (const char* w = std::getenv("BAFQ_PAIR_WAIT_MS")) h->pair_wait_s = std::max(0.0, std::atof(w) / 1000.0);
         const char* env = std::getenv("BAFQ_PAIRS");
         const int mode = env ? std::atoi(env) : -1;
-        bool ok = mode != 0 && h->path_count[P_GLOBAL] > 0 && h->path_count[P_LDS] == 0;
-        for (int c = 0; ok && c < n_chains; ++c) ok = h->models[chain_model[c]].dk > 0;
+        const bool mixed = h->path_count[P_LDS] > 0;
+        bool ok = mode != 0 && h->path_count[P_GLOBAL] > 0;
+        for (int c = 0; ok && c < n_chains; ++c)
+            if (h->schedule[chain_model[c]].path == P_GLOBAL) ok = h->models[chain_model[c]].dk > 0;
         if (ok) {
             // the paired kernels: sweep, rebuild, measurement pass
-            int res = rt_resident_groups(K_SWEEP, P_GLOBAL, rt_block_threads(), h->max_lds[P_GLOBAL]);
-            res = std::min(res, rt_resident_groups(K_PREPARE, P_GLOBAL, rt_block_threads(), h->max_lds[P_GLOBAL]));
-            res = std::min(res, rt_resident_groups(K_MEASURE, P_GLOBAL, rt_block_threads(), h->max_lds[P_GLOBAL]));
+            auto res_of = [&](PathId p) {
+                const int block = p == P_LDS ? rt_lds_block_threads() : rt_block_threads();
+                int r = rt_resident_groups(K_SWEEP, p, block, h->max_lds[p]);
+                r = std::min(r, rt_resident_groups(K_PREPARE, p, block, h->max_lds[p]));
+                return std::min(r, rt_resident_groups(K_MEASURE, p, block, h->max_lds[p]));
+            };
+            const int res = res_of(P_GLOBAL);
             const int cnt = h->path_count[P_GLOBAL];
-            if (res >= 2 && (2 * cnt <= res || mode == 1)) h->npairs = std::min(cnt, res / 2);
+            if (!mixed) {
+                if (res >= 2 && (2 * cnt <= res || mode == 1)) h->npairs = std::min(cnt, res / 2);
+            } else if (res_of(P_LDS) == res && 2 * cnt < res) {
+                h->npairs = cnt;
+                for (int k = 0; k < 2; ++k) h->grid[k][P_LDS] = std::min(h->grid[k][P_LDS], res - 2 * cnt);
+            }
         }
         if (h->npairs > 0 && rt_malloc((void**)&h->d_pairs, sizeof(int) * 3 * (size_t)h->npairs)) {
             *out = h;

@@ -77,11 +77,13 @@ int bafq_get_schedule(const bafq_handle* h, int model, bafq_schedule* out);
  * of bafq_sweep, bafq_prepare and bafq_measure_pass (0: one workgroup per
  * chain).  No counterpart in the reference (one chain per MPI rank).
  * bafq_create pairs the workgroups when every device-memory-path chain then
- * still has its own resident pair, the batch has no LDS-path chains and
- * every model uses submatrix updates (delayed > 0); the environment variable
- * BAFQ_PAIRS=0 turns this off, BAFQ_PAIRS=1 pairs as many workgroups as are
- * resident at once.  Results are identical to the unpaired kernels', bit for
- * bit. */
+ * still has its own resident pair and every model on that path uses
+ * submatrix updates (delayed > 0); with LDS-path chains in the batch, when
+ * both paths' kernels hold the same workgroups per CU and the pairs leave at
+ * least one workgroup for the LDS path, whose grid is then cut to fit beside
+ * them.  The environment variable BAFQ_PAIRS=0 turns this off, BAFQ_PAIRS=1
+ * pairs as many workgroups as are resident at once (batches without LDS-path
+ * chains).  Results are identical to the unpaired kernels', bit for bit. */
 int bafq_pairs(const bafq_handle* h);
 
 /* Sizes: matrix dimension N of a chain, doubles per chain in bafq_read_bin's
